@@ -1,0 +1,603 @@
+"""cupbr_amd — MI355X renderer core behind cupbr's interface.
+
+Python front end over the C ABI in ``include/cupbr.h`` (``lib/libcupbr.so``,
+built by ``python -m cupbr_amd.build``). Class and method names follow the
+reference's C++ interface:
+
+    SceneLoader.load_from_file / destroy_scene  lib/include/Scene/SceneLoader.h:15-21
+    PBRenderer                                   lib/include/Renderer/PBRenderer.h:27-116
+    PostProcessor                                lib/include/PostProcessing/PostProcessor.h:12-52
+    ToneMapper                                   lib/include/Renderer/ToneMapper.h:22-80
+    Camera                                       lib/include/DataStructure/Camera.h:14-86
+
+Every call goes to the HIP library; there is no CPU fallback. A missing
+library or a missing GPU raises ``CupbrError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+
+import numpy as np
+
+__all__ = [
+    "CupbrError", "RenderingMethod", "ToneMappingType", "GeometryType", "MaterialType", "LightType", "Image",
+    "Camera", "Scene", "SceneLoader", "PBRenderer", "PostProcessor", "ToneMapper", "threshold_curve", "lib",
+    "device_name", "OBJECT_DTYPE", "LIGHT_DTYPE",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libcupbr.so")
+
+
+class CupbrError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[cupbr {code}] {message}")
+        self.code = code
+
+
+class RenderingMethod(enum.IntEnum):  # PBRenderer.h:14-22
+    RAYTRACER = 0
+    WHITTED = 1
+    PATHTRACER = 2
+    METROPOLIS = 3
+    GRADIENTDOMAIN = 4
+    VOLUME = 5
+
+
+class ToneMappingType(enum.IntEnum):  # ToneMapper.h:13-17
+    REINHARD = 0
+    GAMMA = 1
+
+
+class GeometryType(enum.IntEnum):  # Geometry.h:15-23
+    SPHERE = 0
+    PLANE = 1
+    QUAD = 2
+    TRIANGLE = 3
+    MESH = 4
+
+
+class MaterialType(enum.IntEnum):  # Material.h:14-22
+    LAMBERT = 0
+    PHONG = 1
+    MIRROR = 2
+    GLASS = 3
+    GGX = 4
+    VOLUME = 5
+
+
+class LightType(enum.IntEnum):  # Light.h:11-15
+    POINT = 0
+    AREA = 1
+
+
+class Image(enum.IntEnum):  # cupbr_image
+    OUTPUT = 0
+    GRADIENT_X = 1
+    GRADIENT_Y = 2
+    BASE = 3
+    GRADIENT_X_FORWARD = 4
+    GRADIENT_X_BACKWARD = 5
+    GRADIENT_Y_FORWARD = 6
+    GRADIENT_Y_BACKWARD = 7
+
+
+# ------------------------------------------------------------------ C structs
+class _Volume(C.Structure):
+    _fields_ = [("sigma_s", C.c_float * 3), ("sigma_a", C.c_float * 3), ("g", C.c_float), ("interface_type", C.c_int32)]
+
+
+class _Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("x_axis", C.c_float * 3), ("y_axis", C.c_float * 3),
+                ("z_axis", C.c_float * 3), ("moved", C.c_int32)]
+
+
+class _BvhStats(C.Structure):
+    _fields_ = [("nodes", C.c_uint32), ("leaves", C.c_uint32), ("primitives", C.c_uint32), ("unbounded", C.c_uint32),
+                ("depth", C.c_uint32), ("stack_entries", C.c_uint32), ("build_ms", C.c_double),
+                ("upload_ms", C.c_double), ("sah_cost", C.c_double)]
+
+
+class _RenderStats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("milliseconds", C.c_double)]
+
+
+# cupbr_object / cupbr_light as numpy records (144 and 64 bytes, no padding)
+OBJECT_DTYPE = np.dtype([
+    ("type", "<i4"), ("params", "<f4", 12), ("first_triangle", "<u4"), ("triangle_count", "<u4"),
+    ("albedo_e", "<f4", 3), ("albedo_d", "<f4", 3), ("albedo_s", "<f4", 3), ("shininess", "<f4"),
+    ("roughness", "<f4"), ("eta", "<f4"), ("sigma_s", "<f4", 3), ("sigma_a", "<f4", 3), ("g", "<f4"),
+    ("interface_type", "<i4"), ("material_type", "<i4")])
+LIGHT_DTYPE = np.dtype([("type", "<i4"), ("position", "<f4", 3), ("intensity", "<f4", 3), ("radiance", "<f4", 3),
+                        ("half_extend1", "<f4", 3), ("half_extend2", "<f4", 3)])
+assert OBJECT_DTYPE.itemsize == 144 and LIGHT_DTYPE.itemsize == 64
+
+_lib = None
+
+
+def lib():
+    """The loaded libcupbr.so (raises CupbrError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CupbrError(-100, f"{LIB_PATH} is missing: build it with `python -m cupbr_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        P, U32, I32, SZ = C.c_void_p, C.c_uint32, C.c_int32, C.c_size_t
+        sig = {
+            "cupbr_last_error": (C.c_char_p, []),
+            "cupbr_device_name": (C.c_int, [C.c_char_p, SZ]),
+            "cupbr_camera_init": (C.c_int, [P, U32, U32]),
+            "cupbr_camera_look_at": (C.c_int, [P, P, P, P, C.c_float, U32, U32]),
+            "cupbr_scene_load_xml": (C.c_int, [C.c_char_p, P]),
+            "cupbr_scene_create": (C.c_int, [P, U32, P, U32, P, U32, P, U32, U32, P, P]),
+            "cupbr_scene_destroy": (C.c_int, [P]),
+            "cupbr_scene_upload": (C.c_int, [P]),
+            "cupbr_scene_get_counts": (C.c_int, [P, P, P, P, P]),
+            "cupbr_scene_get_objects": (C.c_int, [P, P, U32]),
+            "cupbr_scene_get_triangles": (C.c_int, [P, P, U32]),
+            "cupbr_scene_get_lights": (C.c_int, [P, P, U32]),
+            "cupbr_scene_get_volume": (C.c_int, [P, P]),
+            "cupbr_scene_get_bvh_stats": (C.c_int, [P, P]),
+            "cupbr_scene_get_reference_order": (C.c_int, [P, P, U32]),
+            "cupbr_trace_closest": (C.c_int, [P, P, SZ, P, P, P]),
+            "cupbr_trace_closest_device": (C.c_int, [P, P, SZ, P, P, P, P]),
+            "cupbr_trace_visibility": (C.c_int, [P, P, P, SZ, P]),
+            "cupbr_trace_visibility_device": (C.c_int, [P, P, P, SZ, P, P]),
+            "cupbr_renderer_create": (C.c_int, [I32, P]),
+            "cupbr_renderer_destroy": (C.c_int, [P]),
+            "cupbr_renderer_set_output_size": (C.c_int, [P, U32, U32]),
+            "cupbr_renderer_register_scene": (C.c_int, [P, P]),
+            "cupbr_renderer_set_method": (C.c_int, [P, I32]),
+            "cupbr_renderer_get_method": (C.c_int, [P, P]),
+            "cupbr_renderer_reset": (C.c_int, [P]),
+            "cupbr_renderer_set_russian_roulette": (C.c_int, [P, I32]),
+            "cupbr_renderer_set_max_trace_depth": (C.c_int, [P, U32]),
+            "cupbr_renderer_get_max_trace_depth": (C.c_int, [P, P]),
+            "cupbr_renderer_get_frame_index": (C.c_int, [P, P]),
+            "cupbr_render": (C.c_int, [P, P, U32]),
+            "cupbr_renderer_get_image": (C.c_int, [P, I32, P, SZ]),
+            "cupbr_renderer_get_image_device": (C.c_int, [P, I32, P]),
+            "cupbr_renderer_get_stats": (C.c_int, [P, P]),
+            "cupbr_renderer_synchronize": (C.c_int, [P]),
+            "cupbr_post_create": (C.c_int, [P]),
+            "cupbr_post_destroy": (C.c_int, [P]),
+            "cupbr_post_register_image": (C.c_int, [P, P, U32, U32]),
+            "cupbr_post_bloom": (C.c_int, [P, P]),
+            "cupbr_post_get_buffer_device": (C.c_int, [P, P]),
+            "cupbr_post_get_buffer": (C.c_int, [P, P, SZ]),
+            "cupbr_threshold_curve": (C.c_int, [C.c_float, C.c_float, P]),
+            "cupbr_tonemapper_create": (C.c_int, [I32, P]),
+            "cupbr_tonemapper_destroy": (C.c_int, [P]),
+            "cupbr_tonemapper_register_image": (C.c_int, [P, P, U32, U32]),
+            "cupbr_tonemapper_tonemap": (C.c_int, [P]),
+            "cupbr_tonemapper_tonemap_bloom": (C.c_int, [P, P, P]),
+            "cupbr_tonemapper_get_render_buffer": (C.c_int, [P, P, SZ]),
+            "cupbr_tonemapper_get_render_buffer_device": (C.c_int, [P, P]),
+            "cupbr_tonemapper_set_type": (C.c_int, [P, I32]),
+            "cupbr_tonemapper_get_type": (C.c_int, [P, P]),
+            "cupbr_tonemapper_set_exposure": (C.c_int, [P, C.c_float]),
+            "cupbr_tonemapper_save_to_file": (C.c_int, [P, C.c_char_p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise CupbrError(rc, lib().cupbr_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    _check(lib().cupbr_device_name(buf, 256))
+    return buf.value.decode()
+
+
+def threshold_curve(threshold: float = 1.0, knee: float = 1.0) -> np.ndarray:
+    """Interactor::Impl::compute_threshold (lib/src/Interaction/Interactor.cpp:436-440)."""
+    out = (C.c_float * 4)()
+    _check(lib().cupbr_threshold_curve(threshold, knee, out))
+    return np.array(out[:], np.float32)
+
+
+# ------------------------------------------------------------------ Camera
+class Camera:
+    """Pinhole camera: position plus image-plane axes (Camera.h:72-76)."""
+
+    def __init__(self, width: int | None = None, height: int | None = None):
+        self._c = _Camera()
+        if width is not None:
+            _check(lib().cupbr_camera_init(C.byref(self._c), width, height))
+        else:  # Camera() = default (Camera.h:72-76)
+            self._c.x_axis[:] = [1, 0, 0]
+            self._c.y_axis[:] = [0, 1, 0]
+            self._c.z_axis[:] = [0, 0, 1]
+
+    @classmethod
+    def look_at(cls, eye, target, up, vfov_degrees: float, width: int, height: int) -> "Camera":
+        cam = cls()
+        f = lambda v: (C.c_float * 3)(*[float(x) for x in v])  # noqa: E731
+        _check(lib().cupbr_camera_look_at(C.byref(cam._c), f(eye), f(target), f(up), vfov_degrees, width, height))
+        return cam
+
+    def _get(self, k):
+        return np.array(getattr(self._c, k)[:], np.float32)
+
+    position = property(lambda s: s._get("position"))
+    x_axis = property(lambda s: s._get("x_axis"))
+    y_axis = property(lambda s: s._get("y_axis"))
+    z_axis = property(lambda s: s._get("z_axis"))
+
+    def moved(self) -> bool:
+        return bool(self._c.moved)
+
+    def set_moved(self, moved: bool):
+        self._c.moved = int(moved)
+
+    def as_dict(self) -> dict:
+        return {k: self._get(k) for k in ("position", "x_axis", "y_axis", "z_axis")}
+
+
+# ------------------------------------------------------------------ Scene
+def _desc_to_arrays(desc: dict):
+    types = np.asarray(desc["types"], np.int32)
+    n = len(types)
+    objs = np.zeros(n, OBJECT_DTYPE)
+    objs["type"] = types
+    objs["params"] = np.asarray(desc["params"], np.float32).reshape(n, 12)
+    objs["first_triangle"] = np.asarray(desc["first"], np.uint32)
+    objs["triangle_count"] = np.asarray(desc["count"], np.uint32)
+    m = np.asarray(desc["mats"], np.float32).reshape(n, 21)
+    objs["albedo_e"], objs["albedo_d"], objs["albedo_s"] = m[:, 0:3], m[:, 3:6], m[:, 6:9]
+    objs["shininess"], objs["roughness"], objs["eta"] = m[:, 9], m[:, 10], m[:, 11]
+    objs["sigma_s"], objs["sigma_a"], objs["g"] = m[:, 12:15], m[:, 15:18], m[:, 18]
+    objs["interface_type"] = m[:, 19].astype(np.int32)
+    objs["material_type"] = m[:, 20].astype(np.int32)
+    lt = np.asarray(desc["lights"], np.float32).reshape(-1, 16)
+    lights = np.zeros(len(lt), LIGHT_DTYPE)
+    lights["type"] = lt[:, 0].astype(np.int32)
+    lights["position"], lights["intensity"], lights["radiance"] = lt[:, 1:4], lt[:, 4:7], lt[:, 7:10]
+    lights["half_extend1"], lights["half_extend2"] = lt[:, 10:13], lt[:, 13:16]
+    tris = np.ascontiguousarray(np.asarray(desc["tris"], np.float32).reshape(-1, 9))
+    env = desc.get("env")
+    env = None if env is None else np.ascontiguousarray(np.asarray(env, np.float32))
+    vol = _Volume()
+    v = np.asarray(desc.get("volume", [1, 1, 1, 1, 1, 1, 0]), np.float32)
+    vol.sigma_s[:] = v[0:3].tolist()
+    vol.sigma_a[:] = v[3:6].tolist()
+    vol.g = float(v[6])
+    return objs, tris, lights, env, vol
+
+
+class Scene:
+    """Scene handle (the reference's Scene POD, lib/include/Scene/Scene.h:15-34)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def from_desc(cls, desc: dict) -> "Scene":
+        """Builds a scene from the flat description (see cupbr_amd.scenes)."""
+        objs, tris, lights, env, vol = _desc_to_arrays(desc)
+        h = C.c_void_p()
+        eh, ew = (0, 0) if env is None else env.shape[:2]
+        _check(lib().cupbr_scene_create(_ptr(objs), len(objs), _ptr(tris), len(tris), _ptr(lights), len(lights),
+                                        _ptr(env), ew, eh, C.byref(vol), C.byref(h)))
+        return cls(h)
+
+    def destroy(self):
+        if self._h:
+            _check(lib().cupbr_scene_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise CupbrError(-1, "scene was destroyed")
+        return self._h
+
+    def upload(self):
+        """Copies the scene to the GPU now (otherwise the first trace/render does it)."""
+        _check(lib().cupbr_scene_upload(self.handle))
+
+    def counts(self):
+        n, t, l, e = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_int32()
+        _check(lib().cupbr_scene_get_counts(self.handle, C.byref(n), C.byref(t), C.byref(l), C.byref(e)))
+        return {"objects": n.value, "triangles": t.value, "lights": l.value, "environment": bool(e.value)}
+
+    def objects(self) -> np.ndarray:
+        n = self.counts()["objects"]
+        out = np.zeros(n, OBJECT_DTYPE)
+        _check(lib().cupbr_scene_get_objects(self.handle, _ptr(out), n))
+        return out
+
+    def triangles(self) -> np.ndarray:
+        n = self.counts()["triangles"]
+        out = np.zeros((n, 9), np.float32)
+        _check(lib().cupbr_scene_get_triangles(self.handle, _ptr(out), n))
+        return out
+
+    def lights(self) -> np.ndarray:
+        n = self.counts()["lights"]
+        out = np.zeros(n, LIGHT_DTYPE)
+        _check(lib().cupbr_scene_get_lights(self.handle, _ptr(out), n))
+        return out
+
+    def volume(self) -> np.ndarray:
+        v = _Volume()
+        _check(lib().cupbr_scene_get_volume(self.handle, C.byref(v)))
+        return np.array(list(v.sigma_s) + list(v.sigma_a) + [v.g], np.float32)
+
+    def bvh_stats(self) -> dict:
+        s = _BvhStats()
+        _check(lib().cupbr_scene_get_bvh_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _BvhStats._fields_}
+
+    def reference_order(self) -> np.ndarray:
+        n = self.counts()["objects"]
+        out = np.zeros(max(n, 1), np.int32)
+        _check(lib().cupbr_scene_get_reference_order(self.handle, _ptr(out), max(n, 1)))
+        return out[:n]
+
+    def trace_closest(self, rays):
+        """Closest hit for n x 6 rays (origin, normalised direction): (object_id, triangle_id, t).
+
+        numpy input goes through host memory; a torch tensor on the GPU stays on the device."""
+        if _is_torch_gpu(rays):
+            import torch
+            rays = rays.contiguous().float()
+            n = rays.numel() // 6
+            obj = torch.empty(n, dtype=torch.int32, device=rays.device)
+            tri = torch.empty_like(obj)
+            t = torch.empty(n, dtype=torch.float32, device=rays.device)
+            stream = torch.cuda.current_stream(rays.device).cuda_stream
+            _check(lib().cupbr_trace_closest_device(self.handle, rays.data_ptr(), n, obj.data_ptr(), tri.data_ptr(),
+                                                    t.data_ptr(), stream))
+            return obj, tri, t
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = len(rays)
+        obj = np.zeros(n, np.int32)
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        _check(lib().cupbr_trace_closest(self.handle, _ptr(rays), n, _ptr(obj), _ptr(tri), _ptr(t)))
+        return obj, tri, t
+
+    def trace_visibility(self, rays, distance):
+        """Tracing::traceVisibility for n rays: True where nothing is hit within distance."""
+        if _is_torch_gpu(rays):
+            import torch
+            rays = rays.contiguous().float()
+            n = rays.numel() // 6
+            dist = distance.contiguous().float()
+            vis = torch.empty(n, dtype=torch.uint8, device=rays.device)
+            stream = torch.cuda.current_stream(rays.device).cuda_stream
+            _check(lib().cupbr_trace_visibility_device(self.handle, rays.data_ptr(), dist.data_ptr(), n,
+                                                       vis.data_ptr(), stream))
+            return vis.bool()
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        dist = np.ascontiguousarray(distance, np.float32).reshape(-1)
+        vis = np.zeros(len(rays), np.uint8)
+        _check(lib().cupbr_trace_visibility(self.handle, _ptr(rays), _ptr(dist), len(rays), _ptr(vis)))
+        return vis.astype(bool)
+
+
+def _is_torch_gpu(x) -> bool:
+    return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
+
+
+class SceneLoader:
+    """SceneLoader (lib/include/Scene/SceneLoader.h:15-21)."""
+
+    @staticmethod
+    def load_from_file(path: str) -> Scene:
+        h = C.c_void_p()
+        _check(lib().cupbr_scene_load_xml(path.encode(), C.byref(h)))
+        return Scene(h)
+
+    @staticmethod
+    def destroy_scene(scene: Scene):
+        scene.destroy()
+
+
+# ------------------------------------------------------------------ PBRenderer
+class PBRenderer:
+    """PBRenderer (lib/include/Renderer/PBRenderer.h:27-116)."""
+
+    def __init__(self, method: RenderingMethod = RenderingMethod.RAYTRACER):
+        h = C.c_void_p()
+        _check(lib().cupbr_renderer_create(int(method), C.byref(h)))
+        self._h = h
+        self._size = (0, 0)
+        self._scene = None
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().cupbr_renderer_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def set_output_size(self, width: int, height: int):
+        _check(lib().cupbr_renderer_set_output_size(self._h, width, height))
+        self._size = (width, height)
+
+    def register_scene(self, scene: Scene):
+        _check(lib().cupbr_renderer_register_scene(self._h, scene.handle))
+        self._scene = scene
+
+    def set_method(self, method: RenderingMethod):
+        _check(lib().cupbr_renderer_set_method(self._h, int(method)))
+
+    def get_method(self) -> RenderingMethod:
+        m = C.c_int32()
+        _check(lib().cupbr_renderer_get_method(self._h, C.byref(m)))
+        return RenderingMethod(m.value)
+
+    def reset(self):
+        _check(lib().cupbr_renderer_reset(self._h))
+
+    def set_russian_roulette(self, enable: bool):
+        _check(lib().cupbr_renderer_set_russian_roulette(self._h, int(bool(enable))))
+
+    def set_max_trace_depth(self, depth: int):
+        _check(lib().cupbr_renderer_set_max_trace_depth(self._h, depth))
+
+    def get_max_trace_depth(self) -> int:
+        d = C.c_uint32()
+        _check(lib().cupbr_renderer_get_max_trace_depth(self._h, C.byref(d)))
+        return d.value
+
+    def frame_index(self) -> int:
+        f = C.c_uint32()
+        _check(lib().cupbr_renderer_get_frame_index(self._h, C.byref(f)))
+        return f.value
+
+    def render(self, camera: Camera, spp: int = 1):
+        """render(Camera*) repeated spp times (asynchronous on the renderer's stream)."""
+        _check(lib().cupbr_render(self._h, C.byref(camera._c), spp))
+
+    def synchronize(self):
+        _check(lib().cupbr_renderer_synchronize(self._h))
+
+    def get_image(self, which: Image = Image.OUTPUT) -> np.ndarray:
+        w, h = self._size
+        out = np.zeros((h, w, 3), np.float32)
+        _check(lib().cupbr_renderer_get_image(self._h, int(which), _ptr(out), out.size))
+        return out
+
+    def get_output_image(self) -> np.ndarray:
+        return self.get_image(Image.OUTPUT)
+
+    def get_gradient_x(self) -> np.ndarray:
+        return self.get_image(Image.GRADIENT_X)
+
+    def get_gradient_y(self) -> np.ndarray:
+        return self.get_image(Image.GRADIENT_Y)
+
+    def image_device_ptr(self, which: Image = Image.OUTPUT) -> int:
+        p = C.c_void_p()
+        _check(lib().cupbr_renderer_get_image_device(self._h, int(which), C.byref(p)))
+        return p.value
+
+    @property
+    def size(self):
+        return self._size
+
+    def stats(self) -> dict:
+        s = _RenderStats()
+        _check(lib().cupbr_renderer_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _RenderStats._fields_}
+
+
+# ------------------------------------------------------------------ post chain
+class PostProcessor:
+    """PostProcessor (lib/include/PostProcessing/PostProcessor.h:12-52)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().cupbr_post_create(C.byref(h)))
+        self._h = h
+        self._size = (0, 0)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().cupbr_post_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def register_image(self, device_ptr: int, width: int, height: int):
+        _check(lib().cupbr_post_register_image(self._h, C.c_void_p(device_ptr), width, height))
+        self._size = (width, height)
+
+    def register_renderer(self, renderer: PBRenderer):
+        """registerImage(pbrenderer.getOutputImage())"""
+        w, h = renderer.size
+        self.register_image(renderer.image_device_ptr(Image.OUTPUT), w, h)
+
+    def bloom(self, threshold):
+        c = (C.c_float * 4)(*[float(x) for x in threshold])
+        _check(lib().cupbr_post_bloom(self._h, c))
+
+    def get_post_process_buffer(self) -> np.ndarray:
+        w, h = self._size
+        out = np.zeros((h, w, 3), np.float32)
+        _check(lib().cupbr_post_get_buffer(self._h, _ptr(out), out.size))
+        return out
+
+    def buffer_device_ptr(self) -> int:
+        p = C.c_void_p()
+        _check(lib().cupbr_post_get_buffer_device(self._h, C.byref(p)))
+        return p.value
+
+
+class ToneMapper:
+    """ToneMapper (lib/include/Renderer/ToneMapper.h:22-80)."""
+
+    def __init__(self, type: ToneMappingType = ToneMappingType.REINHARD):
+        h = C.c_void_p()
+        _check(lib().cupbr_tonemapper_create(int(type), C.byref(h)))
+        self._h = h
+        self._size = (0, 0)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().cupbr_tonemapper_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def register_image(self, device_ptr: int, width: int, height: int):
+        _check(lib().cupbr_tonemapper_register_image(self._h, C.c_void_p(device_ptr), width, height))
+        self._size = (width, height)
+
+    def register_renderer(self, renderer: PBRenderer):
+        w, h = renderer.size
+        self.register_image(renderer.image_device_ptr(Image.OUTPUT), w, h)
+
+    def tone_map(self):
+        _check(lib().cupbr_tonemapper_tonemap(self._h))
+
+    def tone_map_bloom(self, post: PostProcessor, threshold):
+        c = (C.c_float * 4)(*[float(x) for x in threshold])
+        _check(lib().cupbr_tonemapper_tonemap_bloom(self._h, post._h, c))
+
+    def get_render_buffer(self) -> np.ndarray:
+        w, h = self._size
+        out = np.zeros((h, w, 4), np.uint8)
+        _check(lib().cupbr_tonemapper_get_render_buffer(self._h, _ptr(out), out.size))
+        return out
+
+    def set_type(self, type: ToneMappingType):
+        _check(lib().cupbr_tonemapper_set_type(self._h, int(type)))
+
+    def get_type(self) -> ToneMappingType:
+        t = C.c_int32()
+        _check(lib().cupbr_tonemapper_get_type(self._h, C.byref(t)))
+        return ToneMappingType(t.value)
+
+    def set_exposure(self, exposure: float):
+        _check(lib().cupbr_tonemapper_set_exposure(self._h, exposure))
+
+    def save_to_file(self, path: str):
+        _check(lib().cupbr_tonemapper_save_to_file(self._h, path.encode()))

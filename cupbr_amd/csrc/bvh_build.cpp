@@ -1,0 +1,226 @@
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <limits>
+#include <thread>
+
+namespace cupbr {
+namespace {
+
+struct BNode {
+    float mn[3], mx[3];
+    uint32_t left, right;  // inner
+    uint32_t first, count; // leaf when count > 0
+};
+
+inline float area(const float* mn, const float* mx) {
+    float ex = mx[0] - mn[0], ey = mx[1] - mn[1], ez = mx[2] - mn[2];
+    if (!(ex >= 0.0f) || !(ey >= 0.0f) || !(ez >= 0.0f)) return 0.0f;
+    return 2.0f * (ex * ey + ey * ez + ez * ex);
+}
+
+struct Bin {
+    float mn[3] = {INFINITY, INFINITY, INFINITY};
+    float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint32_t n = 0;
+    void grow(const BuildPrim& p) {
+        for (int k = 0; k < 3; ++k) { mn[k] = std::min(mn[k], p.mn[k]); mx[k] = std::max(mx[k], p.mx[k]); }
+        ++n;
+    }
+    void merge(const Bin& b) {
+        for (int k = 0; k < 3; ++k) { mn[k] = std::min(mn[k], b.mn[k]); mx[k] = std::max(mx[k], b.mx[k]); }
+        n += b.n;
+    }
+};
+
+class BinaryBuilder {
+public:
+    BinaryBuilder(const std::vector<BuildPrim>& p, const BvhBuildOptions& o) : P(p), opt(o) {
+        size_t n = P.size();
+        cen.resize(n * 3);
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) cen[3 * i + k] = 0.5f * (P[i].mn[k] + P[i].mx[k]);
+        idx.resize(n);
+        for (size_t i = 0; i < n; ++i) idx[i] = (uint32_t)i;
+        nodes.resize(std::max<size_t>(1, 2 * n));
+        max_threads = opt.threads > 0 ? opt.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    }
+
+    uint32_t run() {
+        uint32_t root = alloc();
+        build(root, 0, (uint32_t)P.size(), 0);
+        return root;
+    }
+
+    const std::vector<BuildPrim>& P;
+    BvhBuildOptions opt;
+    std::vector<float> cen;
+    std::vector<uint32_t> idx;
+    std::vector<BNode> nodes;
+
+private:
+    std::atomic<uint32_t> next{0};
+    std::atomic<int> active{1};
+    int max_threads = 1;
+
+    uint32_t alloc() { return next.fetch_add(1); }
+
+    void build(uint32_t me, uint32_t lo, uint32_t hi, int depth) {
+        BNode& node = nodes[me];
+        Bin all, cb;
+        for (uint32_t i = lo; i < hi; ++i) {
+            all.grow(P[idx[i]]);
+            BuildPrim c;
+            for (int k = 0; k < 3; ++k) c.mn[k] = c.mx[k] = cen[3 * idx[i] + k];
+            cb.grow(c);
+        }
+        for (int k = 0; k < 3; ++k) { node.mn[k] = all.mn[k]; node.mx[k] = all.mx[k]; }
+        const uint32_t n = hi - lo;
+        const float leaf_cost = opt.intersect_cost * (float)n;
+        if (n <= 1) { make_leaf(node, lo, n); return; }
+
+        constexpr int NB = 32;
+        int best_axis = -1, best_bin = -1;
+        float best_cost = INFINITY;
+        const float parent_area = std::max(area(all.mn, all.mx), 1e-30f);
+        for (int a = 0; a < 3; ++a) {
+            const float ext = cb.mx[a] - cb.mn[a];
+            if (!(ext > 0.0f)) continue;
+            const float scale = (float)NB / ext;
+            Bin bins[NB];
+            for (uint32_t i = lo; i < hi; ++i) {
+                int b = (int)((cen[3 * idx[i] + a] - cb.mn[a]) * scale);
+                b = std::min(NB - 1, std::max(0, b));
+                bins[b].grow(P[idx[i]]);
+            }
+            Bin right[NB];
+            Bin acc;
+            for (int b = NB - 1; b > 0; --b) { acc.merge(bins[b]); right[b] = acc; }
+            Bin left;
+            for (int b = 0; b < NB - 1; ++b) {
+                left.merge(bins[b]);
+                const Bin& r = right[b + 1];
+                if (left.n == 0 || r.n == 0) continue;
+                float cost = opt.traversal_cost +
+                             opt.intersect_cost * (area(left.mn, left.mx) * left.n + area(r.mn, r.mx) * r.n) / parent_area;
+                if (cost < best_cost) { best_cost = cost; best_axis = a; best_bin = b; }
+            }
+        }
+        if (n <= opt.max_leaf && leaf_cost <= best_cost) { make_leaf(node, lo, n); return; }
+
+        uint32_t mid = lo + n / 2;
+        if (best_axis >= 0 && depth < 48) {
+            const float ext = cb.mx[best_axis] - cb.mn[best_axis];
+            const float scale = (float)NB / ext;
+            const float lo_c = cb.mn[best_axis];
+            auto it = std::partition(idx.begin() + lo, idx.begin() + hi, [&](uint32_t p) {
+                int b = (int)((cen[3 * p + best_axis] - lo_c) * scale);
+                b = std::min(NB - 1, std::max(0, b));
+                return b <= best_bin;
+            });
+            uint32_t m = (uint32_t)(it - idx.begin());
+            if (m > lo && m < hi) mid = m;
+        } else {
+            // no usable split (coincident centroids) or too deep: median along the widest axis
+            int a = 0;
+            float best_ext = -1.0f;
+            for (int k = 0; k < 3; ++k)
+                if (cb.mx[k] - cb.mn[k] > best_ext) { best_ext = cb.mx[k] - cb.mn[k]; a = k; }
+            std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi,
+                             [&](uint32_t x, uint32_t y) { return cen[3 * x + a] < cen[3 * y + a]; });
+        }
+        const uint32_t l = alloc(), r = alloc();
+        node.left = l;
+        node.right = r;
+        node.count = 0;
+        if (n > 16384 && active.load() < max_threads) {
+            active.fetch_add(1);
+            std::thread th([=]() { build(l, lo, mid, depth + 1); });
+            build(r, mid, hi, depth + 1);
+            th.join();
+            active.fetch_sub(1);
+        } else {
+            build(l, lo, mid, depth + 1);
+            build(r, mid, hi, depth + 1);
+        }
+    }
+
+    void make_leaf(BNode& node, uint32_t lo, uint32_t n) {
+        node.first = lo;
+        node.count = std::max<uint32_t>(n, 1);
+        node.left = node.right = 0;
+    }
+};
+
+struct Collapser {
+    const std::vector<BNode>& B;
+    Bvh4& out;
+    double root_area = 1.0;
+
+    uint32_t emit(uint32_t b, uint32_t depth) {
+        out.depth = std::max(out.depth, depth + 1);
+        std::vector<uint32_t> kids;
+        const BNode& bn = B[b];
+        if (bn.count) kids.push_back(b);  // a leaf root gets a wrapper node
+        else { kids.push_back(bn.left); kids.push_back(bn.right); }
+        while (kids.size() < 4) {
+            int pick = -1;
+            float best = -1.0f;
+            for (size_t i = 0; i < kids.size(); ++i) {
+                const BNode& k = B[kids[i]];
+                if (k.count) continue;
+                float a = area(k.mn, k.mx);
+                if (a > best) { best = a; pick = (int)i; }
+            }
+            if (pick < 0) break;
+            uint32_t k = kids[pick];
+            kids[pick] = B[k].left;
+            kids.insert(kids.begin() + pick + 1, B[k].right);
+        }
+        const uint32_t me = (uint32_t)out.nodes.size();
+        out.nodes.emplace_back();
+        out.sah_cost += area(bn.mn, bn.mx) / root_area;
+        Node4 n;
+        for (int i = 0; i < 4; ++i) {
+            n.lo_x[i] = n.lo_y[i] = n.lo_z[i] = INFINITY;
+            n.hi_x[i] = n.hi_y[i] = n.hi_z[i] = -INFINITY;
+            n.child[i] = kChildEmpty;
+            n.spare[i] = 0;
+        }
+        for (size_t i = 0; i < kids.size(); ++i) {
+            const BNode& k = B[kids[i]];
+            n.lo_x[i] = k.mn[0]; n.lo_y[i] = k.mn[1]; n.lo_z[i] = k.mn[2];
+            n.hi_x[i] = k.mx[0]; n.hi_y[i] = k.mx[1]; n.hi_z[i] = k.mx[2];
+            if (k.count) {
+                n.child[i] = leaf_word(k.first, k.count);
+                out.leaves++;
+                out.sah_cost += area(k.mn, k.mx) / root_area * k.count;
+            } else {
+                n.child[i] = emit(kids[i], depth + 1);
+            }
+        }
+        out.nodes[me] = n;
+        return me;
+    }
+};
+
+}  // namespace
+
+Bvh4 build_bvh4(const std::vector<BuildPrim>& prims, const BvhBuildOptions& opt_in) {
+    Bvh4 out;
+    if (prims.empty()) return out;
+    BvhBuildOptions opt = opt_in;
+    opt.max_leaf = std::min<uint32_t>(std::max<uint32_t>(opt.max_leaf, 1), kMaxLeafPrims);
+    BinaryBuilder bb(prims, opt);
+    uint32_t root = bb.run();
+    Collapser c{bb.nodes, out};
+    c.root_area = std::max(1e-30f, area(bb.nodes[root].mn, bb.nodes[root].mx));
+    out.nodes.reserve(prims.size() / 2 + 16);
+    c.emit(root, 0);
+    out.order = std::move(bb.idx);
+    return out;
+}
+
+}  // namespace cupbr

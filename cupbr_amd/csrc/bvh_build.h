@@ -1,0 +1,39 @@
+// Host BVH builder: binned-SAH binary tree (task-parallel over subtrees),
+// collapsed into a BVH4 of 128-byte nodes in depth-first order.
+//
+// The reference builds a median split over whole objects with one device
+// allocation per node (lib/src/DataStructure/BoundingVolumeHierarchy.cpp:46-136);
+// this builder works on single primitives (mesh triangles included) so
+// traversal cost is logarithmic in the triangle count. Which primitive wins a
+// tie is decided at traversal time by the reference rank, not by the tree.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "scene_types.h"
+
+namespace cupbr {
+
+struct BuildPrim {
+    float mn[3], mx[3];
+};
+
+struct Bvh4 {
+    std::vector<Node4> nodes;       // nodes[0] is the root
+    std::vector<uint32_t> order;    // leaf order: order[k] = input primitive index
+    uint32_t depth = 0;             // BVH4 levels
+    uint32_t leaves = 0;
+    double sah_cost = 0.0;
+};
+
+struct BvhBuildOptions {
+    uint32_t max_leaf = 8;      // <= kMaxLeafPrims
+    float traversal_cost = 1.0f;
+    float intersect_cost = 1.0f;
+    int threads = 0;            // 0: hardware concurrency
+};
+
+Bvh4 build_bvh4(const std::vector<BuildPrim>& prims, const BvhBuildOptions& opt);
+
+}  // namespace cupbr

@@ -1,0 +1,506 @@
+// Volume, gradient-domain and ray-tracing integrators (one thread per pixel).
+//
+//   volume   volume_kernel + handleMediumInteraction, lib/src/Renderer/VolumeRenderer.cu:11-329
+//   gdpt     gdpt_kernel, init_kernel, optimization_kernel, lib/src/Renderer/GradientDomain.cu:110-402
+//   raytracer raytracing_kernel, lib/src/Renderer/RayTracer.cu:11-65
+// They share the BVH4 traversal and the shading code of the path tracer.
+#include "renderer.h"
+#include "shading.h"
+#include "traverse.h"
+
+namespace cupbr {
+
+constexpr int kIntBlock = 256;
+constexpr int kIntLds = 32;
+
+struct Tracer {
+    const DevScene& S;
+    LaneStack<kIntLds>& st;
+    uint32_t ext = 0, shadow = 0;  // ray counts of this lane
+
+    __device__ __forceinline__ bool closest(v3 o, v3 d, TraceResult& r) {
+        ++ext;
+        traverse<false>(S, o, d, 0.0f, st, r);
+        return r.prim != kNoHit;
+    }
+    // Tracing::traceVisibility, TracingDetail.h:118-183
+    __device__ __forceinline__ bool visible(v3 o, v3 d, float dist) {
+        ++shadow;
+        TraceResult r;
+        return !traverse<true>(S, o, d, dist, st, r);
+    }
+    // Tracing::traceRay(scene, ray, index), TracingDetail.h:68-116: one object, no box test;
+    // a mesh keeps its first strictly-closer triangle (MeshDetail.h:28-35)
+    __device__ bool object(uint32_t obj, v3 o, v3 d, TraceResult& r) {
+        ++ext;
+        r.t = __builtin_inff();
+        r.prim = kNoHit;
+        r.rank = 0xFFFFFFFFu;
+        const ObjInfo& oi = S.objs[obj];
+        for (uint32_t k = 0; k < oi.rank_count; ++k) {
+            const uint32_t prim = S.rank_prim[oi.rank_base + k];
+            if (prim == kNoHit) continue;
+            float t;
+            bool h;
+            if (prim & kUnbBit) {
+                h = plane_quad_test(S.unb[prim & ~kUnbBit], o, d, t);
+            } else {
+                const PrimRec pr = S.prims[prim];
+                if ((__float_as_uint(pr.c.w) & 3u) == kPrimTriangle)
+                    h = mt_test(mk(pr.a.x, pr.a.y, pr.a.z), mk(pr.a.w, pr.b.x, pr.b.y), mk(pr.b.z, pr.b.w, pr.c.x), o, d, t);
+                else
+                    h = sphere_test(mk(pr.a.x, pr.a.y, pr.a.z), pr.a.w, o, d, t);
+            }
+            if (h && t < r.t) {
+                r.t = t;
+                r.prim = prim;
+                r.rank = oi.rank_base + k;
+            }
+        }
+        return r.prim != kNoHit;
+    }
+};
+
+// normal and object of a hit (LocalGeometry, Geometry.h:28-36)
+__device__ __forceinline__ void geometry_of(const DevScene& S, uint32_t prim, v3 P, v3& N, uint32_t& obj) {
+    if (prim & kUnbBit) {
+        const UnbRec& u = S.unb[prim & ~kUnbBit];
+        N = mk(u.n.x, u.n.y, u.n.z);
+        obj = __float_as_uint(u.e1.w);
+        return;
+    }
+    const PrimRec& pr = S.prims[prim];
+    obj = __float_as_uint(pr.c.z);
+    if ((__float_as_uint(pr.c.w) & 3u) == kPrimSphere) {
+        N = normalize(P - mk(pr.a.x, pr.a.y, pr.a.z));
+    } else {
+        const float4 n = S.normals[prim];
+        N = mk(n.x, n.y, n.z);
+    }
+}
+
+__device__ __forceinline__ void add_ray_counts(Counters* c, uint32_t ext, uint32_t shadow) {
+    unsigned long long e = ext, s = shadow;
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off);
+        s += __shfl_xor(s, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&c->extension_rays, e);
+        atomicAdd(&c->shadow_rays, s);
+    }
+}
+
+// ------------------------------------------------------------------ volume
+__device__ v3 volume_path(Tracer& T, const cupbr_camera& cam, uint32_t tid, uint32_t w, uint32_t h, uint32_t frame,
+                          uint32_t max_depth, bool rr) {
+    const DevScene& S = T.S;
+    const float inf = __builtin_inff();
+    uint32_t seed = tea4(tid, frame);
+    v3 o, d;
+    launch_ray(tid % w, tid / w, w, h, cam, seed, o, d);
+    v3 radiance = splat(0.0f), weight = splat(1.0f), ray_start = splat(0.0f), out_dir = splat(0.0f);
+    bool inside = false;
+    uint32_t obj_index = 0;
+    const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
+    cupbr_volume vol = S.volume;
+    uint32_t depth = 0;
+    do {
+        bool next_valid = false;
+        TraceResult hr;
+        const bool hit = inside ? T.object(obj_index, o, d, hr) : T.closest(o, d, hr);
+        const v3 inc = -1.0f * d;
+        if (!hit) {
+            if (S.use_env) radiance += weight * env_lookup(S, d);
+            break;
+        }
+        const v3 P = o + hr.t * d;
+        v3 N;
+        uint32_t obj;
+        geometry_of(S, hr.prim, P, N, obj);
+        const cupbr_material* mp = S.mats + obj;
+        const Mat m = load_mat(mp);
+        if (inside) vol = mp->volume;  // the payload's volume pointer follows the current hit (:277)
+        // handleMediumInteraction :88-212
+        bool medium_event = false;
+        {
+            const float g = vol.g;
+            const v3 sa = ld3(vol.sigma_a), ss = ld3(vol.sigma_s), st = sa + ss;
+            uint32_t ch = (uint32_t)(rnd(seed) * 3.0f);
+            ch = min(ch, 2u);
+            const float stc = comp(st, (int)ch);
+            if (!safe_float_equal(stc, 0.0f)) {
+                const float t = -logf(1.0f - rnd(seed)) / stc;
+                if (t < hr.t) {
+                    medium_event = true;
+                    const v3 ep = o + t * d;
+                    const float sprob = comp(ss, (int)ch) / stc;
+                    if (rnd(seed) < sprob) {
+                        float pdf = 0.0f;
+                        for (int i = 0; i < 3; ++i) pdf += comp(st, i) * expf(-1.0f * comp(st, i) * t);
+                        pdf /= 3.0f;
+                        weight = weight * ss / sprob * vexp(-1.0f * st * t) / pdf;
+                    } else {
+                        weight = splat(0.0f);
+                        break;
+                    }
+                    v3 ldir, le;
+                    float dist;
+                    if (nee_sample(S, m, seed, ep, inc, N, ldir, le, dist)) {
+                        v3 so, sd = normalize(ldir), att = splat(1.0f);
+                        if (inside) {
+                            TraceResult ge;
+                            float gd = 0.0f;
+                            v3 gp = ep;
+                            if (T.object(obj_index, ep, sd, ge)) {
+                                gd = ge.t;
+                                gp = ep + ge.t * sd;
+                            }
+                            att = vexp(-1.0f * st * gd) * hg_phase(g, -dot(inc, ldir));
+                            so = gp + 0.001f * ldir;
+                            dist -= gd;
+                        } else {
+                            so = ep + 0.001f * ldir;
+                        }
+                        const v3 sst = ld3(S.volume.sigma_a) + ld3(S.volume.sigma_s);
+                        if (T.visible(so, sd, dist))
+                            radiance += nl * vexp(-1.0f * sst * fminf(dist, 100000000.0f)) * att * le * weight;
+                    }
+                    const DirPdf hs = hg_sample(g, -1.0f * inc, seed);
+                    out_dir = hs.d;
+                    ray_start = ep;
+                    next_valid = true;
+                } else {
+                    float pdf = 0.0f;
+                    for (int i = 0; i < 3; ++i) pdf += expf(-comp(st, i) * hr.t);
+                    pdf /= 3.0f;
+                    weight = weight * vexp(-1.0f * st * hr.t) / pdf;
+                    ray_start = P;
+                }
+            }
+        }
+        if (!medium_event) {
+            if (m.type == CUPBR_VOLUME_MATERIAL) {  // medium interface :269-278
+                const DirPdf s = sample_direction(m, seed, inc, N);
+                out_dir = s.d;
+                ray_start = P;
+                const bool refl = dot(inc, N) * dot(out_dir, N) > 0;
+                inside = refl ? inside : !inside;
+                obj_index = obj;
+                next_valid = true;
+                vol = inside ? mp->volume : S.volume;
+            } else {
+                if (m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f) {  // directIlluminationVolumetric :24-70
+                    weight = splat(0.0f);
+                } else {
+                    v3 ldir, le;
+                    float dist;
+                    if (nee_sample(S, m, seed, P, inc, N, ldir, le, dist)) {
+                        if (T.visible(P + 0.001f * ldir, normalize(ldir), dist))
+                            radiance += (nl * fmaxf(0.0f, dot(N, ldir))) *
+                                        vexp(-1.0f * (ld3(vol.sigma_s) + ld3(vol.sigma_a)) * fminf(dist, 100000000.0f)) *
+                                        brdf(m, inc, ldir, N) * le * weight;
+                    }
+                }
+                const DirPdf s = sample_direction(m, seed, inc, N);  // indirectIlluminationVolumetric :72-86
+                if (!(norm(s.d) == 0.0f)) {
+                    weight = weight * fabsf(dot(s.d, N)) * brdf(m, inc, s.d, N) / s.pdf;
+                    out_dir = s.d;
+                    ray_start = P;
+                    next_valid = true;
+                }
+            }
+            if (rr) {
+                const float alpha = clampf(fmaxf(weight.x, fmaxf(weight.y, weight.z)), 0.0f, 1.0f);
+                if (rnd(seed) > alpha || safe_float_equal(alpha, 0.0f)) break;
+                weight = weight / alpha;
+            }
+        }
+        o = ray_start + 0.001f * out_dir;  // :298
+        d = normalize(out_dir);
+        if (!next_valid) break;
+        ++depth;
+    } while (depth < max_depth);
+    return radiance;
+}
+
+__device__ __forceinline__ void running_mean(float* px, v3 L, uint32_t f) {  // PathTracer.cu:162-168
+    if (f > 0) {
+        const float a = 1.0f / ((float)f + 1.0f);
+        st3(px, (1.0f - a) * ld3(px) + a * L);
+    } else {
+        st3(px, L);
+    }
+}
+
+__global__ __launch_bounds__(kIntBlock) void k_volume(DevScene S, cupbr_camera cam, uint32_t w, uint32_t h,
+                                                      uint32_t frame0, uint32_t nspp, uint32_t max_depth, int rr,
+                                                      float* __restrict__ img, Counters* c, uint32_t* spill) {
+    __shared__ uint32_t lds[kIntLds * kIntBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    LaneStack<kIntLds> st{lds + threadIdx.x, kIntBlock, spill ? spill + gtid : nullptr, gridDim.x * blockDim.x};
+    Tracer T{S, st};
+    const uint32_t npix = w * h;
+    if (gtid < npix) {
+        for (uint32_t s = 0; s < nspp; ++s) {
+            const v3 L = volume_path(T, cam, gtid, w, h, frame0 + s, max_depth, rr != 0);
+            running_mean(img + 3 * (size_t)gtid, L, frame0 + s);
+        }
+    }
+    add_ray_counts(c, T.ext, T.shadow);
+}
+
+// ------------------------------------------------------------------ gradient-domain
+// collect_radiance, GradientDomain.cu:110-158. The base path records which vertices exist
+// (valid bits); a shift path stops at the first vertex the base path does not have (:127).
+__device__ v3 gdpt_collect(Tracer& T, v3 o, v3 d, uint32_t seed, uint32_t max_depth, uint32_t* valid_out,
+                           uint32_t base_valid, bool shift) {
+    const DevScene& S = T.S;
+    v3 radiance = splat(0.0f), weight = splat(1.0f), out_dir = splat(0.0f);
+    const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
+    uint32_t valid = 0;
+    uint32_t depth = 0;
+    do {
+        bool next_valid = false;
+        if (shift && !((base_valid >> depth) & 1u)) break;
+        TraceResult hr;
+        if (!T.closest(o, d, hr)) {
+            if (S.use_env) radiance += weight * env_lookup(S, d);
+            break;
+        }
+        valid |= 1u << depth;
+        const v3 P = o + hr.t * d;
+        v3 N;
+        uint32_t obj;
+        geometry_of(S, hr.prim, P, N, obj);
+        const Mat m = load_mat(S.mats + obj);
+        const v3 inc = -1.0f * d;
+        radiance += weight * m.albedo_e;
+        if (m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f) {
+            weight = splat(0.0f);
+        } else {
+            v3 ldir, le;
+            float dist;
+            if (nee_sample(S, m, seed, P, inc, N, ldir, le, dist)) {
+                if (T.visible(P + 0.001f * ldir, normalize(ldir), dist))
+                    radiance += (nl * fmaxf(0.0f, dot(N, ldir))) * brdf(m, inc, ldir, N) * le * weight;
+            }
+        }
+        const DirPdf s = sample_direction(m, seed, inc, N);
+        if (!(norm(s.d) == 0.0f)) {
+            weight = weight * fabsf(dot(s.d, N)) * brdf(m, inc, s.d, N) / s.pdf;
+            out_dir = s.d;
+            next_valid = true;
+        }
+        o = P + 0.01f * out_dir;
+        d = normalize(out_dir);
+        if (!next_valid) break;
+        ++depth;
+    } while (depth < max_depth);
+    if (valid_out) *valid_out = valid;
+    return radiance;
+}
+
+// gdpt_kernel, GradientDomain.cu:160-246: images base, gxf, gxb, gyf, gyb
+__global__ __launch_bounds__(kIntBlock) void k_gdpt(DevScene S, cupbr_camera cam, uint32_t w, uint32_t h,
+                                                    uint32_t frame0, uint32_t nspp, uint32_t max_depth,
+                                                    float* __restrict__ base, float* __restrict__ gxf,
+                                                    float* __restrict__ gxb, float* __restrict__ gyf,
+                                                    float* __restrict__ gyb, Counters* c, uint32_t* spill) {
+    __shared__ uint32_t lds[kIntLds * kIntBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    LaneStack<kIntLds> st{lds + threadIdx.x, kIntBlock, spill ? spill + gtid : nullptr, gridDim.x * blockDim.x};
+    Tracer T{S, st};
+    const uint32_t npix = w * h;
+    if (gtid < npix) {
+        const uint32_t px = gtid % w, py = gtid / w;
+        if (px != 0 && px != w - 1 && py != 0 && py != h - 1) {
+            for (uint32_t s = 0; s < nspp; ++s) {
+                const uint32_t f = frame0 + s;
+                uint32_t seed = tea4(gtid, f);
+                v3 ob, db, ol, dl, orr, dr, ou, du, od, dd;
+                launch_ray(px, py, w, h, cam, seed, ob, db);
+                const uint32_t sb = seed;
+                launch_ray(px - 1, py, w, h, cam, seed, ol, dl);
+                const uint32_t sl = seed;
+                launch_ray(px + 1, py, w, h, cam, seed, orr, dr);
+                const uint32_t sr = seed;
+                launch_ray(px, py + 1, w, h, cam, seed, ou, du);
+                const uint32_t su = seed;
+                launch_ray(px, py - 1, w, h, cam, seed, od, dd);
+                const uint32_t sd = seed;
+                uint32_t valid = 0;
+                const v3 Lb = gdpt_collect(T, ob, db, sb, max_depth, &valid, 0, false);
+                const v3 Ll = gdpt_collect(T, ol, dl, sl, max_depth, nullptr, valid, true);
+                const v3 Lr = gdpt_collect(T, orr, dr, sr, max_depth, nullptr, valid, true);
+                const v3 Lu = gdpt_collect(T, ou, du, su, max_depth, nullptr, valid, true);
+                const v3 Ld = gdpt_collect(T, od, dd, sd, max_depth, nullptr, valid, true);
+                const size_t i3 = 3 * (size_t)gtid;
+                running_mean(base + i3, Lb, f);
+                running_mean(gxf + i3, 0.5f * (Lb - Ll), f);
+                running_mean(gxb + i3, 0.5f * (Lr - Lb), f);
+                running_mean(gyf + i3, 0.5f * (Lb - Ld), f);
+                running_mean(gyb + i3, 0.5f * (Lu - Lb), f);
+            }
+        }
+    }
+    add_ray_counts(c, T.ext, T.shadow);
+}
+
+// init_kernel, GradientDomain.cu:248-273
+__global__ void k_gdpt_init(float* recon, const float* base, const float* gxf, const float* gxb, const float* gyf,
+                            const float* gyb, float* gx, float* gy, uint32_t w, uint32_t h) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    st3(recon + 3 * (size_t)i, ld3(base + 3 * (size_t)i));
+    const uint32_t x = i % w, y = i / w;
+    if (x == 0 || y == 0 || x == w - 1 || y == h - 1) return;
+    st3(gx + 3 * (size_t)i, ld3(gxf + 3 * (size_t)i) + ld3(gxb + 3 * (size_t)i));
+    st3(gy + 3 * (size_t)i, ld3(gyf + 3 * (size_t)i) + ld3(gyb + 3 * (size_t)i));
+}
+
+__device__ __forceinline__ float median4(float v0, float v1, float v2, float v3_) {  // compute_median :275-301
+    const float v[4] = {v0, v1, v2, v3_};
+    float m = 0.0f;
+    uint32_t amin = 0, amax = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        m += v[i];
+        if (v[i] < v[amin]) amin = i;
+        if (v[i] > v[amax]) amax = i;
+    }
+    return (m - v[amin] - v[amax]) / 2.0f;
+}
+
+// optimization_kernel, GradientDomain.cu:303-350 (interior pixels; the border of out stays zero)
+__global__ void k_gdpt_opt(const float* __restrict__ recon, const float* __restrict__ gx, const float* __restrict__ gy,
+                           float* __restrict__ out, uint32_t w, uint32_t h) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t x = i % w, y = i / w;
+    if (x == 0 || y == 0 || x == w - 1 || y == h - 1) return;
+    const v3 v1 = ld3(recon + 3 * (size_t)(i - 1)) + ld3(gx + 3 * (size_t)(i - 1));
+    const v3 v2 = ld3(recon + 3 * (size_t)(i - w)) + ld3(gy + 3 * (size_t)(i - w));
+    const v3 v3_ = ld3(recon + 3 * (size_t)(i + 1)) - ld3(gx + 3 * (size_t)i);
+    const v3 v4 = ld3(recon + 3 * (size_t)(i + w)) - ld3(gy + 3 * (size_t)i);
+    st3(out + 3 * (size_t)i, mk(median4(v1.x, v2.x, v3_.x, v4.x), median4(v1.y, v2.y, v3_.y, v4.y),
+                                median4(v1.z, v2.z, v3_.z, v4.z)));
+}
+
+// ------------------------------------------------------------------ ray tracer
+// raytracing_kernel, RayTracer.cu:11-54: primary hit, every light as a point light, shadow rays
+__global__ __launch_bounds__(kIntBlock) void k_raytrace(DevScene S, cupbr_camera cam, uint32_t w, uint32_t h,
+                                                        float* __restrict__ img, Counters* c, uint32_t* spill) {
+    __shared__ uint32_t lds[kIntLds * kIntBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    LaneStack<kIntLds> st{lds + threadIdx.x, kIntBlock, spill ? spill + gtid : nullptr, gridDim.x * blockDim.x};
+    Tracer T{S, st};
+    if (gtid < w * h) {
+        const uint32_t px = gtid % w, py = gtid / w;
+        const float rx = 2.0f * (((float)px + 0.0f) / (float)w - 0.5f);
+        const float ry = 2.0f * (((float)py + 0.0f) / (float)h - 0.5f);
+        const v3 pos = ld3(cam.position);
+        const v3 wp = pos + ld3(cam.z_axis) + rx * ld3(cam.x_axis) + ry * ld3(cam.y_axis);
+        const v3 d = normalize(wp - pos);
+        TraceResult hr;
+        v3 radiance = splat(0.0f);
+        if (T.closest(pos, d, hr)) {
+            const v3 P = pos + hr.t * d;
+            v3 N;
+            uint32_t obj;
+            geometry_of(S, hr.prim, P, N, obj);
+            const Mat m = load_mat(S.mats + obj);
+            const v3 inc = normalize(pos - P);
+            for (uint32_t l = 0; l < S.n_lights; ++l) {
+                const v3 lp = ld3(S.lights[l].position);
+                const v3 ldir = normalize(lp - P);
+                const v3 b = brdf(m, inc, ldir, N);
+                const float dist = norm(P - lp);
+                const v3 lrad = ld3(S.lights[l].intensity) / (dist * dist);
+                const float cos_t = fmaxf(0.0f, dot(N, ldir));
+                if (T.visible(P - kEpsilon * d, normalize(ldir), dist)) radiance += b * lrad * cos_t;
+            }
+        }
+        st3(img + 3 * (size_t)gtid, radiance);
+    }
+    add_ray_counts(c, T.ext, T.shadow);
+}
+
+// ------------------------------------------------------------------ host launchers
+static int prepare(cupbr_renderer* r, uint32_t threads, uint32_t** spill, Counters** c) {
+    int rc;
+    if ((rc = r->wf.counters.ensure(sizeof(Counters)))) return rc;
+    *c = r->wf.counters.as<Counters>();
+    CUPBR_HIP_CHECK(hipMemsetAsync(*c, 0, sizeof(Counters), r->stream));
+    *spill = nullptr;
+    const uint32_t entries = r->scene->stack_entries;
+    if (entries > (uint32_t)kIntLds) {
+        if ((rc = r->wf.spill.ensure((size_t)(entries - kIntLds) * threads * 4))) return rc;
+        *spill = r->wf.spill.as<uint32_t>();
+    }
+    return CUPBR_OK;
+}
+
+int render_volume(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp) {
+    const uint32_t npix = r->width * r->height;
+    const uint32_t blocks = (npix + kIntBlock - 1) / kIntBlock;
+    uint32_t* spill;
+    Counters* c;
+    int rc = prepare(r, blocks * kIntBlock, &spill, &c);
+    if (rc) return rc;
+    // one launch per 4 samples keeps each launch short
+    for (uint32_t s = 0; s < spp; s += 4) {
+        hipLaunchKernelGGL(k_volume, dim3(blocks), dim3(kIntBlock), 0, r->stream, r->scene->dev, cam, r->width,
+                           r->height, frame0 + s, std::min(4u, spp - s), std::max(1u, r->max_trace_depth),
+                           (int)r->russian_roulette, r->images[CUPBR_IMAGE_OUTPUT].as<float>(), c, spill);
+    }
+    CUPBR_HIP_CHECK(hipGetLastError());
+    return CUPBR_OK;
+}
+
+int render_gdpt(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp) {
+    const uint32_t w = r->width, h = r->height, npix = w * h;
+    const uint32_t blocks = (npix + kIntBlock - 1) / kIntBlock;
+    uint32_t* spill;
+    Counters* c;
+    int rc = prepare(r, blocks * kIntBlock, &spill, &c);
+    if (rc) return rc;
+    float* im[kNumImages];
+    for (int k = 0; k < kNumImages; ++k) im[k] = r->images[k].as<float>();
+    for (uint32_t s = 0; s < spp; s += 2) {
+        hipLaunchKernelGGL(k_gdpt, dim3(blocks), dim3(kIntBlock), 0, r->stream, r->scene->dev, cam, w, h, frame0 + s,
+                           std::min(2u, spp - s), std::max(1u, r->max_trace_depth), im[CUPBR_IMAGE_BASE],
+                           im[CUPBR_IMAGE_GRADIENT_X_FORWARD], im[CUPBR_IMAGE_GRADIENT_X_BACKWARD],
+                           im[CUPBR_IMAGE_GRADIENT_Y_FORWARD], im[CUPBR_IMAGE_GRADIENT_Y_BACKWARD], c, spill);
+    }
+    // reconstruction (:380-401); a frame's reconstruction depends only on the current running
+    // means, so it runs once after the call's last frame
+    hipLaunchKernelGGL(k_gdpt_init, dim3(blocks), dim3(kIntBlock), 0, r->stream, im[CUPBR_IMAGE_OUTPUT],
+                       im[CUPBR_IMAGE_BASE], im[CUPBR_IMAGE_GRADIENT_X_FORWARD], im[CUPBR_IMAGE_GRADIENT_X_BACKWARD],
+                       im[CUPBR_IMAGE_GRADIENT_Y_FORWARD], im[CUPBR_IMAGE_GRADIENT_Y_BACKWARD],
+                       im[CUPBR_IMAGE_GRADIENT_X], im[CUPBR_IMAGE_GRADIENT_Y], w, h);
+    float* a = im[CUPBR_IMAGE_OUTPUT];
+    float* b = r->temp.as<float>();
+    for (int it = 0; it < 50; ++it) {  // ping-pong; 50 is even so the result lands in the output image
+        hipLaunchKernelGGL(k_gdpt_opt, dim3(blocks), dim3(kIntBlock), 0, r->stream, a, im[CUPBR_IMAGE_GRADIENT_X],
+                           im[CUPBR_IMAGE_GRADIENT_Y], b, w, h);
+        std::swap(a, b);
+    }
+    CUPBR_HIP_CHECK(hipGetLastError());
+    return CUPBR_OK;
+}
+
+int render_raytracer(cupbr_renderer* r, const cupbr_camera& cam) {
+    const uint32_t npix = r->width * r->height;
+    const uint32_t blocks = (npix + kIntBlock - 1) / kIntBlock;
+    uint32_t* spill;
+    Counters* c;
+    int rc = prepare(r, blocks * kIntBlock, &spill, &c);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_raytrace, dim3(blocks), dim3(kIntBlock), 0, r->stream, r->scene->dev, cam, r->width, r->height,
+                       r->images[CUPBR_IMAGE_OUTPUT].as<float>(), c, spill);
+    CUPBR_HIP_CHECK(hipGetLastError());
+    return CUPBR_OK;
+}
+
+}  // namespace cupbr

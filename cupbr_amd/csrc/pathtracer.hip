@@ -1,0 +1,333 @@
+// PT+NEE in wavefront form (pathtracer_kernel_nee, lib/src/Renderer/PathTracer.cu:97-169).
+//
+// Per batch of paths (one path = one pixel of one reference frame):
+//   raygen      camera rays into queue 0                       (PathTracer.cu:112-117)
+//   per bounce: extend   closest hit for every queued ray      (:126)
+//               shade    emission, NEE sample -> shadow queue,  (:127-158)
+//                        Russian roulette, BSDF sample -> next queue
+//               shadow   any-hit visibility; adds the NEE term  (:64-73)
+//   accumulate  the reference running mean, frame by frame      (:162-168)
+// Queues are compacted per wave with a ballot + one atomic, and every kernel is a
+// persistent grid whose waves take 64-entry chunks from a device work counter, so
+// no host round trip happens between bounces.
+//
+// Differences from the reference that cannot change the image: a path whose
+// throughput became exactly zero (back-facing hit, :36-41) stops instead of
+// bouncing on with zero weight, and a shadow ray whose contribution is exactly
+// zero is not traced.
+#include "renderer.h"
+#include "shading.h"
+#include "traverse.h"
+
+namespace cupbr {
+
+constexpr int kPtBlock = 256;
+constexpr int kPtLds = 32;
+
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t* counter) {
+    uint32_t base = 0;
+    if (__lane_id() == 0) base = atomicAdd(counter, 64u);
+    return __shfl(base, 0);
+}
+
+// appends the lanes with `want` set to a queue; returns the slot of this lane
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* count) {
+    const unsigned long long m = __ballot(want);
+    if (m == 0) return 0;
+    uint32_t base = 0;
+    if (__lane_id() == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    const unsigned long long below = (1ull << __lane_id()) - 1ull;
+    return base + (uint32_t)__popcll(m & below);
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0,
+                                                        uint32_t npath, float4* __restrict__ q_o,
+                                                        float4* __restrict__ q_d, float4* __restrict__ weight,
+                                                        float4* __restrict__ rad, Counters* c) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        c->queue_count[0] = npath;
+        c->work_extend = 0;
+    }
+    if (p >= npath) return;
+    const uint32_t npix = w * h;
+    const uint32_t s = p / npix, pix = p - s * npix;
+    uint32_t seed = tea4(pix, frame0 + s);  // PathTracer.cu:112
+    v3 o, d;
+    launch_ray(pix % w, pix / w, w, h, cam, seed, o, d);
+    q_o[p] = make_float4(o.x, o.y, o.z, 0.0f);
+    q_d[p] = make_float4(d.x, d.y, d.z, __uint_as_float(p));
+    weight[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(seed));
+    rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_pt_extend(DevScene S, int cur, const float4* __restrict__ q_o,
+                                                        const float4* __restrict__ q_d, float2* __restrict__ q_hit,
+                                                        Counters* c, uint32_t* spill, uint32_t spill_stride) {
+    __shared__ uint32_t lds[kPtLds * kPtBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = c->queue_count[cur];
+    if (gtid == 0) {
+        c->queue_count[cur ^ 1] = 0;
+        c->shadow_count = 0;
+        c->work_shade = 0;
+        c->extension_rays += n;
+    }
+    LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
+    for (;;) {
+        const uint32_t base = wave_fetch(&c->work_extend);
+        if (base >= n) break;
+        const uint32_t i = base + __lane_id();
+        if (i < n) {
+            const float4 o4 = q_o[i], d4 = q_d[i];
+            TraceResult r;
+            traverse<false>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r);
+            q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
+        }
+    }
+}
+
+// geometry of a closest-hit record: world normal and object id (LocalGeometry, Geometry.h:28-36)
+__device__ __forceinline__ void hit_geometry(const DevScene& S, uint32_t prim, v3 P, v3& N, uint32_t& obj) {
+    if (prim & kUnbBit) {
+        const UnbRec& u = S.unb[prim & ~kUnbBit];
+        N = mk(u.n.x, u.n.y, u.n.z);
+        obj = __float_as_uint(u.e1.w);
+        return;
+    }
+    const float4 c = S.prims[prim].c;
+    obj = __float_as_uint(c.z);
+    if ((__float_as_uint(c.w) & 3u) == kPrimSphere) {
+        const float4 a = S.prims[prim].a;
+        N = normalize(P - mk(a.x, a.y, a.z));  // SphereDetail.h:66
+    } else {
+        const float4 n = S.normals[prim];
+        N = mk(n.x, n.y, n.z);
+    }
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int rr,
+                                                       const float4* __restrict__ q_o, const float4* __restrict__ q_d,
+                                                       const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
+                                                       float4* __restrict__ nq_d, float4* __restrict__ s_o,
+                                                       float4* __restrict__ s_d, float4* __restrict__ s_c,
+                                                       float4* __restrict__ weight, float4* __restrict__ rad,
+                                                       Counters* c) {
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = c->queue_count[cur];
+    if (gtid == 0) c->work_shadow = 0;
+    const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
+    for (;;) {
+        const uint32_t base = wave_fetch(&c->work_shade);
+        if (base >= n) break;
+        const uint32_t i = base + __lane_id();
+        bool want_next = false, want_shadow = false;
+        float4 no = {}, nd = {}, so = {}, sd = {}, sc = {};
+        if (i < n) {
+            const float4 o4 = q_o[i], d4 = q_d[i];
+            const float2 hh = q_hit[i];
+            const uint32_t p = __float_as_uint(d4.w);
+            const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+            const float4 W = weight[p];
+            const float4 R = rad[p];
+            v3 w3 = mk(W.x, W.y, W.z);
+            uint32_t seed = __float_as_uint(W.w);
+            v3 L = mk(R.x, R.y, R.z);
+            const uint32_t prim = __float_as_uint(hh.y);
+            if (prim == kNoHit) {  // :127-135
+                if (S.use_env) L += w3 * env_lookup(S, d);
+            } else {
+                const v3 P = o + hh.x * d;  // geom.P = origin + t * direction
+                v3 N;
+                uint32_t obj;
+                hit_geometry(S, prim, P, N, obj);
+                const Mat m = load_mat(S.mats + obj);
+                const v3 inc = -1.0f * d;
+                L += w3 * m.albedo_e;  // emissiveIllumintation :20-26
+                bool alive = true;
+                if (m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f) {  // :36-41
+                    w3 = splat(0.0f);
+                    alive = false;
+                } else {
+                    // directIllumination :43-73
+                    v3 ldir, le;
+                    float dist = 0.0f;
+                    const bool have = nee_sample(S, m, seed, P, inc, N, ldir, le, dist);
+                    if (have) {
+                        const v3 contrib = (nl * fmaxf(0.0f, dot(N, ldir))) * brdf(m, inc, ldir, N) * le * w3;
+                        if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
+                            const v3 sorg = P + 0.001f * ldir;
+                            const v3 sdir = normalize(ldir);
+                            want_shadow = true;
+                            so = make_float4(sorg.x, sorg.y, sorg.z, dist);
+                            sd = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(p));
+                            sc = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+                        }
+                    }
+                    if (rr) {  // :141-152
+                        const float alpha = clampf(fmaxf(w3.x, fmaxf(w3.y, w3.z)), 0.0f, 1.0f);
+                        if (rnd(seed) > alpha || safe_float_equal(alpha, 0.0f)) {
+                            alive = false;
+                            w3 = splat(0.0f);
+                        } else {
+                            w3 = w3 / alpha;
+                        }
+                    }
+                    if (alive) {  // indirectIllumination :76-95
+                        const DirPdf sp = sample_direction(m, seed, inc, N);
+                        if (norm(sp.d) < kEpsilon) {
+                            alive = false;
+                        } else {
+                            w3 = w3 * fabsf(dot(sp.d, N)) * brdf(m, inc, sp.d, N) / sp.pdf;
+                            if (bounce + 1 < max_depth) {  // :156-160
+                                const v3 norg = P + 0.01f * sp.d;
+                                const v3 ndir = normalize(sp.d);
+                                want_next = true;
+                                no = make_float4(norg.x, norg.y, norg.z, 0.0f);
+                                nd = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(p));
+                            }
+                        }
+                    }
+                }
+                weight[p] = make_float4(w3.x, w3.y, w3.z, __uint_as_float(seed));
+            }
+            rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
+        }
+        const uint32_t ns = wave_append(want_next, &c->queue_count[cur ^ 1]);
+        if (want_next) {
+            nq_o[ns] = no;
+            nq_d[ns] = nd;
+        }
+        const uint32_t ss = wave_append(want_shadow, &c->shadow_count);
+        if (want_shadow) {
+            s_o[ss] = so;
+            s_d[ss] = sd;
+            s_c[ss] = sc;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_pt_shadow(DevScene S, const float4* __restrict__ s_o,
+                                                        const float4* __restrict__ s_d, const float4* __restrict__ s_c,
+                                                        float4* __restrict__ rad, Counters* c, uint32_t* spill,
+                                                        uint32_t spill_stride) {
+    __shared__ uint32_t lds[kPtLds * kPtBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = c->shadow_count;
+    if (gtid == 0) {
+        c->work_extend = 0;
+        c->shadow_rays += n;
+    }
+    LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
+    for (;;) {
+        const uint32_t base = wave_fetch(&c->work_shadow);
+        if (base >= n) break;
+        const uint32_t i = base + __lane_id();
+        if (i < n) {
+            const float4 o4 = s_o[i], d4 = s_d[i];
+            TraceResult r;
+            const bool occluded = traverse<true>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r);
+            if (!occluded) {  // traceVisibility true: radiance += contribution (:66-73)
+                const uint32_t p = __float_as_uint(d4.w);
+                const float4 cc = s_c[i];
+                float4 R = rad[p];
+                v3 L = mk(R.x, R.y, R.z);
+                L += mk(cc.x, cc.y, cc.z);
+                rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
+            }
+        }
+    }
+}
+
+// running mean of PathTracer.cu:162-168, applied frame by frame in order
+__global__ __launch_bounds__(kPtBlock) void k_accumulate(float* __restrict__ img, const float4* __restrict__ rad,
+                                                         uint32_t npix, uint32_t frame0, uint32_t nspp) {
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    v3 acc = ld3(img + 3 * (size_t)pix);
+    for (uint32_t s = 0; s < nspp; ++s) {
+        const float4 R = rad[(size_t)s * npix + pix];
+        const v3 L = mk(R.x, R.y, R.z);
+        const uint32_t f = frame0 + s;
+        if (f > 0) {
+            const float a = 1.0f / ((float)f + 1.0f);
+            acc = (1.0f - a) * acc + a * L;
+        } else {
+            acc = L;
+        }
+    }
+    st3(img + 3 * (size_t)pix, acc);
+}
+
+static int ensure_wavefront(cupbr_renderer* r, uint32_t paths, uint32_t grid_threads) {
+    Wavefront& wf = r->wf;
+    int rc;
+    if (paths > wf.capacity) {
+        const size_t n = paths;
+        if ((rc = wf.weight.ensure(n * 16))) return rc;
+        if ((rc = wf.radiance.ensure(n * 16))) return rc;
+        for (int k = 0; k < 2; ++k) {
+            if ((rc = wf.q_o[k].ensure(n * 16))) return rc;
+            if ((rc = wf.q_d[k].ensure(n * 16))) return rc;
+        }
+        if ((rc = wf.q_hit.ensure(n * 8))) return rc;
+        if ((rc = wf.s_o.ensure(n * 16))) return rc;
+        if ((rc = wf.s_d.ensure(n * 16))) return rc;
+        if ((rc = wf.s_c.ensure(n * 16))) return rc;
+        wf.capacity = paths;
+    }
+    if ((rc = wf.counters.ensure(sizeof(Counters)))) return rc;
+    const uint32_t entries = r->scene->stack_entries;
+    if (entries > (uint32_t)kPtLds) {
+        if ((rc = wf.spill.ensure((size_t)(entries - kPtLds) * grid_threads * 4))) return rc;
+    }
+    return CUPBR_OK;
+}
+
+int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp) {
+    const uint32_t w = r->width, h = r->height, npix = w * h;
+    if (npix == 0 || spp == 0) return CUPBR_OK;
+    // paths in flight: enough to fill the GPU at the deepest bounce, bounded to ~8M
+    const uint32_t target = 1u << 22, cap = 1u << 23;
+    uint32_t batch = std::max<uint32_t>(1, (target + npix - 1) / npix);
+    batch = std::min(batch, std::max<uint32_t>(1, cap / npix));
+    batch = std::min(batch, spp);
+    const int blocks = device_cu_count() * 8;
+    const uint32_t grid_threads = (uint32_t)blocks * kPtBlock;
+    int rc = ensure_wavefront(r, batch * npix, grid_threads);
+    if (rc) return rc;
+    Wavefront& wf = r->wf;
+    Counters* c = wf.counters.as<Counters>();
+    CUPBR_HIP_CHECK(hipMemsetAsync(c, 0, sizeof(Counters), r->stream));
+    uint32_t* spill = r->scene->stack_entries > (uint32_t)kPtLds ? wf.spill.as<uint32_t>() : nullptr;
+    const DevScene& S = r->scene->dev;
+    float* img = r->images[CUPBR_IMAGE_OUTPUT].as<float>();
+    const uint32_t bounces = std::max<uint32_t>(1, r->max_trace_depth);  // do { } while (depth < max)
+    for (uint32_t s0 = 0; s0 < spp; s0 += batch) {
+        const uint32_t nb = std::min(batch, spp - s0);
+        const uint32_t npath = nb * npix;
+        hipLaunchKernelGGL(k_pt_raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
+                           h, frame0 + s0, npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
+                           wf.weight.as<float4>(), wf.radiance.as<float4>(), c);
+        for (uint32_t b = 0; b < bounces; ++b) {
+            const int cur = (int)(b & 1);
+            hipLaunchKernelGGL(k_pt_extend, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
+                               wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill, grid_threads);
+            hipLaunchKernelGGL(k_pt_shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
+                               (int)r->russian_roulette, wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),
+                               wf.q_hit.as<float2>(), wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(),
+                               wf.s_o.as<float4>(), wf.s_d.as<float4>(), wf.s_c.as<float4>(), wf.weight.as<float4>(),
+                               wf.radiance.as<float4>(), c);
+            hipLaunchKernelGGL(k_pt_shadow, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, wf.s_o.as<float4>(),
+                               wf.s_d.as<float4>(), wf.s_c.as<float4>(), wf.radiance.as<float4>(), c, spill,
+                               grid_threads);
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, img,
+                           wf.radiance.as<float4>(), npix, frame0 + s0, nb);
+    }
+    CUPBR_HIP_CHECK(hipGetLastError());
+    return CUPBR_OK;
+}
+
+}  // namespace cupbr

@@ -1,0 +1,169 @@
+// Bloom and tone mapping.
+//
+// The reference builds a 6-level pyramid in 13 launches (PostProcessor.cpp:125-142,
+// PyramidConstructor.cu:9-169), but its upsampling reads the *down* pyramid
+// (`up[i] = tent(down[i+1])/16 + down[i]`, :79-80,111), so the output depends on
+// levels 0 and 1 only:
+//     out(p) = hdr(p) + ( tent3x3(down1)(p/2) / 16 + down0(p) )
+//     down0  = threshold(hdr),  down1(q) = box2x2(down0)(2q)   (edge-clamped)
+// This file computes exactly that, in the reference's FP32 operation order, in
+// one pass per 64x8 tile: the tile's 34x6 block of down1 (with its one-texel
+// halo) is built in LDS from L2-resident hdr reads, then every output pixel
+// reads its own hdr value once. HBM traffic is one FP32 RGB read and one write
+// per pixel (+ 4 bytes when the RGBA8 tone map is fused in).
+#include <cstdio>
+#include <vector>
+
+#include "post.h"
+#include "vecmath.h"
+
+namespace cupbr {
+
+constexpr int kTileW = 64, kTileH = 8;
+constexpr int kD1W = kTileW / 2 + 2, kD1H = kTileH / 2 + 2;
+
+// thresholding_kernel, PyramidConstructor.cu:9-30
+__device__ __forceinline__ v3 threshold_px(v3 c, float4 t) {
+    float b = c.x > c.y ? c.x : c.y;
+    b = c.z > b ? c.z : b;
+    float rq = clampf(b - t.y, 0.0f, t.z);
+    rq = t.w * rq * rq;
+    const float s = fmaxf(rq, b - t.x) / fmaxf(b, 1e-5f);
+    return v3{c.x * s, c.y * s, c.z * s};
+}
+
+// reinhard_kernel / gamma_kernel, ToneMapper.cu:17-69 (truncating casts; 1.0 - expf() and the
+// blue channel's 1.0f/2.2 are double expressions in the reference)
+__device__ __forceinline__ uchar4 tonemap_px(v3 c, int32_t type, float exposure) {
+    float m[3];
+    const float cc[3] = {c.x, c.y, c.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (type == CUPBR_REINHARD) {
+            const float base = (float)(1.0 - (double)expf(-cc[k] * exposure));
+            m[k] = powf(base, k == 2 ? (float)(1.0f / 2.2) : 1.0f / 2.2f);
+        } else {
+            const float x = cc[k];
+            m[k] = x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+        }
+    }
+    return make_uchar4((uint8_t)(clampf(m[0], 0.0f, 1.0f) * 255.0f), (uint8_t)(clampf(m[1], 0.0f, 1.0f) * 255.0f),
+                       (uint8_t)(clampf(m[2], 0.0f, 1.0f) * 255.0f), 255);
+}
+
+__device__ __forceinline__ v3 ld_px(const float* img, uint32_t w, int x, int y) {
+    return ld3(img + 3 * ((size_t)y * w + x));
+}
+
+__global__ __launch_bounds__(256) void k_bloom(const float* __restrict__ hdr, float* __restrict__ out,
+                                               uchar4* __restrict__ rgba, uint32_t w, uint32_t h, float4 curve,
+                                               int32_t type, float exposure) {
+    __shared__ float d1[3][kD1H][kD1W];
+    const int x0 = blockIdx.x * kTileW, y0 = blockIdx.y * kTileH;
+    const int w1 = (int)(w / 2), h1 = (int)(h / 2);
+    const int qx0 = x0 / 2 - 1, qy0 = y0 / 2 - 1;
+    const int tid = threadIdx.x;
+    if (tid < kD1W * kD1H) {
+        const int j = tid / kD1W, i = tid - j * kD1W;
+        int qx = qx0 + i, qy = qy0 + j;
+        qx = qx < 0 ? 0 : (qx >= w1 ? w1 - 1 : qx);  // upscale_and_combine_kernel :101-104
+        qy = qy < 0 ? 0 : (qy >= h1 ? h1 - 1 : qy);
+        v3 r = splat(0.0f);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {  // downsample_kernel :51-64, reads (0,0),(0,1),(1,0),(1,1)
+            int sx = 2 * qx + (k >> 1), sy = 2 * qy + (k & 1);
+            if (sx >= (int)w) sx = (int)w - 1;
+            if (sy >= (int)h) sy = (int)h - 1;
+            r += threshold_px(ld_px(hdr, w, sx, sy), curve);
+        }
+        r = r / 4.0f;
+        d1[0][j][i] = r.x;
+        d1[1][j][i] = r.y;
+        d1[2][j][i] = r.z;
+    }
+    __syncthreads();
+    const int tx = tid & (kTileW - 1);
+#pragma unroll
+    for (int row = 0; row < kTileH; row += 256 / kTileW) {
+        const int x = x0 + tx, y = y0 + row + tid / kTileW;
+        if (x >= (int)w || y >= (int)h) continue;
+        const v3 c = ld_px(hdr, w, x, y);
+        const v3 d0 = threshold_px(c, curve);
+        const int cx = x / 2 - qx0, cy = y / 2 - qy0;
+        v3 res = splat(0.0f);
+        const float kw[3] = {1.0f, 2.0f, 1.0f};
+#pragma unroll
+        for (int u = -1; u < 2; ++u)
+#pragma unroll
+            for (int v = -1; v < 2; ++v) {
+                const float k = kw[u + 1] * kw[v + 1];
+                res += v3{d1[0][cy + v][cx + u], d1[1][cy + v][cx + u], d1[2][cy + v][cx + u]} * k;
+            }
+        const v3 up0 = res / 16.0f + d0;  // :111
+        const v3 o = c + up0;             // combine_kernel :124
+        st3(out + 3 * ((size_t)y * w + x), o);
+        if (rgba) rgba[(size_t)y * w + x] = tonemap_px(o, type, exposure);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tonemap(const float* __restrict__ hdr, uchar4* __restrict__ rgba, size_t n,
+                                                 int32_t type, float exposure) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rgba[i] = tonemap_px(ld3(hdr + 3 * i), type, exposure);
+}
+
+int post_bloom(cupbr_post* p, const float curve[4], uint8_t* rgba, int32_t type, float exposure) {
+    if (p->width < 2 || p->height < 2)
+        return set_error(CUPBR_ERR_INVALID_ARGUMENT, "bloom needs an image of at least 2x2 pixels");
+    dim3 grid((p->width + kTileW - 1) / kTileW, (p->height + kTileH - 1) / kTileH);
+    float4 c = make_float4(curve[0], curve[1], curve[2], curve[3]);
+    hipLaunchKernelGGL(k_bloom, grid, dim3(256), 0, p->stream, p->hdr, p->output.as<float>(),
+                       reinterpret_cast<uchar4*>(rgba), p->width, p->height, c, type, exposure);
+    CUPBR_HIP_CHECK(hipGetLastError());
+    return CUPBR_OK;
+}
+
+int tonemap(cupbr_tonemapper* t) {
+    const size_t n = (size_t)t->width * t->height;
+    hipLaunchKernelGGL(k_tonemap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, t->stream, t->hdr,
+                       t->rgba.as<uchar4>(), n, t->type, t->exposure);
+    CUPBR_HIP_CHECK(hipGetLastError());
+    return CUPBR_OK;
+}
+
+// 32-bit BI_RGB bitmap, bottom-up: buffer row 0 is written first and shown at the bottom,
+// as stbi_write_bmp does after stbi_flip_vertically_on_write(true) (ToneMapper.cu:194-195)
+int write_bmp(const char* path, const uint8_t* rgba, uint32_t w, uint32_t h) {
+    FILE* f = fopen(path, "wb");
+    if (!f) return set_error(CUPBR_ERR_FILE, std::string("cannot write ") + path, true);
+    const uint32_t data = w * h * 4, off = 14 + 40, size = off + data;
+    uint8_t hdr[54] = {'B', 'M'};
+    auto put32 = [&](int at, uint32_t v) { for (int k = 0; k < 4; ++k) hdr[at + k] = (uint8_t)(v >> (8 * k)); };
+    put32(2, size);
+    put32(10, off);
+    put32(14, 40);
+    put32(18, w);
+    put32(22, h);
+    hdr[26] = 1;
+    hdr[28] = 32;
+    put32(34, data);
+    put32(38, 2835);
+    put32(42, 2835);
+    bool ok = fwrite(hdr, 1, 54, f) == 54;
+    std::vector<uint8_t> row((size_t)w * 4);
+    for (uint32_t y = 0; y < h && ok; ++y) {
+        for (uint32_t x = 0; x < w; ++x) {
+            const uint8_t* s = rgba + 4 * ((size_t)y * w + x);
+            row[4 * x] = s[2];
+            row[4 * x + 1] = s[1];
+            row[4 * x + 2] = s[0];
+            row[4 * x + 3] = s[3];
+        }
+        ok = fwrite(row.data(), 1, row.size(), f) == row.size();
+    }
+    fclose(f);
+    return ok ? CUPBR_OK : set_error(CUPBR_ERR_FILE, std::string("write failed: ") + path, true);
+}
+
+}  // namespace cupbr

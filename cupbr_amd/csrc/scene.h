@@ -1,0 +1,62 @@
+// Host-side scene handle (cupbr_scene) and its device copy.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/cupbr.h"
+#include "device.h"
+#include "scene_types.h"
+
+struct cupbr_scene {
+    // the scene as loaded (the reference's Scene POD, lib/include/Scene/Scene.h:15-34)
+    std::vector<cupbr_object> objects;
+    std::vector<float> triangles;  // 9 floats per mesh triangle
+    std::vector<cupbr_light> lights;
+    std::vector<float> env;        // env_w * env_h * 3, row 0 first as stored in the file
+    uint32_t env_w = 0, env_h = 0;
+    cupbr_volume volume{};
+
+    // derived
+    std::vector<int32_t> ref_order;    // reference BVH leaf order of the objects
+    std::vector<uint32_t> rank_base;   // first reference rank of each object
+    cupbr_bvh_stats stats{};
+    uint32_t stack_entries = 0;        // worst-case traversal stack depth
+    cupbr::DevScene dev{};
+    bool uploaded = false;
+    // host copies of the device arrays (built by scene_finalize, uploaded by scene_upload)
+    std::vector<cupbr::Node4> h_nodes;
+    std::vector<cupbr::PrimRec> h_prims;
+    std::vector<float4> h_normals;
+    std::vector<cupbr::UnbRec> h_unb;
+    std::vector<cupbr::ObjInfo> h_info;
+    std::vector<cupbr_material> h_mats;
+    std::vector<uint32_t> h_rank_prim;
+    std::vector<void*> allocations;
+    int device = 0;
+    cupbr::DevBuf spill;               // traversal stack spill for the batch query kernels
+};
+
+namespace cupbr {
+
+// default Volume (Volume.h:15-21) and Material (Material.h:35-43)
+cupbr_volume default_volume();
+cupbr_material default_material();
+
+// builds ranks and the BVH4 on the host (no GPU needed); returns a CUPBR_ status
+int scene_finalize(cupbr_scene* s);
+// copies the scene to the current device once; every GPU entry point calls it first
+int scene_upload(const cupbr_scene* s);
+void scene_release_device(cupbr_scene* s);
+
+// XML / OBJ / RGBE loading (loader.cpp)
+int load_scene_xml(const std::string& path, cupbr_scene* s);
+int load_rgbe(const std::string& path, std::vector<float>& rgb, uint32_t& w, uint32_t& h);
+
+// batch queries (trace.hip)
+int trace_closest_device(const cupbr_scene* s, const float* rays, size_t n, int32_t* obj, int32_t* tri, float* t,
+                         hipStream_t stream);
+int trace_visibility_device(const cupbr_scene* s, const float* rays, const float* dist, size_t n, uint8_t* vis,
+                            hipStream_t stream);
+
+}  // namespace cupbr

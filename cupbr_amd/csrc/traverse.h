@@ -1,0 +1,245 @@
+// Closest-hit and any-hit traversal of the BVH4, plus the reference's
+// primitive tests.
+//
+// Result semantics (what must match the reference bit for bit):
+//   The reference walks its object BVH depth-first and keeps the first
+//   strictly closer hit (lib/src/DataStructure/BoundingVolumeHierarchyDetail.h:11-101),
+//   testing each object's own box with a biased slab test first
+//   (lib/src/Geometry/AABBDetail.h:18-50). Inner boxes of that tree contain their
+//   children's boxes, so the result is: among primitives whose object box passes
+//   that biased test, the minimum of (t, reference rank), where rank is the
+//   primitive's position in the reference DFS order (mesh triangles in index
+//   order inside their mesh). This traversal computes exactly that minimum over
+//   its own SAH BVH4: boxes are culled conservatively (slack on both ends of
+//   the interval), and the biased object-box test runs lazily, only for a
+//   candidate that would become the new best.
+//
+// Stack: a short per-lane stack in LDS (lane-minor, so a wave's pushes hit 64
+// distinct banks), continued in a global spill area for deep trees.
+#pragma once
+
+#include "scene_types.h"
+
+namespace cupbr {
+
+// Triangle::computeRayIntersection, lib/src/Geometry/TriangleDetail.h:21-57.
+// Returns true for a hit with t >= 0; NaN results report true with NaN t and are
+// discarded by the callers' ordered comparisons, as in the reference.
+__device__ __forceinline__ bool mt_test(v3 v1, v3 e1, v3 e2, v3 o, v3 d, float& t) {
+    v3 pvec = cross(d, e2);
+    float det = dot(e1, pvec);
+    if (safe_float_equal(det, 0.0f)) return false;
+    float inv_det = 1.0f / det;
+    v3 tvec = o - v1;
+    float u = dot(tvec, pvec) * inv_det;
+    if (u < 0.0f || u > 1.0f) return false;
+    v3 qvec = cross(tvec, e1);
+    float v = dot(d, qvec) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(e2, qvec) * inv_det;
+    return !(t < 0.0f);
+}
+
+// Sphere::computeRayIntersection, lib/src/Geometry/SphereDetail.h:33-74
+__device__ __forceinline__ bool sphere_test(v3 c, float r, v3 o, v3 d, float& t) {
+    v3 os = o - c;
+    float b = 2.0f * dot(d, os);
+    float cc = dot(os, os) - r * r;
+    float disc = b * b - 4.0f * cc;
+    if (disc > 0) {
+        float s = sqrtf(disc);
+        float q = b < 0 ? (-b - s) / 2.0f : (-b + s) / 2.0f;
+        float t0 = q, t1 = cc / q;
+        float mn = fminf(t0, t1), mx = fmaxf(t0, t1);
+        float tt = mn < 0 ? mx : mn;
+        if (tt >= 0) { t = tt; return true; }
+    }
+    return false;
+}
+
+// Plane::computeRayIntersection (PlaneDetail.h:19-46) and Quad (QuadDetail.h:21-57)
+__device__ __forceinline__ bool plane_quad_test(const UnbRec& u, v3 o, v3 d, float& t) {
+    v3 p = mk(u.p.x, u.p.y, u.p.z), n = mk(u.n.x, u.n.y, u.n.z);
+    float pdotn = dot(p, n), odotn = dot(o, n), ddotn = dot(d, n);
+    if (safe_float_equal(ddotn, 0.0f)) return false;
+    float tt = (pdotn - odotn) / ddotn;
+    if (tt < 0) return false;
+    if (__float_as_int(u.p.w) == kUnbQuad) {
+        v3 e1 = mk(u.e1.x, u.e1.y, u.e1.z), e2 = mk(u.e2.x, u.e2.y, u.e2.z);
+        v3 ip = o + tt * d;
+        float x = dot(ip - p, e1), y = dot(ip - p, e2);
+        float s1 = dot(e1, e1), s2 = dot(e2, e2);
+        if (x < -s1 || x > s1 || y < -s2 || y > s2) return false;
+    }
+    t = tt;
+    return true;
+}
+
+// AABB::hit, lib/src/Geometry/AABBDetail.h:18-50: 1/(d+1e-5) with no sign correction
+__device__ __forceinline__ bool ref_aabb_hit(const float* mn, const float* mx, v3 o, v3 d) {
+    bool hit = true;
+    float tmin = 0.0f, tmax = __builtin_inff();
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float inv = 1.0f / (comp(d, a) + kEpsilon);
+        float t0 = (mn[a] - comp(o, a)) * inv;
+        float t1 = (mx[a] - comp(o, a)) * inv;
+        if (inv < 0.0f) { float tmp = t0; t0 = t1; t1 = tmp; }
+        tmin = t0 > tmin ? t0 : tmin;
+        tmax = t1 < tmax ? t1 : tmax;
+        if (tmax <= tmin) hit = false;
+    }
+    return hit;
+}
+
+struct TraceResult {
+    float t;
+    uint32_t prim;  // leaf-order prim index, kUnbBit|unbounded index, or kNoHit
+    uint32_t rank;
+};
+
+// Per-lane stack: entries [0, kLds) in LDS at lds[i * lds_stride], the rest in global
+// memory at spill[(i - kLds) * spill_stride].
+template <int kLds>
+struct LaneStack {
+    uint32_t* lds;
+    uint32_t lds_stride;
+    uint32_t* spill;
+    uint32_t spill_stride;
+    __device__ __forceinline__ void put(int i, uint32_t v) {
+        if (i < kLds) lds[i * lds_stride] = v;
+        else spill[(size_t)(i - kLds) * spill_stride] = v;
+    }
+    __device__ __forceinline__ uint32_t get(int i) const {
+        if (i < kLds) return lds[i * lds_stride];
+        return spill[(size_t)(i - kLds) * spill_stride];
+    }
+};
+
+#define CUPBR_CSWAP(a, b)                                             \
+    if (key[b] < key[a]) {                                             \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk;               \
+        uint32_t tv = val[a]; val[a] = val[b]; val[b] = tv;            \
+    }
+
+// kAny=false: closest hit (best.t = +inf, best.prim = kNoHit on a miss).
+// kAny=true : any valid hit with t <= dist; returns whether one exists.
+template <bool kAny, int kLds>
+__device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float dist, LaneStack<kLds>& st,
+                                         TraceResult& best) {
+    const float inf = __builtin_inff();
+    best.t = inf;
+    best.prim = kNoHit;
+    best.rank = 0xFFFFFFFFu;
+
+    // planes and quads first: their hits tighten the interval for the BVH
+    for (uint32_t i = 0; i < S.n_unb; ++i) {
+        const UnbRec u = S.unb[i];
+        float t;
+        if (!plane_quad_test(u, o, d, t)) continue;
+        if (kAny) {
+            if (t <= dist) return true;
+            continue;
+        }
+        uint32_t rank = __float_as_uint(u.n.w);
+        if (t < best.t || (t == best.t && rank < best.rank)) {
+            best.t = t;
+            best.prim = kUnbBit | i;
+            best.rank = rank;
+        }
+    }
+    if (S.n_nodes == 0) return false;
+
+    const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+    const bool sx = ix >= 0.0f, sy = iy >= 0.0f, sz = iz >= 0.0f;
+    const float tlo = -1e-5f;
+    int sp = 0;
+    uint32_t cur = 0;
+    for (;;) {
+        if (cur & kLeafBit) {
+            const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const PrimRec pr = S.prims[first + k];
+                const uint32_t flags = __float_as_uint(pr.c.w);
+                float t;
+                bool h;
+                if ((flags & 3u) == kPrimTriangle)
+                    h = mt_test(mk(pr.a.x, pr.a.y, pr.a.z), mk(pr.a.w, pr.b.x, pr.b.y), mk(pr.b.z, pr.b.w, pr.c.x), o, d, t);
+                else
+                    h = sphere_test(mk(pr.a.x, pr.a.y, pr.a.z), pr.a.w, o, d, t);
+                if (!h) continue;
+                if (kAny) {
+                    if (!(t <= dist)) continue;
+                    if (flags & kPrimNeedsBoxCheck) {
+                        const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
+                        if (!ref_aabb_hit(ob.bmin, ob.bmax, o, d)) continue;
+                    }
+                    return true;
+                }
+                const uint32_t rank = __float_as_uint(pr.c.y);
+                if (t < best.t || (t == best.t && rank < best.rank)) {
+                    if (flags & kPrimNeedsBoxCheck) {
+                        const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
+                        if (!ref_aabb_hit(ob.bmin, ob.bmax, o, d)) continue;
+                    }
+                    best.t = t;
+                    best.prim = first + k;
+                    best.rank = rank;
+                }
+            }
+        } else {
+            const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
+            const float4 lx = nf[0], ly = nf[1], lz = nf[2], hx = nf[3], hy = nf[4], hz = nf[5];
+            const uint4 ch = *reinterpret_cast<const uint4*>(nf + 6);
+            const float4 nx = sx ? lx : hx, fx = sx ? hx : lx;
+            const float4 ny = sy ? ly : hy, fy = sy ? hy : ly;
+            const float4 nz = sz ? lz : hz, fz = sz ? hz : lz;
+            const float lim = kAny ? dist * 1.0001f + 1e-6f : best.t * 1.0001f + 1e-6f;
+            float key[4];
+            uint32_t val[4] = {ch.x, ch.y, ch.z, ch.w};
+#define CUPBR_CHILD(i, c)                                                                          \
+    {                                                                                              \
+        float tn = fmaxf(fmaxf((nx.c - o.x) * ix, (ny.c - o.y) * iy), fmaxf((nz.c - o.z) * iz, tlo)); \
+        float tf = fminf(fminf((fx.c - o.x) * ix, (fy.c - o.y) * iy), fminf((fz.c - o.z) * iz, lim)); \
+        bool ok = val[i] != kChildEmpty && tn <= tf * 1.000002f;                                  \
+        key[i] = ok ? tn : inf;                                                                    \
+    }
+            CUPBR_CHILD(0, x)
+            CUPBR_CHILD(1, y)
+            CUPBR_CHILD(2, z)
+            CUPBR_CHILD(3, w)
+#undef CUPBR_CHILD
+            CUPBR_CSWAP(0, 1)
+            CUPBR_CSWAP(2, 3)
+            CUPBR_CSWAP(0, 2)
+            CUPBR_CSWAP(1, 3)
+            CUPBR_CSWAP(1, 2)
+            if (key[0] < inf) {
+                if (key[3] < inf) st.put(sp++, val[3]);
+                if (key[2] < inf) st.put(sp++, val[2]);
+                if (key[1] < inf) st.put(sp++, val[1]);
+                cur = val[0];
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        cur = st.get(--sp);
+    }
+    return false;
+}
+#undef CUPBR_CSWAP
+
+// object id and triangle-in-mesh index of a closest-hit result
+__device__ __forceinline__ void hit_ids(const DevScene& S, const TraceResult& r, int32_t& obj, int32_t& tri) {
+    if (r.prim == kNoHit) { obj = -1; tri = -1; return; }
+    if (r.prim & kUnbBit) {
+        obj = (int32_t)__float_as_uint(S.unb[r.prim & ~kUnbBit].e1.w);
+        tri = -1;
+        return;
+    }
+    obj = (int32_t)__float_as_uint(S.prims[r.prim].c.z);
+    const ObjInfo& ob = S.objs[obj];
+    tri = ob.type == CUPBR_MESH ? (int32_t)(r.rank - ob.rank_base) : -1;
+}
+
+}  // namespace cupbr

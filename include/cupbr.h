@@ -1,0 +1,268 @@
+/*
+ * cupbr.h — C ABI of the MI355X renderer core.
+ *
+ * The reference exposes this path as C++ classes (SceneLoader, PBRenderer,
+ * PostProcessor, ToneMapper) and plain types (Scene, Camera, Material,
+ * Light, Volume, Image). This header replaces them with opaque handles and
+ * POD structs; include/cupbr.hpp rebuilds the reference's class interface on
+ * top of it. Every entry point names the reference interface it replaces
+ * (paths relative to the reference repository root).
+ *
+ * Conventions
+ *   - Every function returns CUPBR_OK (0) or a negative CUPBR_ERR_* code;
+ *     cupbr_last_error() returns the message of the last failure on the
+ *     calling thread. Where the reference printed a message to std::cerr and
+ *     returned, the same message is printed and a code is returned.
+ *   - Images are FP32 RGB, 12 bytes per pixel, row-major, row 0 = bottom
+ *     (Image<Vector3float>, lib/include/DataStructure/Image.h:12-138).
+ *   - Render buffers are RGBA8 (RenderBuffer, lib/include/DataStructure/RenderBuffer.h:8).
+ *   - "_device" entry points take and return device pointers (HIP memory on
+ *     the current device); the others copy through host memory.
+ *   - A handle is used by one host thread at a time. Calls are asynchronous
+ *     on the handle's HIP stream unless they return host data.
+ */
+#ifndef CUPBR_H
+#define CUPBR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status codes */
+#define CUPBR_OK 0
+#define CUPBR_ERR_INVALID_ARGUMENT (-1)
+#define CUPBR_ERR_NO_OUTPUT_SIZE (-2)   /* PBRenderer.cu:140-144 */
+#define CUPBR_ERR_NO_SCENE (-3)         /* PBRenderer.cu:146-150 */
+#define CUPBR_ERR_FILE (-4)             /* SceneLoader.cpp:148-152 */
+#define CUPBR_ERR_PARSE (-5)            /* missing XML elements (SceneLoader.cpp:155-156,176) */
+#define CUPBR_ERR_GEOMETRY_TYPE (-6)    /* SceneLoader.cpp:264-268 */
+#define CUPBR_ERR_LIGHT_TYPE (-7)       /* SceneLoader.cpp:288-292 */
+#define CUPBR_ERR_MATERIAL_TYPE (-8)    /* SceneLoader.cpp:66-69 */
+#define CUPBR_ERR_NOT_REGISTERED (-9)   /* ToneMapper.cu:154-157 */
+#define CUPBR_ERR_UNSUPPORTED (-10)     /* WHITTED / METROPOLIS, PBRenderer.cu:161-170,187-191 */
+#define CUPBR_ERR_HIP (-11)
+#define CUPBR_ERR_OUT_OF_MEMORY (-12)
+#define CUPBR_ERR_BUFFER_SIZE (-13)
+
+/* ------------------------------------------------------------------ enums */
+/* RenderingMethod, lib/include/Renderer/PBRenderer.h:14-22 */
+enum cupbr_rendering_method {
+    CUPBR_RAYTRACER = 0,
+    CUPBR_WHITTED = 1,
+    CUPBR_PATHTRACER = 2,
+    CUPBR_METROPOLIS = 3,
+    CUPBR_GRADIENTDOMAIN = 4,
+    CUPBR_VOLUME = 5
+};
+/* GeometryType, lib/include/Geometry/Geometry.h:15-23 */
+enum cupbr_geometry_type { CUPBR_SPHERE = 0, CUPBR_PLANE = 1, CUPBR_QUAD = 2, CUPBR_TRIANGLE = 3, CUPBR_MESH = 4 };
+/* MaterialType, lib/include/Geometry/Material.h:14-22 */
+enum cupbr_material_type {
+    CUPBR_LAMBERT = 0, CUPBR_PHONG = 1, CUPBR_MIRROR = 2, CUPBR_GLASS = 3, CUPBR_GGX = 4, CUPBR_VOLUME_MATERIAL = 5
+};
+/* Interface, lib/include/DataStructure/Volume.h:6-10 */
+enum cupbr_interface { CUPBR_INTERFACE_NONE = 0, CUPBR_INTERFACE_GLASS = 1 };
+/* LightType, lib/include/DataStructure/Light.h:11-15 */
+enum cupbr_light_type { CUPBR_POINT = 0, CUPBR_AREA = 1 };
+/* ToneMappingType, lib/include/Renderer/ToneMapper.h:13-17 */
+enum cupbr_tone_mapping_type { CUPBR_REINHARD = 0, CUPBR_GAMMA = 1 };
+/* images a renderer owns (PBRenderer.cu:35-43) */
+enum cupbr_image {
+    CUPBR_IMAGE_OUTPUT = 0,     /* getOutputImage(), PBRenderer.cu:232-236 */
+    CUPBR_IMAGE_GRADIENT_X = 1, /* getGradientX(), :238-242 */
+    CUPBR_IMAGE_GRADIENT_Y = 2, /* getGradientY(), :244-248 */
+    CUPBR_IMAGE_BASE = 3,       /* GDPT primal running mean */
+    CUPBR_IMAGE_GRADIENT_X_FORWARD = 4,
+    CUPBR_IMAGE_GRADIENT_X_BACKWARD = 5,
+    CUPBR_IMAGE_GRADIENT_Y_FORWARD = 6,
+    CUPBR_IMAGE_GRADIENT_Y_BACKWARD = 7
+};
+
+/* ------------------------------------------------------------------ POD types */
+/* Volume, lib/include/DataStructure/Volume.h:15-21 (defaults sigma_s = sigma_a = 1, g = 0) */
+typedef struct cupbr_volume {
+    float sigma_s[3];
+    float sigma_a[3];
+    float g;
+    int32_t interface_type;
+} cupbr_volume;
+
+/* Material, lib/include/Geometry/Material.h:35-43 (84 bytes, same fields and order) */
+typedef struct cupbr_material {
+    float albedo_e[3];
+    float albedo_d[3];
+    float albedo_s[3];
+    float shininess;
+    float roughness;
+    float eta;
+    cupbr_volume volume;
+    int32_t type;
+} cupbr_material;
+
+/* Light, lib/include/DataStructure/Light.h:28-34 */
+typedef struct cupbr_light {
+    int32_t type;
+    float position[3];
+    float intensity[3];
+    float radiance[3];
+    float half_extend1[3];
+    float half_extend2[3];
+} cupbr_light;
+
+/* Camera, lib/include/DataStructure/Camera.h:72-85: pinhole at position, image plane at
+ * position + z_axis spanned by x_axis (right, scaled by the aspect ratio) and y_axis (up). */
+typedef struct cupbr_camera {
+    float position[3];
+    float x_axis[3];
+    float y_axis[3];
+    float z_axis[3];
+    int32_t moved; /* Camera::moved(): non-zero restarts accumulation (PBRenderer.cu:174-177) */
+} cupbr_camera;
+
+/* One scene object (Geometry and its subclasses, lib/include/Geometry/).
+ * params by type:
+ *   SPHERE   center[3], radius                       (Sphere.h)
+ *   PLANE    position[3], normal[3]                  (Plane.h; normal is normalised on load)
+ *   QUAD     position[3], normal[3], extend1[3], extend2[3]   (Quad.h)
+ *   TRIANGLE vertex1[3], vertex2[3], vertex3[3]      (Triangle.h)
+ *   MESH     box min[3], box max[3]; triangles [first_triangle, +triangle_count) of the
+ *            triangle array (Mesh.h; the box is the min/max over the mesh's vertices as
+ *            ObjLoader.cpp:24-47 computes it). */
+typedef struct cupbr_object {
+    int32_t type;
+    float params[12];
+    uint32_t first_triangle;
+    uint32_t triangle_count;
+    cupbr_material material;
+} cupbr_object;
+
+typedef struct cupbr_bvh_stats {
+    uint32_t nodes;          /* 128-byte BVH4 nodes */
+    uint32_t leaves;
+    uint32_t primitives;     /* bounded primitives in the BVH */
+    uint32_t unbounded;      /* planes and quads tested for every ray */
+    uint32_t depth;          /* BVH4 levels */
+    uint32_t stack_entries;  /* traversal stack entries needed in the worst case */
+    double build_ms;
+    double upload_ms;
+    double sah_cost;
+} cupbr_bvh_stats;
+
+typedef struct cupbr_render_stats {
+    uint64_t samples;          /* camera paths traced by the last cupbr_render call */
+    uint64_t extension_rays;   /* closest-hit rays (device queue counters) */
+    uint64_t shadow_rays;      /* NEE visibility rays */
+    double milliseconds;       /* device time of the last cupbr_render call (HIP events) */
+} cupbr_render_stats;
+
+typedef struct cupbr_scene cupbr_scene;
+typedef struct cupbr_renderer cupbr_renderer;
+typedef struct cupbr_post cupbr_post;
+typedef struct cupbr_tonemapper cupbr_tonemapper;
+
+/* ------------------------------------------------------------------ errors / device */
+const char* cupbr_last_error(void);
+int cupbr_device_name(char* out, size_t n);
+
+/* ------------------------------------------------------------------ camera */
+/* Camera(width, height), lib/src/DataStructure/Camera.cpp:6-11: origin, looking +z, 90 deg vertical FOV */
+int cupbr_camera_init(cupbr_camera* cam, uint32_t width, uint32_t height);
+/* extension: pinhole with pose and vertical field of view (degrees), same axis convention */
+int cupbr_camera_look_at(cupbr_camera* cam, const float eye[3], const float target[3], const float up[3],
+                         float vfov_degrees, uint32_t width, uint32_t height);
+
+/* ------------------------------------------------------------------ scene */
+/* Scene SceneLoader::loadFromFile(const std::string&), lib/src/Scene/SceneLoader.cpp:140-454.
+ * Relative mesh paths resolve against the working directory, then against the XML file's directory. */
+int cupbr_scene_load_xml(const char* path, cupbr_scene** out);
+/* the Scene POD (lib/include/Scene/Scene.h:15-34) built from arrays; triangles: n_triangles x 9 floats
+ * (v1, v2, v3); env_rgb: env_w x env_h x 3 floats (NULL for none); volume NULL for the defaults */
+int cupbr_scene_create(const cupbr_object* objects, uint32_t n_objects, const float* triangles, uint32_t n_triangles,
+                       const cupbr_light* lights, uint32_t n_lights, const float* env_rgb, uint32_t env_w,
+                       uint32_t env_h, const cupbr_volume* volume, cupbr_scene** out);
+/* Scene construction above runs on the host (parsing, reference ranks, BVH build); the device copy
+ * is made by the first GPU call that needs it, or here explicitly (the reference uploads inside
+ * loadFromFile, SceneLoader.cpp:370-420) */
+int cupbr_scene_upload(cupbr_scene* scene);
+/* void SceneLoader::destroyScene(Scene&), SceneLoader.cpp:456-522 */
+int cupbr_scene_destroy(cupbr_scene* scene);
+int cupbr_scene_get_counts(const cupbr_scene* scene, uint32_t* n_objects, uint32_t* n_triangles, uint32_t* n_lights,
+                           int32_t* use_environment_map);
+int cupbr_scene_get_objects(const cupbr_scene* scene, cupbr_object* out, uint32_t n);
+int cupbr_scene_get_triangles(const cupbr_scene* scene, float* out, uint32_t n_triangles);
+int cupbr_scene_get_lights(const cupbr_scene* scene, cupbr_light* out, uint32_t n);
+int cupbr_scene_get_volume(const cupbr_scene* scene, cupbr_volume* out);
+int cupbr_scene_get_bvh_stats(const cupbr_scene* scene, cupbr_bvh_stats* out);
+/* reference BVH leaf order of the objects (BoundingVolumeHierarchy.cpp:46-136); n >= n_objects */
+int cupbr_scene_get_reference_order(const cupbr_scene* scene, int32_t* order, uint32_t n);
+
+/* ------------------------------------------------------------------ closest hit / visibility */
+/* Tracing::traceRay(Scene&, const Ray&), lib/src/Core/TracingDetail.h:47-66: rays are n x {o.xyz, d.xyz}
+ * with d normalised (Ray's constructor, RayDetail.h:6-13). Outputs scene_index (-1 on a miss),
+ * the triangle index inside a mesh (-1 otherwise) and the hit depth (+inf on a miss). */
+int cupbr_trace_closest(const cupbr_scene* scene, const float* rays, size_t n, int32_t* object_id,
+                        int32_t* triangle_id, float* t);
+int cupbr_trace_closest_device(const cupbr_scene* scene, const float* rays, size_t n, int32_t* object_id,
+                               int32_t* triangle_id, float* t, void* hip_stream);
+/* Tracing::traceVisibility(Scene&, const float&, const Ray&), TracingDetail.h:118-183 */
+int cupbr_trace_visibility(const cupbr_scene* scene, const float* rays, const float* distance, size_t n,
+                           uint8_t* visible);
+int cupbr_trace_visibility_device(const cupbr_scene* scene, const float* rays, const float* distance, size_t n,
+                                  uint8_t* visible, void* hip_stream);
+
+/* ------------------------------------------------------------------ renderer (PBRenderer.h:27-116) */
+int cupbr_renderer_create(int32_t method, cupbr_renderer** out);       /* PBRenderer(RenderingMethod) */
+int cupbr_renderer_destroy(cupbr_renderer* r);                          /* ~PBRenderer */
+int cupbr_renderer_set_output_size(cupbr_renderer* r, uint32_t width, uint32_t height); /* setOutputSize */
+int cupbr_renderer_register_scene(cupbr_renderer* r, const cupbr_scene* scene);         /* registerScene */
+int cupbr_renderer_set_method(cupbr_renderer* r, int32_t method);       /* setMethod (clears buffers) */
+int cupbr_renderer_get_method(const cupbr_renderer* r, int32_t* method);/* getMethod */
+int cupbr_renderer_reset(cupbr_renderer* r);                            /* reset */
+int cupbr_renderer_set_russian_roulette(cupbr_renderer* r, int32_t enable); /* setRussianRoulette (default off) */
+int cupbr_renderer_set_max_trace_depth(cupbr_renderer* r, uint32_t depth);  /* setMaxTraceDepth */
+int cupbr_renderer_get_max_trace_depth(const cupbr_renderer* r, uint32_t* depth); /* getMaxTraceDepth */
+int cupbr_renderer_get_frame_index(const cupbr_renderer* r, uint32_t* frame_index);
+/* render(Camera*), PBRenderer.cu:137-230, repeated spp times: each sample is one reference frame with
+ * its own seed tea<4>(pixel, frameIndex) and the running mean of PathTracer.cu:162-168. */
+int cupbr_render(cupbr_renderer* r, const cupbr_camera* camera, uint32_t spp);
+int cupbr_renderer_get_image(const cupbr_renderer* r, int32_t which, float* host_rgb, size_t n_floats);
+int cupbr_renderer_get_image_device(const cupbr_renderer* r, int32_t which, float** device_rgb);
+int cupbr_renderer_get_stats(const cupbr_renderer* r, cupbr_render_stats* out);
+int cupbr_renderer_synchronize(cupbr_renderer* r);
+
+/* ------------------------------------------------------------------ post processing (PostProcessor.h:12-52) */
+int cupbr_post_create(cupbr_post** out);
+int cupbr_post_destroy(cupbr_post* p);
+/* registerImage(Image<Vector3float>*): borrows a device FP32 RGB image (e.g. a renderer's output) */
+int cupbr_post_register_image(cupbr_post* p, const float* device_rgb, uint32_t width, uint32_t height);
+/* bloom(Vector4float threshold), PostProcessor.cpp:125-142; curve = Interactor.cpp:436-440 */
+int cupbr_post_bloom(cupbr_post* p, const float threshold_curve[4]);
+int cupbr_post_get_buffer_device(const cupbr_post* p, float** device_rgb);   /* getPostProcessBuffer */
+int cupbr_post_get_buffer(const cupbr_post* p, float* host_rgb, size_t n_floats);
+/* threshold curve from (threshold, knee), Interactor::Impl::compute_threshold */
+int cupbr_threshold_curve(float threshold, float knee, float curve[4]);
+
+/* ------------------------------------------------------------------ tone mapping (ToneMapper.h:22-80) */
+int cupbr_tonemapper_create(int32_t type, cupbr_tonemapper** out);
+int cupbr_tonemapper_destroy(cupbr_tonemapper* t);
+int cupbr_tonemapper_register_image(cupbr_tonemapper* t, const float* device_rgb, uint32_t width, uint32_t height);
+int cupbr_tonemapper_tonemap(cupbr_tonemapper* t);                       /* toneMap */
+/* extension: bloom of p's registered image and tone map of the bloomed result in one pass
+ * (equals cupbr_post_bloom then registering the post buffer and calling toneMap) */
+int cupbr_tonemapper_tonemap_bloom(cupbr_tonemapper* t, cupbr_post* p, const float threshold_curve[4]);
+int cupbr_tonemapper_get_render_buffer(const cupbr_tonemapper* t, uint8_t* host_rgba, size_t n_bytes);
+int cupbr_tonemapper_get_render_buffer_device(const cupbr_tonemapper* t, uint8_t** device_rgba);
+int cupbr_tonemapper_set_type(cupbr_tonemapper* t, int32_t type);
+int cupbr_tonemapper_get_type(const cupbr_tonemapper* t, int32_t* type);
+int cupbr_tonemapper_set_exposure(cupbr_tonemapper* t, float exposure);
+/* saveToFile(path): 32-bit BMP, rows flipped so row 0 of the image is the bottom (ToneMapper.cu:183-198) */
+int cupbr_tonemapper_save_to_file(const cupbr_tonemapper* t, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUPBR_H */

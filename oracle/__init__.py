@@ -1,0 +1,191 @@
+"""ORACLE — test infrastructure only.
+
+CPU restatement of cupbr's renderer core (``cpu_ref.hpp`` / ``cpu_ref.cpp``,
+built into ``liboracle.so`` by ``oracle/Makefile``) plus an independent
+scene loader (``scene_xml.py``). Only ``tests/``, ``__graft_entry__.smoke()``
+and ``bench.py``'s ``cpu_baseline`` leg import this package, and only as the
+checker or the timed CPU baseline; the product library never calls it.
+
+Parity status of each oracle function is recorded in its test module and in
+the design notes; functions without an independent pin say so there.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from .scene_xml import load_scene_xml, load_obj, default_material, GEOM, MAT, LIGHT  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+PT, GDPT, VOLUME = 2, 4, 5
+
+
+def build(quiet: bool = True) -> str:
+    """Compile liboracle.so from cpu_ref.cpp with g++ (-O3, -ffp-contract=off, no fast-math)."""
+    r = subprocess.run(["make", "-C", _HERE, "-s"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        L.orc_scene_create.restype = P
+        L.orc_scene_create.argtypes = [P, P, P, P, P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, C.c_uint32, P]
+        L.orc_scene_destroy.argtypes = [P]
+        L.orc_tea4.restype = C.c_uint32
+        L.orc_tea4.argtypes = [C.c_uint32, C.c_uint32]
+        L.orc_rnd_seq.argtypes = [C.c_uint32, C.c_uint32, P, P]
+        L.orc_leaf_order.restype = C.c_uint32
+        L.orc_leaf_order.argtypes = [P, P, P]
+        L.orc_trace.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, P, P, P, P, P, C.c_int]
+        L.orc_visibility.argtypes = [P, C.c_int, C.c_int, P, P, C.c_uint64, P, C.c_int]
+        L.orc_render.argtypes = [P, C.c_int, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.c_int, C.c_int, C.c_int, P, P, P]
+        L.orc_gdpt.argtypes = [P, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
+                               C.c_int, P]
+        L.orc_gdpt_reconstruct.argtypes = [C.c_uint32, C.c_uint32, P, C.c_int, P, P, P]
+        L.orc_bloom.argtypes = [P, C.c_uint32, C.c_uint32, P, P]
+        L.orc_tonemap.argtypes = [P, C.c_uint64, C.c_int, C.c_float, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def tea4(a: int, b: int) -> int:
+    return int(lib().orc_tea4(a & 0xFFFFFFFF, b & 0xFFFFFFFF))
+
+
+def rnd_seq(seed: int, n: int):
+    out = np.zeros(n, np.float32)
+    st = np.zeros(n, np.uint32)
+    lib().orc_rnd_seq(seed & 0xFFFFFFFF, n, _p(out), _p(st))
+    return out, st
+
+
+def camera_array(cam) -> np.ndarray:
+    """cam: dict with position, x_axis, y_axis, z_axis (Camera.h:72-75)."""
+    return np.concatenate([np.asarray(cam[k], np.float32) for k in ("position", "x_axis", "y_axis", "z_axis")]).astype(np.float32)
+
+
+class OracleScene:
+    """Both precisions of one scene; ``mode`` selects the traversal form (0 tree, 1 flat, 2 SAH)."""
+
+    def __init__(self, desc: dict):
+        self._keep = {k: desc[k] for k in ("types", "params", "first", "count", "mats", "tris", "lights", "volume")}
+        d = {
+            "types": _c(desc["types"], np.int32), "params": _c(desc["params"], np.float32),
+            "first": _c(desc["first"], np.uint32), "count": _c(desc["count"], np.uint32),
+            "mats": _c(desc["mats"], np.float32), "tris": _c(desc["tris"], np.float32).reshape(-1, 9),
+            "lights": _c(desc["lights"], np.float32).reshape(-1, 16), "volume": _c(desc["volume"], np.float32),
+        }
+        env = desc.get("env")
+        self.env = None if env is None else _c(env, np.float32)
+        self._arr = d
+        n = len(d["types"])
+        eh, ew = (0, 0) if self.env is None else self.env.shape[:2]
+        self.n_objects = n
+        self.h = lib().orc_scene_create(_p(d["types"]), _p(d["params"]), _p(d["first"]), _p(d["count"]), _p(d["mats"]),
+                                        n, _p(d["tris"]), len(d["tris"]), _p(d["lights"]), len(d["lights"]),
+                                        _p(self.env), ew, eh, _p(d["volume"]))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def leaf_order(self):
+        order = np.zeros(max(self.n_objects, 1), np.int32)
+        base = np.zeros(max(self.n_objects, 1), np.uint32)
+        k = lib().orc_leaf_order(self.h, _p(order), _p(base))
+        return order[:k], base[:self.n_objects]
+
+    def trace(self, rays, precision=32, mode=2, threads=0, want_geometry=False):
+        rays = _c(rays, np.float32).reshape(-1, 6)
+        n = len(rays)
+        obj = np.zeros(n, np.int32)
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        P = np.zeros((n, 3), np.float64) if want_geometry else None
+        N = np.zeros((n, 3), np.float64) if want_geometry else None
+        lib().orc_trace(self.h, precision, mode, _p(rays), n, _p(obj), _p(tri), _p(t), _p(P), _p(N), threads)
+        if want_geometry:
+            return obj, tri, t, P, N
+        return obj, tri, t
+
+    def visibility(self, rays, dist, precision=32, mode=2, threads=0):
+        rays = _c(rays, np.float32).reshape(-1, 6)
+        dist = _c(dist, np.float32)
+        vis = np.zeros(len(rays), np.uint8)
+        lib().orc_visibility(self.h, precision, mode, _p(rays), _p(dist), len(rays), _p(vis), threads)
+        return vis.astype(bool)
+
+    def render(self, cam, w, h, frame0=0, nframes=1, depth=5, rr=False, method=PT, precision=64, mode=2, threads=0,
+               img=None, stats=False):
+        """Runs frames [frame0, frame0+nframes) with the reference running mean; returns (img, sum, sumsq)."""
+        img = np.zeros((h, w, 3), np.float64) if img is None else _c(img, np.float64).copy()
+        s = np.zeros((h, w, 3), np.float64) if stats else None
+        s2 = np.zeros((h, w, 3), np.float64) if stats else None
+        camf = camera_array(cam)
+        lib().orc_render(self.h, precision, method, _p(camf), w, h, frame0, nframes, depth, int(rr), mode, threads,
+                         _p(img), _p(s), _p(s2))
+        return (img, s, s2) if stats else img
+
+    def gdpt(self, cam, w, h, frame0=0, nframes=1, depth=5, precision=64, mode=2, threads=0, bufs=None):
+        """Returns the 5 running-mean buffers (base, gx_fwd, gx_bwd, gy_fwd, gy_bwd) as [5,h,w,3]."""
+        bufs = np.zeros((5, h, w, 3), np.float64) if bufs is None else _c(bufs, np.float64).copy()
+        camf = camera_array(cam)
+        lib().orc_gdpt(self.h, precision, _p(camf), w, h, frame0, nframes, depth, mode, threads, _p(bufs))
+        return bufs
+
+
+def gdpt_reconstruct(bufs5, iters=50):
+    b = _c(bufs5, np.float32)
+    _, h, w, _ = b.shape
+    recon = np.zeros((h, w, 3), np.float32)
+    gx = np.zeros((h, w, 3), np.float32)
+    gy = np.zeros((h, w, 3), np.float32)
+    lib().orc_gdpt_reconstruct(w, h, _p(b), iters, _p(recon), _p(gx), _p(gy))
+    return recon, gx, gy
+
+
+def bloom(hdr, threshold_curve):
+    hdr = _c(hdr, np.float32)
+    h, w, _ = hdr.shape
+    thr = _c(threshold_curve, np.float32)
+    out = np.zeros_like(hdr)
+    lib().orc_bloom(_p(hdr), w, h, _p(thr), _p(out))
+    return out
+
+
+def tonemap(hdr, kind=0, exposure=1.0):
+    hdr = _c(hdr, np.float32)
+    n = hdr.size // 3
+    out = np.zeros(hdr.shape[:-1] + (4,), np.uint8)
+    lib().orc_tonemap(_p(hdr), n, kind, exposure, _p(out))
+    return out
+
+
+def threshold_curve(threshold=1.0, knee=1.0):
+    """Interactor::Impl::compute_threshold, lib/src/Interaction/Interactor.cpp:436-440."""
+    t = np.float32(threshold)
+    k = np.float32(knee)
+    return np.array([t, k - t, np.float32(2.0) * k, np.float32(0.25) / k], np.float32)
