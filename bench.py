@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Headline benchmark: PT+NEE at 1920x1080 on the ~1M-triangle scene.
+
+One step = one 64-spp frame (64 reference frames of the running mean, depth 8)
+of the 1,021,120-triangle scene (cupbr_amd.scenes.million_triangles), followed
+by bloom (threshold 1, knee 1) and the Reinhard tone map, all on the GPU with
+the scene resident in HBM. Prints one JSON line:
+
+  value        Msamples/s = 1920*1080*64*steps / timed seconds (samples = camera paths)
+  roofline     the extension (closest-hit) traversal kernel, the dominant stage:
+               algorithmic bytes per launch = BVH4 nodes visited x 128 B + triangles
+               tested x 48 B + 40 B of queue traffic per ray (visit counts from an
+               instrumented pass), divided by the kernel's average launch time measured
+               with HIP events on the renderer's stream during the timed steps
+  cpu_baseline the CPU oracle (the reference estimator restated in C++, SAH BVH, FP32)
+               timed on this host on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--cpu-spp", type=int, default=2, help="samples per pixel of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this renderer core runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sample sharding is described in DESIGN.md, not implemented)"}))
+        sys.exit(2)
+
+    import numpy as np
+
+    import cupbr_amd as cb
+    from cupbr_amd import scenes
+
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    W, H, SPP = args.width, args.height, args.spp
+    t0 = time.time()
+    desc, cam_args = scenes.million_triangles(W, H)
+    t_gen = time.time() - t0
+    t0 = time.time()
+    scene = cb.Scene.from_desc(desc)
+    scene.upload()
+    t_build = time.time() - t0
+    bvh = scene.bvh_stats()
+    log(f"scene: {len(desc['tris'])} triangles, {len(desc['types'])} objects; generated in {t_gen:.1f}s, "
+        f"BVH4 build {bvh['build_ms']:.0f} ms + upload {bvh['upload_ms']:.0f} ms ({bvh['nodes']} nodes, depth {bvh['depth']})")
+    cam = cb.Camera.look_at(cam_args["eye"], cam_args["target"], cam_args["up"], cam_args["vfov"], W, H)
+
+    r = cb.PBRenderer(cb.RenderingMethod.PATHTRACER)
+    r.set_output_size(W, H)
+    r.register_scene(scene)
+    r.set_max_trace_depth(args.depth)
+    r.set_russian_roulette(False)
+    post = cb.PostProcessor()
+    post.register_renderer(r)
+    tm = cb.ToneMapper(cb.ToneMappingType.REINHARD)
+    tm.register_renderer(r)
+    curve = cb.threshold_curve(1.0, 1.0)
+
+    def step():
+        r.reset()  # frameIndex = 0: each step is a fresh 64-spp frame (PBRenderer::reset)
+        r.render(cam, SPP)
+        r.synchronize()
+        tm.tone_map_bloom(post, curve)
+
+    # instrumented pass (not timed): traversal visit counts of one step
+    r.set_traversal_counting(True)
+    step()
+    sc = r.stats()
+    r.set_traversal_counting(False)
+    ext_rays = sc["extension_rays"]
+    bytes_per_ext_ray = (sc["extension_node_visits"] * 128 + sc["extension_prim_tests"] * 48 + ext_rays * 40) / max(ext_rays, 1)
+
+    for _ in range(args.warmup):
+        step()
+    r.synchronize()
+    r.set_profiling(True)
+    stage_ms = np.zeros(4)
+    stage_n = np.zeros(4)
+    rays = np.zeros(2)
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        st = r.stats()  # synchronizes the renderer stream
+        stage_ms += st["stage_ms"]
+        stage_n += st["stage_launches"]
+        rays += (st["extension_rays"], st["shadow_rays"])
+    tm_buf = tm.get_render_buffer()  # synchronizes the post chain
+    t_total = time.perf_counter() - t_start
+    r.set_profiling(False)
+
+    samples = W * H * SPP * args.steps
+    value = samples / t_total / 1e6
+    ms_per_step = t_total / args.steps * 1e3
+    ext_launch_ms = stage_ms[0] / max(stage_n[0], 1)
+    ext_bytes_per_launch = bytes_per_ext_ray * rays[0] / max(stage_n[0], 1)
+    achieved = ext_bytes_per_launch / (ext_launch_ms * 1e-3) / 1e9
+    img = r.get_output_image()
+    log(f"GPU: {value:.1f} Msamples/s, {ms_per_step:.1f} ms/step, {(rays.sum()) / t_total / 1e6:.0f} Mrays/s "
+        f"(ext {rays[0] / args.steps / 1e6:.1f}M + shadow {rays[1] / args.steps / 1e6:.1f}M per step); stage ms/step "
+        f"extend {stage_ms[0] / args.steps:.1f} shade {stage_ms[1] / args.steps:.1f} shadow {stage_ms[2] / args.steps:.1f} "
+        f"other {stage_ms[3] / args.steps:.1f}; image mean {img.mean():.4f} finite {bool(np.isfinite(img).all())}; "
+        f"rgba mean {tm_buf[..., :3].mean():.1f}")
+    log(f"extend: {bytes_per_ext_ray:.0f} algorithmic B/ray ({sc['extension_node_visits'] / max(ext_rays, 1):.1f} nodes, "
+        f"{sc['extension_prim_tests'] / max(ext_rays, 1):.1f} tris per ray), {ext_launch_ms:.3f} ms/launch, {achieved:.0f} GB/s")
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "extend_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.time()
+        ref = oracle.OracleScene(desc, precisions=(32,))
+        t_ref_build = time.time() - t0
+        t0 = time.perf_counter()
+        ref.render(cam.as_dict(), W, H, 0, args.cpu_spp, depth=args.depth, precision=32, mode=2, threads=threads)
+        t_cpu = time.perf_counter() - t0
+        cpu_value = W * H * args.cpu_spp / t_cpu / 1e6
+        cpu = {"value": round(cpu_value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_spp} spp of the same 1920x1080 frame ({W * H * args.cpu_spp} paths, depth "
+                         f"{args.depth}) on the C++ oracle with its own SAH BVH, FP32, {threads} threads, "
+                         f"{t_cpu:.1f}s (scene build {t_ref_build:.1f}s not timed)"}
+        log(f"CPU oracle: {cpu_value:.3f} Msamples/s on {threads} threads ({t_cpu:.1f}s); GPU/CPU = {value / cpu_value:.0f}x")
+
+    out = {
+        "metric": "Msamples/s at 1080p/1M-tri",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene, cupbr_amd.scenes.million_triangles)",
+        "config": {
+            "workload": f"PT+NEE {W}x{H}, {SPP} spp, depth {args.depth}, {len(desc['tris'])} triangles "
+                        f"({len(desc['types'])} mesh objects), 4 area lights + 512x256 env map, then bloom + "
+                        f"Reinhard tone map",
+            "width": W, "height": H, "spp": SPP, "max_depth": args.depth, "triangles": int(len(desc["tris"])),
+            "russian_roulette": False, "rng": "reference TEA<4> seed + LCG",
+            "mrays_per_s": round(float(rays.sum()) / t_total / 1e6, 1),
+            "bvh_build_ms": round(bvh["build_ms"], 1),
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_pt_extend (closest-hit BVH4 traversal)",
+                     "bytes_per_ray": round(bytes_per_ext_ray, 1), "avg_launch_ms": round(ext_launch_ms, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -102,7 +102,9 @@ class _BvhStats(C.Structure):
 
 class _RenderStats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
-                ("milliseconds", C.c_double)]
+                ("milliseconds", C.c_double), ("stage_ms", C.c_double * 4), ("stage_launches", C.c_uint32 * 4),
+                ("extension_node_visits", C.c_uint64), ("extension_prim_tests", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_prim_tests", C.c_uint64)]
 
 
 # cupbr_object / cupbr_light as numpy records (144 and 64 bytes, no padding)
@@ -162,6 +164,8 @@ def lib():
             "cupbr_renderer_get_image_device": (C.c_int, [P, I32, P]),
             "cupbr_renderer_get_stats": (C.c_int, [P, P]),
             "cupbr_renderer_synchronize": (C.c_int, [P]),
+            "cupbr_renderer_set_profiling": (C.c_int, [P, I32]),
+            "cupbr_renderer_set_traversal_counting": (C.c_int, [P, I32]),
             "cupbr_post_create": (C.c_int, [P]),
             "cupbr_post_destroy": (C.c_int, [P]),
             "cupbr_post_register_image": (C.c_int, [P, P, U32, U32]),
@@ -501,10 +505,21 @@ class PBRenderer:
     def size(self):
         return self._size
 
+    def set_profiling(self, enable: bool):
+        """Per-stage device time (HIP events on the renderer's stream), reported by stats()."""
+        _check(lib().cupbr_renderer_set_profiling(self._h, int(bool(enable))))
+
+    def set_traversal_counting(self, enable: bool):
+        """Counts BVH4 node visits and primitive tests (reported by stats())."""
+        _check(lib().cupbr_renderer_set_traversal_counting(self._h, int(bool(enable))))
+
     def stats(self) -> dict:
         s = _RenderStats()
         _check(lib().cupbr_renderer_get_stats(self._h, C.byref(s)))
-        return {k: getattr(s, k) for k, _ in _RenderStats._fields_}
+        out = {k: getattr(s, k) for k, _ in _RenderStats._fields_}
+        out["stage_ms"] = list(s.stage_ms)
+        out["stage_launches"] = list(s.stage_launches)
+        return out
 
 
 # ------------------------------------------------------------------ post chain

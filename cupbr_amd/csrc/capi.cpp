@@ -368,6 +368,7 @@ int cupbr_render(cupbr_renderer* r, const cupbr_camera* cam, uint32_t spp) {
     if (int rc = scene_upload(r->scene)) return rc;
     if (cam->moved) r->frame_index = 0;  // :174-177
     r->stats = cupbr_render_stats{};
+    r->timer.used = 0;
     CUPBR_HIP_CHECK(hipEventRecord(r->ev_start, r->stream));
     int rc = CUPBR_OK;
     switch (r->method) {
@@ -402,8 +403,33 @@ int cupbr_renderer_get_stats(const cupbr_renderer* cr, cupbr_render_stats* out) 
         CUPBR_HIP_CHECK(hipMemcpy(&c, r->wf.counters.ptr, sizeof c, hipMemcpyDeviceToHost));
         r->stats.extension_rays = c.extension_rays;
         r->stats.shadow_rays = c.shadow_rays;
+        r->stats.extension_node_visits = c.ext_nodes;
+        r->stats.extension_prim_tests = c.ext_prims;
+        r->stats.shadow_node_visits = c.shadow_nodes;
+        r->stats.shadow_prim_tests = c.shadow_prims;
+    }
+    for (int k = 0; k < 4; ++k) { r->stats.stage_ms[k] = 0.0; r->stats.stage_launches[k] = 0; }
+    for (size_t i = 1; i < r->timer.used; ++i) {
+        const int st = r->timer.stage[i];
+        if (st < 0 || st > 3) continue;
+        float e = 0.0f;
+        CUPBR_HIP_CHECK(hipEventElapsedTime(&e, r->timer.pool[i - 1], r->timer.pool[i]));
+        r->stats.stage_ms[st] += e;
+        r->stats.stage_launches[st] += 1;
     }
     *out = r->stats;
+    return CUPBR_OK;
+}
+
+int cupbr_renderer_set_profiling(cupbr_renderer* r, int32_t enable) {
+    CHECK_ARG(r, "null renderer");
+    r->profiling = enable != 0;
+    return CUPBR_OK;
+}
+
+int cupbr_renderer_set_traversal_counting(cupbr_renderer* r, int32_t enable) {
+    CHECK_ARG(r, "null renderer");
+    r->counting = enable != 0;
     return CUPBR_OK;
 }
 

@@ -7,9 +7,11 @@
 //                        Russian roulette, BSDF sample -> next queue
 //               shadow   any-hit visibility; adds the NEE term  (:64-73)
 //   accumulate  the reference running mean, frame by frame      (:162-168)
-// Queues are compacted per wave with a ballot + one atomic, and every kernel is a
-// persistent grid whose waves take 64-entry chunks from a device work counter, so
-// no host round trip happens between bounces.
+// Queue lengths live on the device, so no host round trip happens between
+// bounces. extend is a persistent grid whose waves take 64-ray chunks from a
+// work counter (traversal cost varies per ray); shade and shadow walk their
+// queues with a static grid stride, and shade appends to the next queues with one
+// atomic per 256-entry block (ballot + LDS prefix).
 //
 // Differences from the reference that cannot change the image: a path whose
 // throughput became exactly zero (back-facing hit, :36-41) stops instead of
@@ -22,6 +24,7 @@
 namespace cupbr {
 
 constexpr int kPtBlock = 256;
+constexpr int kPtWaves = kPtBlock / 64;
 constexpr int kPtLds = 32;
 
 __device__ __forceinline__ uint32_t wave_fetch(uint32_t* counter) {
@@ -30,15 +33,40 @@ __device__ __forceinline__ uint32_t wave_fetch(uint32_t* counter) {
     return __shfl(base, 0);
 }
 
-// appends the lanes with `want` set to a queue; returns the slot of this lane
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* count) {
+// Block-wide append: every thread of the block calls it; lanes with `want` get consecutive
+// slots of the queue whose length is *count (one atomic per block).
+__device__ __forceinline__ uint32_t block_append(bool want, uint32_t* count, uint32_t* s) {
     const unsigned long long m = __ballot(want);
-    if (m == 0) return 0;
-    uint32_t base = 0;
-    if (__lane_id() == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
-    base = __shfl(base, 0);
-    const unsigned long long below = (1ull << __lane_id()) - 1ull;
-    return base + (uint32_t)__popcll(m & below);
+    const uint32_t wave = threadIdx.x >> 6, lane = __lane_id();
+    if (lane == 0) s[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kPtWaves; ++w) {
+            const uint32_t c = s[w];
+            s[w] = tot;
+            tot += c;
+        }
+        s[kPtWaves] = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = s[kPtWaves] + s[wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads();
+    return slot;
+}
+
+// adds a wave's traversal counts to two device counters
+__device__ __forceinline__ void wave_add_counts(const TravCount& tc, unsigned long long* nodes,
+                                                unsigned long long* prims) {
+    unsigned long long a = tc.nodes, b = tc.prims;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(nodes, a);
+        atomicAdd(prims, b);
+    }
 }
 
 __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0,
@@ -47,8 +75,8 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
                                                         float4* __restrict__ rad, Counters* c) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
-        c->queue_count[0] = npath;
-        c->work_extend = 0;
+        c->queue_count[0].v = npath;
+        c->work_extend.v = 0;
     }
     if (p >= npath) return;
     const uint32_t npix = w * h;
@@ -62,30 +90,32 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
     rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+template <bool kCount>
 __global__ __launch_bounds__(kPtBlock) void k_pt_extend(DevScene S, int cur, const float4* __restrict__ q_o,
                                                         const float4* __restrict__ q_d, float2* __restrict__ q_hit,
                                                         Counters* c, uint32_t* spill, uint32_t spill_stride) {
     __shared__ uint32_t lds[kPtLds * kPtBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = c->queue_count[cur];
+    const uint32_t n = c->queue_count[cur].v;
     if (gtid == 0) {
-        c->queue_count[cur ^ 1] = 0;
-        c->shadow_count = 0;
-        c->work_shade = 0;
+        c->queue_count[cur ^ 1].v = 0;
+        c->shadow_count.v = 0;
         c->extension_rays += n;
     }
     LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
+    TravCount tc;
     for (;;) {
-        const uint32_t base = wave_fetch(&c->work_extend);
+        const uint32_t base = wave_fetch(&c->work_extend.v);
         if (base >= n) break;
         const uint32_t i = base + __lane_id();
         if (i < n) {
             const float4 o4 = q_o[i], d4 = q_d[i];
             TraceResult r;
-            traverse<false>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r);
+            traverse<false, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r, &tc);
             q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
         }
     }
+    if (kCount) wave_add_counts(tc, &c->ext_nodes, &c->ext_prims);
 }
 
 // geometry of a closest-hit record: world normal and object id (LocalGeometry, Geometry.h:28-36)
@@ -114,14 +144,12 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
                                                        float4* __restrict__ s_d, float4* __restrict__ s_c,
                                                        float4* __restrict__ weight, float4* __restrict__ rad,
                                                        Counters* c) {
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = c->queue_count[cur];
-    if (gtid == 0) c->work_shadow = 0;
+    __shared__ uint32_t s_append[kPtWaves + 1];
+    const uint32_t n = c->queue_count[cur].v;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->work_extend.v = 0;  // extend of the next bounce starts at 0
     const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
-    for (;;) {
-        const uint32_t base = wave_fetch(&c->work_shade);
-        if (base >= n) break;
-        const uint32_t i = base + __lane_id();
+    for (uint32_t base = blockIdx.x * kPtBlock; base < n; base += gridDim.x * kPtBlock) {
+        const uint32_t i = base + threadIdx.x;
         bool want_next = false, want_shadow = false;
         float4 no = {}, nd = {}, so = {}, sd = {}, sc = {};
         if (i < n) {
@@ -153,8 +181,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
                     // directIllumination :43-73
                     v3 ldir, le;
                     float dist = 0.0f;
-                    const bool have = nee_sample(S, m, seed, P, inc, N, ldir, le, dist);
-                    if (have) {
+                    if (nee_sample(S, m, seed, P, inc, N, ldir, le, dist)) {
                         const v3 contrib = (nl * fmaxf(0.0f, dot(N, ldir))) * brdf(m, inc, ldir, N) * le * w3;
                         if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
                             const v3 sorg = P + 0.001f * ldir;
@@ -194,12 +221,12 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
             }
             rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
         }
-        const uint32_t ns = wave_append(want_next, &c->queue_count[cur ^ 1]);
+        const uint32_t ns = block_append(want_next, &c->queue_count[cur ^ 1].v, s_append);
         if (want_next) {
             nq_o[ns] = no;
             nq_d[ns] = nd;
         }
-        const uint32_t ss = wave_append(want_shadow, &c->shadow_count);
+        const uint32_t ss = block_append(want_shadow, &c->shadow_count.v, s_append);
         if (want_shadow) {
             s_o[ss] = so;
             s_d[ss] = sd;
@@ -208,36 +235,33 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
     }
 }
 
+template <bool kCount>
 __global__ __launch_bounds__(kPtBlock) void k_pt_shadow(DevScene S, const float4* __restrict__ s_o,
                                                         const float4* __restrict__ s_d, const float4* __restrict__ s_c,
                                                         float4* __restrict__ rad, Counters* c, uint32_t* spill,
                                                         uint32_t spill_stride) {
     __shared__ uint32_t lds[kPtLds * kPtBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = c->shadow_count;
-    if (gtid == 0) {
-        c->work_extend = 0;
-        c->shadow_rays += n;
-    }
+    const uint32_t n = c->shadow_count.v;
+    if (gtid == 0) c->shadow_rays += n;
     LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
-    for (;;) {
-        const uint32_t base = wave_fetch(&c->work_shadow);
-        if (base >= n) break;
-        const uint32_t i = base + __lane_id();
-        if (i < n) {
-            const float4 o4 = s_o[i], d4 = s_d[i];
-            TraceResult r;
-            const bool occluded = traverse<true>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r);
-            if (!occluded) {  // traceVisibility true: radiance += contribution (:66-73)
-                const uint32_t p = __float_as_uint(d4.w);
-                const float4 cc = s_c[i];
-                float4 R = rad[p];
-                v3 L = mk(R.x, R.y, R.z);
-                L += mk(cc.x, cc.y, cc.z);
-                rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
-            }
+    TravCount tc;
+    const uint32_t stride = gridDim.x * kPtBlock;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const float4 o4 = s_o[i], d4 = s_d[i];
+        TraceResult r;
+        const bool occluded =
+            traverse<true, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r, &tc);
+        if (!occluded) {  // traceVisibility true: radiance += contribution (:66-73)
+            const uint32_t p = __float_as_uint(d4.w);
+            const float4 cc = s_c[i];
+            const float4 R = rad[p];
+            v3 L = mk(R.x, R.y, R.z);
+            L += mk(cc.x, cc.y, cc.z);
+            rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
         }
     }
+    if (kCount) wave_add_counts(tc, &c->shadow_nodes, &c->shadow_prims);
 }
 
 // running mean of PathTracer.cu:162-168, applied frame by frame in order
@@ -304,27 +328,35 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     const DevScene& S = r->scene->dev;
     float* img = r->images[CUPBR_IMAGE_OUTPUT].as<float>();
     const uint32_t bounces = std::max<uint32_t>(1, r->max_trace_depth);  // do { } while (depth < max)
+    auto extend = r->counting ? k_pt_extend<true> : k_pt_extend<false>;
+    auto shadow = r->counting ? k_pt_shadow<true> : k_pt_shadow<false>;
+    if ((rc = stage_mark(r, -1))) return rc;
     for (uint32_t s0 = 0; s0 < spp; s0 += batch) {
         const uint32_t nb = std::min(batch, spp - s0);
         const uint32_t npath = nb * npix;
         hipLaunchKernelGGL(k_pt_raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
                            h, frame0 + s0, npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
                            wf.weight.as<float4>(), wf.radiance.as<float4>(), c);
+        if ((rc = stage_mark(r, 3))) return rc;
         for (uint32_t b = 0; b < bounces; ++b) {
             const int cur = (int)(b & 1);
-            hipLaunchKernelGGL(k_pt_extend, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
+            hipLaunchKernelGGL(extend, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
                                wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill, grid_threads);
+            if ((rc = stage_mark(r, 0))) return rc;
             hipLaunchKernelGGL(k_pt_shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
                                (int)r->russian_roulette, wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),
                                wf.q_hit.as<float2>(), wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(),
                                wf.s_o.as<float4>(), wf.s_d.as<float4>(), wf.s_c.as<float4>(), wf.weight.as<float4>(),
                                wf.radiance.as<float4>(), c);
-            hipLaunchKernelGGL(k_pt_shadow, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, wf.s_o.as<float4>(),
+            if ((rc = stage_mark(r, 1))) return rc;
+            hipLaunchKernelGGL(shadow, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, wf.s_o.as<float4>(),
                                wf.s_d.as<float4>(), wf.s_c.as<float4>(), wf.radiance.as<float4>(), c, spill,
                                grid_threads);
+            if ((rc = stage_mark(r, 2))) return rc;
         }
         hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, img,
                            wf.radiance.as<float4>(), npix, frame0 + s0, nb);
+        if ((rc = stage_mark(r, 3))) return rc;
     }
     CUPBR_HIP_CHECK(hipGetLastError());
     return CUPBR_OK;

@@ -118,17 +118,23 @@ int post_bloom(cupbr_post* p, const float curve[4], uint8_t* rgba, int32_t type,
         return set_error(CUPBR_ERR_INVALID_ARGUMENT, "bloom needs an image of at least 2x2 pixels");
     dim3 grid((p->width + kTileW - 1) / kTileW, (p->height + kTileH - 1) / kTileH);
     float4 c = make_float4(curve[0], curve[1], curve[2], curve[3]);
+    // the registered image is borrowed: finish pending writes to it first, and return only when
+    // the outputs are complete (the reference synchronises after every launch, PyramidConstructor.cu:134)
+    CUPBR_HIP_CHECK(hipDeviceSynchronize());
     hipLaunchKernelGGL(k_bloom, grid, dim3(256), 0, p->stream, p->hdr, p->output.as<float>(),
                        reinterpret_cast<uchar4*>(rgba), p->width, p->height, c, type, exposure);
     CUPBR_HIP_CHECK(hipGetLastError());
+    CUPBR_HIP_CHECK(hipStreamSynchronize(p->stream));
     return CUPBR_OK;
 }
 
 int tonemap(cupbr_tonemapper* t) {
     const size_t n = (size_t)t->width * t->height;
+    CUPBR_HIP_CHECK(hipDeviceSynchronize());
     hipLaunchKernelGGL(k_tonemap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, t->stream, t->hdr,
                        t->rgba.as<uchar4>(), n, t->type, t->exposure);
     CUPBR_HIP_CHECK(hipGetLastError());
+    CUPBR_HIP_CHECK(hipStreamSynchronize(t->stream));  // ToneMapper.cu:172
     return CUPBR_OK;
 }
 

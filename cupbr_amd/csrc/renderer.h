@@ -2,6 +2,8 @@
 // (lib/src/Renderer/PBRenderer.cu:26-52) plus the wavefront buffers.
 #pragma once
 
+#include <vector>
+
 #include "device.h"
 #include "scene.h"
 
@@ -9,15 +11,44 @@ namespace cupbr {
 
 constexpr int kNumImages = 8;  // cupbr_image
 
-// Device counters of one render call. Work counters hand out 64-entry chunks to
-// persistent waves; each is reset by the kernel that runs after its last user.
+// Device counters of one render call, one 128-byte line each so that atomics on
+// different counters never serialise on the same L2 line. The extension work
+// counter hands out 64-ray chunks to persistent waves; each counter is reset by
+// the kernel that runs after its last user.
+struct alignas(128) CounterLine {
+    uint32_t v;
+    uint32_t pad[31];
+};
 struct Counters {
-    uint32_t queue_count[2];   // extension queues (ping-pong by bounce parity)
-    uint32_t shadow_count;
-    uint32_t work_extend, work_shade, work_shadow;
-    uint32_t pad[2];
+    CounterLine queue_count[2];  // extension queues (ping-pong by bounce parity)
+    CounterLine shadow_count;
+    CounterLine work_extend;
     unsigned long long extension_rays;
     unsigned long long shadow_rays;
+    unsigned long long ext_nodes, ext_prims, shadow_nodes, shadow_prims;  // traversal counting
+};
+
+// HIP events recorded between consecutive launches on the renderer's stream;
+// stage time = time between the event before a launch and the event after it
+struct StageTimer {
+    std::vector<hipEvent_t> pool;
+    std::vector<int> stage;  // stage[i]: stage of the launch that ends at event i (-1: first event)
+    size_t used = 0;
+    int mark(hipStream_t st, int s) {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return CUPBR_ERR_HIP;
+            pool.push_back(e);
+            stage.push_back(-1);
+        }
+        stage[used] = s;
+        if (hipEventRecord(pool[used], st) != hipSuccess) return CUPBR_ERR_HIP;
+        ++used;
+        return CUPBR_OK;
+    }
+    ~StageTimer() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
 };
 
 struct Wavefront {
@@ -48,11 +79,16 @@ struct cupbr_renderer {
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     cupbr_render_stats stats{};
+    bool profiling = false;
+    bool counting = false;
+    cupbr::StageTimer timer;
 };
 
 namespace cupbr {
 
 int renderer_clear_images(cupbr_renderer* r);
+// records a stage boundary when profiling is on (stage: 0 extend, 1 shade, 2 shadow, 3 other)
+inline int stage_mark(cupbr_renderer* r, int stage) { return r->profiling ? r->timer.mark(r->stream, stage) : CUPBR_OK; }
 // one reference frame per sample: frames [frame0, frame0+spp) of method PATHTRACER
 int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp);
 int render_volume(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp);

@@ -122,11 +122,16 @@ struct LaneStack {
         uint32_t tv = val[a]; val[a] = val[b]; val[b] = tv;            \
     }
 
+// visit counters of one lane (only updated by the counting instantiation)
+struct TravCount {
+    uint32_t nodes = 0, prims = 0;
+};
+
 // kAny=false: closest hit (best.t = +inf, best.prim = kNoHit on a miss).
 // kAny=true : any valid hit with t <= dist; returns whether one exists.
-template <bool kAny, int kLds>
+template <bool kAny, int kLds, bool kCount = false>
 __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float dist, LaneStack<kLds>& st,
-                                         TraceResult& best) {
+                                         TraceResult& best, TravCount* cnt = nullptr) {
     const float inf = __builtin_inff();
     best.t = inf;
     best.prim = kNoHit;
@@ -157,8 +162,9 @@ __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float di
     uint32_t cur = 0;
     for (;;) {
         if (cur & kLeafBit) {
-            const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
-            for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t first = leaf_first(cur), n_leaf = leaf_count(cur);
+            if (kCount) cnt->prims += n_leaf;
+            for (uint32_t k = 0; k < n_leaf; ++k) {
                 const PrimRec pr = S.prims[first + k];
                 const uint32_t flags = __float_as_uint(pr.c.w);
                 float t;
@@ -188,6 +194,7 @@ __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float di
                 }
             }
         } else {
+            if (kCount) cnt->nodes += 1;
             const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
             const float4 lx = nf[0], ly = nf[1], lz = nf[2], hx = nf[3], hy = nf[4], hz = nf[5];
             const uint4 ch = *reinterpret_cast<const uint4*>(nf + 6);

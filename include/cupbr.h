@@ -156,6 +156,15 @@ typedef struct cupbr_render_stats {
     uint64_t extension_rays;   /* closest-hit rays (device queue counters) */
     uint64_t shadow_rays;      /* NEE visibility rays */
     double milliseconds;       /* device time of the last cupbr_render call (HIP events) */
+    /* with cupbr_renderer_set_profiling(r, 1): per-stage device time (HIP events around every
+     * launch on the renderer's stream) and launch counts; stage 0 extend, 1 shade, 2 shadow,
+     * 3 ray generation + accumulation */
+    double stage_ms[4];
+    uint32_t stage_launches[4];
+    /* with cupbr_renderer_set_traversal_counting(r, 1): BVH4 nodes visited and primitives
+     * tested by the extension (closest-hit) and shadow (any-hit) traversals */
+    uint64_t extension_node_visits, extension_prim_tests;
+    uint64_t shadow_node_visits, shadow_prim_tests;
 } cupbr_render_stats;
 
 typedef struct cupbr_scene cupbr_scene;
@@ -231,6 +240,9 @@ int cupbr_render(cupbr_renderer* r, const cupbr_camera* camera, uint32_t spp);
 int cupbr_renderer_get_image(const cupbr_renderer* r, int32_t which, float* host_rgb, size_t n_floats);
 int cupbr_renderer_get_image_device(const cupbr_renderer* r, int32_t which, float** device_rgb);
 int cupbr_renderer_get_stats(const cupbr_renderer* r, cupbr_render_stats* out);
+/* extension (no reference counterpart): instrumentation for throughput and roofline reports */
+int cupbr_renderer_set_profiling(cupbr_renderer* r, int32_t enable);
+int cupbr_renderer_set_traversal_counting(cupbr_renderer* r, int32_t enable);
 int cupbr_renderer_synchronize(cupbr_renderer* r);
 
 /* ------------------------------------------------------------------ post processing (PostProcessor.h:12-52) */

@@ -42,7 +42,8 @@ def lib():
         L = C.CDLL(_LIB_PATH)
         P = C.c_void_p
         L.orc_scene_create.restype = P
-        L.orc_scene_create.argtypes = [P, P, P, P, P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, C.c_uint32, P]
+        L.orc_scene_create.argtypes = [P, P, P, P, P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, C.c_uint32, P,
+                                       C.c_int]
         L.orc_scene_destroy.argtypes = [P]
         L.orc_tea4.restype = C.c_uint32
         L.orc_tea4.argtypes = [C.c_uint32, C.c_uint32]
@@ -58,6 +59,16 @@ def lib():
         L.orc_gdpt_reconstruct.argtypes = [C.c_uint32, C.c_uint32, P, C.c_int, P, P, P]
         L.orc_bloom.argtypes = [P, C.c_uint32, C.c_uint32, P, P]
         L.orc_tonemap.argtypes = [P, C.c_uint64, C.c_int, C.c_float, P]
+        L.orc_brdf.argtypes = [P, P, P, P, P]
+        L.orc_sample_direction.argtypes = [P, C.c_uint32, P, P, C.c_uint32, P]
+        L.orc_hg_phase.restype = C.c_double
+        L.orc_hg_phase.argtypes = [C.c_double, C.c_double]
+        L.orc_hg_sample.argtypes = [C.c_double, P, C.c_uint32, C.c_uint32, P]
+        L.orc_d_ggx.restype = C.c_double
+        L.orc_d_ggx.argtypes = [C.c_double, C.c_double]
+        L.orc_light_sample.argtypes = [P, C.c_uint32, P, C.c_uint32, P]
+        L.orc_to_local_frame.argtypes = [P, P, P]
+        L.orc_aabb_hit.argtypes = [P, P, P, C.c_uint64, P]
         _lib = L
     return _lib
 
@@ -87,9 +98,11 @@ def camera_array(cam) -> np.ndarray:
 
 
 class OracleScene:
-    """Both precisions of one scene; ``mode`` selects the traversal form (0 tree, 1 flat, 2 SAH)."""
+    """One scene in FP32 and/or FP64 (``precisions`` = (32,), (64,) or both); ``mode`` selects the
+    traversal form (0 reference object BVH, 1 its leaf order, 2 SAH BVH)."""
 
-    def __init__(self, desc: dict):
+    def __init__(self, desc: dict, precisions=(32, 64)):
+        self.precisions = tuple(precisions)
         self._keep = {k: desc[k] for k in ("types", "params", "first", "count", "mats", "tris", "lights", "volume")}
         d = {
             "types": _c(desc["types"], np.int32), "params": _c(desc["params"], np.float32),
@@ -105,7 +118,8 @@ class OracleScene:
         self.n_objects = n
         self.h = lib().orc_scene_create(_p(d["types"]), _p(d["params"]), _p(d["first"]), _p(d["count"]), _p(d["mats"]),
                                         n, _p(d["tris"]), len(d["tris"]), _p(d["lights"]), len(d["lights"]),
-                                        _p(self.env), ew, eh, _p(d["volume"]))
+                                        _p(self.env), ew, eh, _p(d["volume"]),
+                                        (1 if 32 in self.precisions else 0) | (2 if 64 in self.precisions else 0))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -118,7 +132,12 @@ class OracleScene:
         k = lib().orc_leaf_order(self.h, _p(order), _p(base))
         return order[:k], base[:self.n_objects]
 
+    def _need(self, precision):
+        if precision not in self.precisions:
+            raise ValueError(f"scene was built without FP{precision}")
+
     def trace(self, rays, precision=32, mode=2, threads=0, want_geometry=False):
+        self._need(precision)
         rays = _c(rays, np.float32).reshape(-1, 6)
         n = len(rays)
         obj = np.zeros(n, np.int32)
@@ -132,6 +151,7 @@ class OracleScene:
         return obj, tri, t
 
     def visibility(self, rays, dist, precision=32, mode=2, threads=0):
+        self._need(precision)
         rays = _c(rays, np.float32).reshape(-1, 6)
         dist = _c(dist, np.float32)
         vis = np.zeros(len(rays), np.uint8)
@@ -140,7 +160,9 @@ class OracleScene:
 
     def render(self, cam, w, h, frame0=0, nframes=1, depth=5, rr=False, method=PT, precision=64, mode=2, threads=0,
                img=None, stats=False):
-        """Runs frames [frame0, frame0+nframes) with the reference running mean; returns (img, sum, sumsq)."""
+        """Runs frames [frame0, frame0+nframes) with the reference running mean; returns img, or
+        (img, sum, sumsq) of the per-frame samples with stats=True."""
+        self._need(precision)
         img = np.zeros((h, w, 3), np.float64) if img is None else _c(img, np.float64).copy()
         s = np.zeros((h, w, 3), np.float64) if stats else None
         s2 = np.zeros((h, w, 3), np.float64) if stats else None
@@ -151,6 +173,7 @@ class OracleScene:
 
     def gdpt(self, cam, w, h, frame0=0, nframes=1, depth=5, precision=64, mode=2, threads=0, bufs=None):
         """Returns the 5 running-mean buffers (base, gx_fwd, gx_bwd, gy_fwd, gy_bwd) as [5,h,w,3]."""
+        self._need(precision)
         bufs = np.zeros((5, h, w, 3), np.float64) if bufs is None else _c(bufs, np.float64).copy()
         camf = camera_array(cam)
         lib().orc_gdpt(self.h, precision, _p(camf), w, h, frame0, nframes, depth, mode, threads, _p(bufs))
@@ -189,3 +212,55 @@ def threshold_curve(threshold=1.0, knee=1.0):
     t = np.float32(threshold)
     k = np.float32(knee)
     return np.array([t, k - t, np.float32(2.0) * k, np.float32(0.25) / k], np.float32)
+
+
+# ---- probes of single functions (FP64)
+def _d(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def brdf(mat21, inc, out, normal):
+    r = np.zeros(3)
+    lib().orc_brdf(_p(_c(mat21, np.float32)), _p(_d(inc)), _p(_d(out)), _p(_d(normal)), _p(r))
+    return r
+
+
+def sample_direction(mat21, seed, inc, normal, n):
+    """n successive Material::sampleDirection draws from one RNG stream: [n,4] (dir.xyz, pdf)."""
+    out = np.zeros((n, 4))
+    lib().orc_sample_direction(_p(_c(mat21, np.float32)), seed & 0xFFFFFFFF, _p(_d(inc)), _p(_d(normal)), n, _p(out))
+    return out
+
+
+def hg_phase(g, cos_theta):
+    return lib().orc_hg_phase(float(g), float(cos_theta))
+
+
+def hg_sample(g, forward, seed, n):
+    out = np.zeros((n, 4))
+    lib().orc_hg_sample(float(g), _p(_d(forward)), seed & 0xFFFFFFFF, n, _p(out))
+    return out
+
+
+def d_ggx(ndoth, roughness):
+    return lib().orc_d_ggx(float(ndoth), float(roughness))
+
+
+def light_sample(light16, seed, pos, n):
+    """n successive Light::sample draws: [n,7] (dir.xyz, distance, radiance.xyz)."""
+    out = np.zeros((n, 7))
+    lib().orc_light_sample(_p(_c(light16, np.float32)), seed & 0xFFFFFFFF, _p(_d(pos)), n, _p(out))
+    return out
+
+
+def to_local_frame(normal, d):
+    out = np.zeros(3)
+    lib().orc_to_local_frame(_p(_d(normal)), _p(_d(d)), _p(out))
+    return out
+
+
+def aabb_hit(bmin, bmax, rays):
+    rays = _c(rays, np.float32).reshape(-1, 6)
+    out = np.zeros(len(rays), np.uint8)
+    lib().orc_aabb_hit(_p(_c(bmin, np.float32)), _p(_c(bmax, np.float32)), _p(rays), len(rays), _p(out))
+    return out.astype(bool)

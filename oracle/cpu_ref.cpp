@@ -992,12 +992,16 @@ static void gdpt_impl(const SceneT<R>& S, const float* camf, uint32_t w, uint32_
 
 extern "C" {
 
+// precisions: bit 0 builds the FP32 scene, bit 1 the FP64 scene
 void* orc_scene_create(const int32_t* types, const float* params, const uint32_t* first, const uint32_t* count,
                        const float* mats, uint32_t n_obj, const float* tris, uint32_t n_tri, const float* lights,
-                       uint32_t n_lights, const float* env, uint32_t env_w, uint32_t env_h, const float* volume7) {
+                       uint32_t n_lights, const float* env, uint32_t env_w, uint32_t env_h, const float* volume7,
+                       int precisions) {
     auto* s = new OrcScene();
-    s->f = make_scene<float>(types, params, first, count, mats, n_obj, tris, n_tri, lights, n_lights, env, env_w, env_h, volume7);
-    s->d = make_scene<double>(types, params, first, count, mats, n_obj, tris, n_tri, lights, n_lights, env, env_w, env_h, volume7);
+    if (precisions & 1)
+        s->f = make_scene<float>(types, params, first, count, mats, n_obj, tris, n_tri, lights, n_lights, env, env_w, env_h, volume7);
+    if (precisions & 2)
+        s->d = make_scene<double>(types, params, first, count, mats, n_obj, tris, n_tri, lights, n_lights, env, env_w, env_h, volume7);
     return s;
 }
 
@@ -1020,9 +1024,11 @@ void orc_rnd_seq(uint32_t seed, uint32_t n, float* out, uint32_t* states) {
 // reference DFS leaf order of the objects and their rank bases
 uint32_t orc_leaf_order(void* p, int32_t* order, uint32_t* rank_base) {
     auto* s = (OrcScene*)p;
-    for (size_t i = 0; i < s->f->leaf_order.size(); ++i) order[i] = s->f->leaf_order[i];
-    for (size_t i = 0; i < s->f->rank_base.size(); ++i) rank_base[i] = s->f->rank_base[i];
-    return (uint32_t)s->f->leaf_order.size();
+    const std::vector<int>& lo = s->f ? s->f->leaf_order : s->d->leaf_order;
+    const std::vector<uint32_t>& rb = s->f ? s->f->rank_base : s->d->rank_base;
+    for (size_t i = 0; i < lo.size(); ++i) order[i] = lo[i];
+    for (size_t i = 0; i < rb.size(); ++i) rank_base[i] = rb[i];
+    return (uint32_t)lo.size();
 }
 
 // rays: n x {o.xyz, d.xyz} (d normalised by the caller as Ray's constructor does)
@@ -1052,6 +1058,58 @@ void orc_gdpt(void* p, int precision, const float* cam, uint32_t w, uint32_t h, 
     auto* s = (OrcScene*)p;
     if (precision == 64) gdpt_impl(*s->d, cam, w, h, frame0, nframes, depth, mode, threads, bufs);
     else gdpt_impl(*s->f, cam, w, h, frame0, nframes, depth, mode, threads, bufs);
+}
+
+// ---- probes of single functions (FP64), for the closed-form checks of the tests
+void orc_brdf(const float* mat21, const double* i3, const double* o3, const double* n3, double* out3) {
+    Material<double> m = read_mat<double>(mat21);
+    V3<double> r = brdf(m, V3<double>(i3[0], i3[1], i3[2]), V3<double>(o3[0], o3[1], o3[2]), V3<double>(n3[0], n3[1], n3[2]));
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
+// n samples of Material::sampleDirection from consecutive RNG states starting at seed; out4: dir.xyz, pdf
+void orc_sample_direction(const float* mat21, uint32_t seed, const double* i3, const double* n3, uint32_t n, double* out4) {
+    Material<double> m = read_mat<double>(mat21);
+    for (uint32_t k = 0; k < n; ++k) {
+        V4<double> s = sampleDirection(m, seed, V3<double>(i3[0], i3[1], i3[2]), V3<double>(n3[0], n3[1], n3[2]));
+        out4[4 * k] = s.v.x; out4[4 * k + 1] = s.v.y; out4[4 * k + 2] = s.v.z; out4[4 * k + 3] = s.w;
+    }
+}
+double orc_hg_phase(double g, double c) { return hg_phase(g, c); }
+void orc_hg_sample(double g, const double* fwd3, uint32_t seed, uint32_t n, double* out4) {
+    for (uint32_t k = 0; k < n; ++k) {
+        V4<double> s = hg_sample(g, V3<double>(fwd3[0], fwd3[1], fwd3[2]), seed);
+        out4[4 * k] = s.v.x; out4[4 * k + 1] = s.v.y; out4[4 * k + 2] = s.v.z; out4[4 * k + 3] = s.w;
+    }
+}
+double orc_d_ggx(double ndoth, double rough) { return D_GGX(ndoth, rough); }
+// n samples of Light::sample at pos; out7: dir.xyz, dist, radiance.xyz
+void orc_light_sample(const float* light16, uint32_t seed, const double* pos3, uint32_t n, double* out7) {
+    Light<double> L;
+    L.type = (int)light16[0];
+    L.position = V3<double>(light16[1], light16[2], light16[3]);
+    L.intensity = V3<double>(light16[4], light16[5], light16[6]);
+    L.radiance = V3<double>(light16[7], light16[8], light16[9]);
+    L.halfExtend1 = V3<double>(light16[10], light16[11], light16[12]);
+    L.halfExtend2 = V3<double>(light16[13], light16[14], light16[15]);
+    for (uint32_t k = 0; k < n; ++k) {
+        V3<double> dir;
+        double dist;
+        V3<double> le = light_sample(L, seed, V3<double>(pos3[0], pos3[1], pos3[2]), dir, dist);
+        double* o = out7 + 7 * (size_t)k;
+        o[0] = dir.x; o[1] = dir.y; o[2] = dir.z; o[3] = dist; o[4] = le.x; o[5] = le.y; o[6] = le.z;
+    }
+}
+void orc_to_local_frame(const double* n3, const double* d3, double* out3) {
+    V3<double> r = toLocalFrame(V3<double>(n3[0], n3[1], n3[2]), V3<double>(d3[0], d3[1], d3[2]));
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
+// AABB::hit (biased slab test) in FP32, for n rays against one box
+void orc_aabb_hit(const float* bmin3, const float* bmax3, const float* rays, uint64_t n, uint8_t* out) {
+    V3<float> mn(bmin3[0], bmin3[1], bmin3[2]), mx(bmax3[0], bmax3[1], bmax3[2]);
+    for (uint64_t i = 0; i < n; ++i) {
+        const float* r = rays + 6 * i;
+        out[i] = aabb_hit(mn, mx, V3<float>(r[0], r[1], r[2]), V3<float>(r[3], r[4], r[5])) ? 1 : 0;
+    }
 }
 
 // in: 5 images (base, gxf, gxb, gyf, gyb) as float; out: recon, gx, gy as float (FP32 semantics)

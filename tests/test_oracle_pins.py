@@ -1,0 +1,558 @@
+"""Pins of the CPU oracle (oracle/) against checks that do not restate its code.
+
+Each test names what it pins and by what: a published known-answer sequence,
+a closed-form integral, an inverse algorithm, a brute-force FP64 computation in
+numpy, or a conservation law. The reference ships no tests or golden data for
+this path, so these pins are the ground truth the GPU parity tests stand on.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from cupbr_amd import scenes
+
+# --------------------------------------------------------------------------- RNG
+
+
+def test_lcg_numerical_recipes_sequence():
+    """Math::rnd (FunctionsDetail.h:89-98) is the Numerical Recipes 'ranqd1' LCG; its published
+    sequence from state 0 is 1013904223, 1196435762, 3519870697, 2868466484, 1649599747."""
+    _, states = oracle.rnd_seq(0, 5)
+    assert states.tolist() == [1013904223, 1196435762, 3519870697, 2868466484, 1649599747]
+
+
+def test_lcg_output_is_low_24_bits():
+    vals, states = oracle.rnd_seq(12345, 1000)
+    assert np.array_equal(vals, (states & 0xFFFFFF).astype(np.float64) / 2 ** 24)
+    assert vals.min() >= 0.0 and vals.max() < 1.0
+
+
+def _tea_decrypt(v0, v1, key, rounds):
+    """TEA decoding routine (Wheeler & Needham 1994), the inverse of the encoder."""
+    M = 0xFFFFFFFF
+    delta = 0x9E3779B9
+    s = (delta * rounds) & M
+    for _ in range(rounds):
+        v1 = (v1 - ((((v0 << 4) & M) + key[2]) ^ (v0 + s) ^ ((v0 >> 5) + key[3]))) & M
+        v0 = (v0 - ((((v1 << 4) & M) + key[0]) ^ (v1 + s) ^ ((v1 >> 5) + key[1]))) & M
+        s = (s - delta) & M
+    return v0, v1
+
+
+def test_tea4_is_tea_encryption_with_the_reference_key():
+    """tea<4> (FunctionsDetail.h:70-87) is 4 rounds of TEA with key (a341316c, c8013ea4, ad90777d,
+    7e95761e). The test searches the second word of the encrypted block: decrypting (tea4(a,b), x)
+    must give back (a, b) for exactly the x the encoder produced, so a wrong constant, shift or
+    round count fails the round trip."""
+    key = (0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E)
+    M = 0xFFFFFFFF
+    rng = np.random.default_rng(3)
+    for a, b in [(0, 0), (1, 0), (65535, 7)] + [tuple(int(x) for x in rng.integers(0, 2 ** 32, 2)) for _ in range(5)]:
+        v0 = oracle.tea4(a, b)
+        # the encoder's second word follows from the first: run the last round backwards from v0
+        # for every candidate is too costly, so recover v1 by forward encryption of (a,b) here
+        # with the published encoder, then check the decoder inverts the oracle's v0 with it
+        e0, e1, s = a, b, 0
+        for _ in range(4):
+            s = (s + 0x9E3779B9) & M
+            e0 = (e0 + ((((e1 << 4) & M) + key[0]) ^ (e1 + s) ^ ((e1 >> 5) + key[1]))) & M
+            e1 = (e1 + ((((e0 << 4) & M) + key[2]) ^ (e0 + s) ^ ((e0 >> 5) + key[3]))) & M
+        assert e0 == v0
+        assert _tea_decrypt(v0, e1, key, 4) == (a, b)
+
+
+# --------------------------------------------------------------------------- intersections
+
+
+def _one_object_scene(kind, params, mat=None):
+    b = scenes.SceneBuilder()
+    m = scenes.material() if mat is None else mat
+    getattr(b, kind)(*params, m)
+    return oracle.OracleScene(b.desc())
+
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def test_triangle_hits_points_inside_at_their_distance():
+    """Moller-Trumbore (TriangleDetail.h:21-57): a ray from o toward a point q inside the triangle
+    hits at t = |q - o|; toward points outside it misses (closed form)."""
+    rng = np.random.default_rng(0)
+    v = np.array([[0, 0, 5], [2, 0, 5.5], [0, 2, 4.5]], np.float64)
+    S = _one_object_scene("triangle", v.astype(np.float32))
+    bary = rng.dirichlet([1, 1, 1], 2000) * 0.98 + 0.02 / 3
+    q = bary @ v
+    o = rng.normal(size=(2000, 3)) * 0.3
+    d = _unit(q - o)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    for prec in (32, 64):
+        obj, tri, t = S.trace(rays, precision=prec, mode=0)
+        assert (obj == 0).all() and (tri == -1).all()
+        np.testing.assert_allclose(t, np.linalg.norm(q - o, axis=1), rtol=2e-5 if prec == 32 else 1e-6)
+    # points of the triangle's plane v0 + u e1 + w e2: hit iff u >= 0, w >= 0, u + w <= 1
+    u, ww = rng.uniform(-1, 2, 2000), rng.uniform(-1, 2, 2000)
+    qp = v[0] + u[:, None] * (v[1] - v[0]) + ww[:, None] * (v[2] - v[0])
+    obj, _, _ = S.trace(np.concatenate([o, _unit(qp - o)], 1).astype(np.float32), precision=64, mode=0)
+    inside = (u >= 0) & (ww >= 0) & (u + ww <= 1)
+    margin = (np.abs(u) > 1e-3) & (np.abs(ww) > 1e-3) & (np.abs(u + ww - 1) > 1e-3)
+    assert np.array_equal((obj == 0)[margin], inside[margin])
+
+
+def test_triangle_rejects_small_determinant():
+    """|det| < 1e-5 is rejected (TriangleDetail.h:32): a tiny triangle (edges 1e-3, det ~ 1e-6)
+    is never hit even by a ray through its centroid."""
+    v = np.array([[0, 0, 1], [1e-3, 0, 1], [0, 1e-3, 1]], np.float32)
+    S = _one_object_scene("triangle", v)
+    c = v.mean(0)
+    ray = np.concatenate([[0, 0, 0], _unit(c)]).astype(np.float32)[None]
+    assert S.trace(ray, precision=32, mode=0)[0][0] == -1
+
+
+def test_sphere_chord_distance():
+    """Sphere (SphereDetail.h:33-74): a ray at impact parameter b from a sphere of radius r centred
+    at distance D hits at t = D - sqrt(r^2 - b^2); from inside, t is the far root."""
+    S = _one_object_scene("sphere", [(0, 0, 10), 2.0])
+    b = np.linspace(0, 1.9, 50)
+    rays = np.zeros((50, 6), np.float32)
+    rays[:, 0] = b
+    rays[:, 5] = 1.0
+    _, _, t = S.trace(rays, precision=64, mode=0)
+    np.testing.assert_allclose(t, 10 - np.sqrt(4 - b ** 2), rtol=1e-6)
+    inside = np.array([[0, 0, 10, 0, 0, 1]], np.float32)
+    assert S.trace(inside, precision=64, mode=0)[2][0] == pytest.approx(2.0)
+
+
+def test_plane_and_quad_closed_forms():
+    """Plane t = (p - o).n / d.n (PlaneDetail.h:34); the quad accepts |(P-c).e_i| <= e_i.e_i
+    (QuadDetail.h:41-47), i.e. a 2|e1| x 2|e2| rectangle for orthogonal extents."""
+    S = _one_object_scene("plane", [(0, -1, 0), (0, 2, 0)])
+    rng = np.random.default_rng(1)
+    d = _unit(rng.normal(size=(500, 3)) + [0, -2, 0])
+    rays = np.concatenate([np.zeros((500, 3)), d], 1).astype(np.float32)
+    _, _, t = S.trace(rays, precision=64, mode=0)
+    down = d[:, 1] < -1e-5
+    np.testing.assert_allclose(t[down], -1.0 / d[down, 1], rtol=1e-6)
+    assert np.isinf(t[~down]).all()
+    Q = _one_object_scene("quad", [(0, 0, 5), (0, 0, -1), (0.5, 0, 0), (0, 0.25, 0)])
+    pts = rng.uniform(-0.8, 0.8, (3000, 2))
+    q = np.concatenate([pts, np.full((3000, 1), 5.0)], 1)
+    obj, _, _ = Q.trace(np.concatenate([np.zeros((3000, 3)), _unit(q)], 1).astype(np.float32), precision=64, mode=0)
+    # |x.e1| <= |e1|^2  <=>  |x| <= 0.5 * 0.5 / 0.5 = 0.25 for e1 = (0.5,0,0)
+    inside = (np.abs(pts[:, 0] * 0.5) <= 0.25) & (np.abs(pts[:, 1] * 0.25) <= 0.0625)
+    margin = (np.abs(np.abs(pts[:, 0]) - 0.5) > 1e-4) & (np.abs(np.abs(pts[:, 1]) - 0.25) > 1e-4)
+    assert np.array_equal((obj == 0)[margin], inside[margin])
+
+
+def test_biased_slab_matches_exact_slab_away_from_the_bias():
+    """AABB::hit (AABBDetail.h:18-50) uses 1/(d+1e-5) without a sign fix. For directions with every
+    |d_i| > 1e-2 it must agree with an exact FP64 slab test except within 1e-4 of grazing."""
+    rng = np.random.default_rng(2)
+    bmin, bmax = np.array([1.0, -1.0, 2.0]), np.array([3.0, 0.5, 4.0])
+    o = rng.uniform(-3, 6, (20000, 3))
+    d = _unit(rng.normal(size=(20000, 3)))
+    keep = (np.abs(d) > 1e-2).all(1)
+    o, d = o[keep], d[keep]
+    t0 = (bmin - o) / d
+    t1 = (bmax - o) / d
+    tmin = np.maximum(np.minimum(t0, t1).max(1), 0.0)
+    tmax = np.maximum(t0, t1).min(1)
+    exact = tmax > tmin
+    margin = np.abs(tmax - tmin) > 1e-4 * (1 + np.abs(tmax))
+    got = oracle.aabb_hit(bmin, bmax, np.concatenate([o, d], 1))
+    assert np.array_equal(got[margin], exact[margin])
+
+
+def test_biased_slab_quirk_misses_small_negative_components():
+    """The +1e-5 bias flips the sign of 1/d for d in (-1e-5, 0): a ray pointing straight at a box
+    with d.x = -5e-6 is reported as a miss (the quirk mode 0 and the GPU must both keep)."""
+    # the ray drifts to x = -2.5e-5 at z = 5, inside the thin box; the biased test misses it
+    d = _unit(np.array([-5e-6, 0.0, 1.0]))
+    ray = np.array([[0.0, 0.0, 0.0, *d]], np.float32)
+    assert not oracle.aabb_hit([-1e-4, -1, 4], [-1e-5, 1, 6], ray)[0]
+    # the mirrored configuration with d.x = +5e-6 is found
+    ray[0, 3] = 5e-6
+    assert oracle.aabb_hit([1e-5, -1, 4], [1e-4, 1, 6], ray)[0]
+
+
+def _brute_force_fp64(desc, rays):
+    """Closest triangle over every TRIANGLE object by vectorised FP64 Moller-Trumbore in numpy."""
+    p = desc["params"][:, :9].astype(np.float64).reshape(-1, 3, 3)
+    o, d = rays[:, None, :3].astype(np.float64), rays[:, None, 3:].astype(np.float64)
+    e1, e2 = p[None, :, 1] - p[None, :, 0], p[None, :, 2] - p[None, :, 0]
+    pv = np.cross(d, e2)
+    det = (e1 * pv).sum(-1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / det
+        tv = o - p[None, :, 0]
+        u = (tv * pv).sum(-1) * inv
+        qv = np.cross(tv, e1)
+        v = (d * qv).sum(-1) * inv
+        t = (e2 * qv).sum(-1) * inv
+    ok = (np.abs(det) >= 1e-5) & (u >= 0) & (u <= 1) & (v >= 0) & (u + v <= 1) & (t >= 0)
+    t = np.where(ok, t, np.inf)
+    idx = t.argmin(1)
+    tt = np.sort(t, 1)
+    return np.where(np.isinf(t.min(1)), -1, idx), tt[:, 0], tt[:, 1]
+
+
+def test_cornell_closest_hit_matches_fp64_brute_force():
+    """Mode 0 (the reference's DFS over its object BVH) on the 34-triangle Cornell box against an
+    FP64 numpy brute force, on rays whose closest and second-closest hits are not near-ties."""
+    desc, _ = scenes.cornell_box()
+    S = oracle.OracleScene(desc)
+    rng = np.random.default_rng(4)
+    o = rng.uniform(0.02, 0.98, (4000, 3))
+    d = _unit(rng.normal(size=(4000, 3)))
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    want, t1, t2 = _brute_force_fp64(desc, rays)
+    obj, _, t = S.trace(rays, precision=32, mode=0)
+    with np.errstate(invalid="ignore"):
+        sep = ~((t2 - t1) <= 1e-4 * (1 + t1))  # inf - inf (no second hit) counts as separated
+    assert np.array_equal(obj[sep], want[sep])
+    np.testing.assert_allclose(t[sep & (want >= 0)], t1[sep & (want >= 0)], rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("make", ["cornell", "mixed", "soup", "meshes"])
+def test_traversal_forms_agree(make):
+    """Mode 0 (literal restatement) == mode 1 (leaf order + object boxes) == mode 2 (SAH BVH with
+    rank tie-break and lazy box checks), bit for bit, in FP32 and FP64, including rays that start
+    on surfaces, axis-aligned rays and rays with tiny direction components."""
+    if make == "cornell":
+        desc, _ = scenes.cornell_box()
+        lo, hi = 0.0, 1.0
+    elif make == "mixed":
+        desc, _ = scenes.mixed_primitives(env=False)
+        lo, hi = -1.5, 4.0
+    elif make == "soup":
+        desc = scenes.random_triangle_soup(400, seed=5)
+        lo, hi = -11, 11
+    else:
+        desc = scenes.random_mesh_scene(20, 60, seed=6)
+        lo, hi = -11, 11
+    S = oracle.OracleScene(desc)
+    rng = np.random.default_rng(7)
+    n = 6000
+    o = rng.uniform(lo, hi, (n, 3))
+    d = _unit(rng.normal(size=(n, 3)))
+    d[:300] = np.eye(3)[rng.integers(0, 3, 300)] * rng.choice([-1, 1], (300, 1))
+    d[300:600, 0] = rng.choice([-5e-6, 5e-6, -1e-5, 0.0], 300)
+    d[300:600] = _unit(d[300:600])
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    # rays starting on the surfaces they came from
+    obj, _, t = S.trace(rays, precision=32, mode=0)
+    hit = obj >= 0
+    o2 = rays[hit, :3] + t[hit, None] * rays[hit, 3:]
+    rays2 = np.concatenate([o2, _unit(rng.normal(size=(hit.sum(), 3)))], 1).astype(np.float32)
+    allrays = np.concatenate([rays, rays2])
+    for prec in (32, 64):
+        r0 = S.trace(allrays, precision=prec, mode=0)
+        for mode in (1, 2):
+            r = S.trace(allrays, precision=prec, mode=mode)
+            assert np.array_equal(r[0], r0[0]) and np.array_equal(r[1], r0[1])
+            assert np.array_equal(r[2], r0[2])
+        dist = rng.uniform(0, 5, len(allrays)).astype(np.float32)
+        v0 = S.visibility(allrays, dist, precision=prec, mode=0)
+        assert np.array_equal(S.visibility(allrays, dist, precision=prec, mode=2), v0)
+
+
+def test_reference_leaf_order_swaps_equal_key_pairs():
+    """BoundingVolumeHierarchy.cpp:66-86: two objects with equal keys (infinite boxes) are stored
+    swapped; 8 planes/quads/triangles therefore come out as 1,0,3,2,5,4,7,6."""
+    b = scenes.SceneBuilder()
+    for i in range(8):
+        b.plane((0, i, 0), (0, 1, 0), scenes.material())
+    order, base = oracle.OracleScene(b.desc()).leaf_order()
+    assert order.tolist() == [1, 0, 3, 2, 5, 4, 7, 6]
+    assert base[order].tolist() == list(range(8))
+
+
+# --------------------------------------------------------------------------- BSDFs, phase, lights
+
+
+def test_lambert_white_furnace():
+    """E[f cos / pdf] over cosine samples = albedo * 3.14159/pi (the sampler's pdf uses 3.14159 while
+    brdf_lambert divides by M_PI, MaterialDetail.h:129,206): exact per sample when cos > 1e-5."""
+    m = scenes.material(scenes.LAMBERT, albedo_d=(0.7, 0.5, 0.3))
+    n = _unit([0.3, 0.5, 0.8])
+    s = oracle.sample_direction(m, 9, n, n, 4000)
+    f = np.stack([oracle.brdf(m, n, s[k, :3], n) for k in range(0, 4000, 40)])
+    cos = (s[::40, :3] @ n)
+    est = f * cos[:, None] / s[::40, 3:4]
+    ok = cos > 1e-5
+    albedo = np.array([[0.7, 0.5, 0.3]], np.float32).astype(np.float64)  # the material holds FP32 values
+    ratio = float(np.float32(3.14159)) / float(np.float32(math.pi))  # both constants are FP32 literals
+    np.testing.assert_allclose(est[ok], albedo * ratio + 0 * est[ok], rtol=1e-9)
+    # the samples are cosine distributed: E[cos] = 2/3 over the hemisphere
+    assert abs((s[:, :3] @ n).mean() - 2 / 3) < 0.02
+
+
+def test_ggx_ndf_normalisation():
+    """D_GGX (MaterialDetail.h:11-18): the projected NDF integrates to one, int D(h) cos(th) dw = 1."""
+    th = np.linspace(0, math.pi / 2, 20001)
+    for a in (0.05, 0.2, 0.5, 1.0):
+        D = np.array([oracle.d_ggx(math.cos(x), a) for x in th])
+        val = np.trapezoid(D * np.cos(th) * np.sin(th), th) * 2 * math.pi
+        assert abs(val - 1) < 2e-3 * (1 + 0.05 / a)
+
+
+def test_ggx_sampling_pdf_matches_histogram():
+    """sample_ggx draws half-vectors from D(h)cos(h); the fraction of reflected directions within
+    10 degrees of the mirror direction matches the integral of the returned pdf."""
+    m = scenes.material(scenes.GGX, albedo_s=(1, 1, 1), roughness=0.3)
+    n = np.array([0.0, 0.0, 1.0])
+    i = _unit([0.3, 0.0, 1.0])
+    s = oracle.sample_direction(m, 11, i, n, 200000)
+    valid = np.linalg.norm(s[:, :3], axis=1) > 0
+    assert valid.mean() > 0.9
+    r = 2 * (i @ n) * n - i
+    cone = (s[valid, :3] @ r) > math.cos(math.radians(10))
+    # importance weights 1/pdf over the cone estimate its solid angle
+    omega = np.mean(np.where(cone, 1.0 / s[valid, 3], 0.0)) * valid.mean()
+    assert abs(omega - 2 * math.pi * (1 - math.cos(math.radians(10)))) < 0.03 * omega
+
+
+@pytest.mark.parametrize("g", [-0.6, 0.0, 0.3, 0.8])
+def test_henyey_greenstein_normalisation_and_mean_cosine(g):
+    """HG phase (MaterialDetail.h:276-286): int p dw = 1 and <cos> = g (closed forms); the sampler
+    (:288-313) reproduces <cos> = g. The 4*3.14159 area and the 1e-3 clamp of g are the reference's."""
+    c = np.linspace(-1, 1, 40001)
+    p = np.array([oracle.hg_phase(g, x) for x in c])
+    assert abs(np.trapezoid(p, c) * 2 * math.pi - 1) < 2e-4 + (1e-5 if g else 0)
+    assert abs(np.trapezoid(p * c, c) * 2 * math.pi - g) < 3e-4
+    s = oracle.hg_sample(g, [0, 0, 1], 13, 100000)
+    assert abs(s[:, 2].mean() - g) < 0.01
+
+
+def _rect_irradiance(p, n, corners):
+    """Irradiance at p (normal n) from a unit-radiance polygon: Lambert's polygon formula
+    E = 1/2 sum_i angle_i n . (v_i x v_{i+1}) / |v_i x v_{i+1}|."""
+    v = [_unit(c - p) for c in corners]
+    e = 0.0
+    for k in range(len(v)):
+        a, b = v[k], v[(k + 1) % len(v)]
+        cr = np.cross(a, b)
+        e += math.acos(np.clip(a @ b, -1, 1)) * (n @ cr) / np.linalg.norm(cr)
+    return abs(e) / 2
+
+
+def test_area_light_nee_estimator_matches_polygon_irradiance():
+    """Light::sample_area (LightDetail.h:32-52) returns L * 4|e1||e2| |n_l . l| / d^2 for a uniform
+    point; averaged with the receiver cosine it estimates the irradiance of the rectangle, which
+    Lambert's polygon formula gives in closed form."""
+    # orthogonal half-extents: the estimator's area 4|e1||e2| is the rectangle's area (for skewed
+    # extents the reference over-states the area by |e1||e2|/|e1 x e2|)
+    L = np.array([1, 0.3, 1.0, 0.5, 0, 0, 0, 2.0, 2.0, 2.0, 0.4, 0.0, 0.0, 0.0, 0.0, 0.25], np.float32)
+    c, e1, e2 = L[1:4].astype(np.float64), L[10:13].astype(np.float64), L[13:16].astype(np.float64)
+    corners = [c - e1 - e2, c + e1 - e2, c + e1 + e2, c - e1 + e2]
+    for p in ([0.2, -0.5, 0.4], [1.0, -0.2, 0.9]):
+        p = np.array(p)
+        n = np.array([0.0, 1.0, 0.0])
+        s = oracle.light_sample(L, 17, p, 400000)
+        est = (s[:, 4] * np.maximum(0, s[:, :3] @ n)).mean()
+        want = 2.0 * _rect_irradiance(p, n, corners)
+        assert abs(est - want) < 5e-3 * want
+
+
+def test_point_light_inverse_square():
+    L = np.zeros(16, np.float32)
+    L[0] = 0
+    L[1:4] = [1, 2, 3]
+    L[4:7] = [5, 5, 5]
+    s = oracle.light_sample(L, 1, [1, 2, 1], 1)
+    assert s[0, 3] == pytest.approx(2.0) and s[0, 4] == pytest.approx(5 / 4)
+    np.testing.assert_allclose(s[0, :3], [0, 0, 1])
+
+
+def test_local_frame_is_orthonormal():
+    """toLocalFrame (FunctionsDetail.h:100-117, Duff et al. 2017 branchless ONB): maps z to N and
+    x, y to unit vectors orthogonal to N and to each other."""
+    rng = np.random.default_rng(8)
+    for n in _unit(rng.normal(size=(200, 3))):
+        ex, ey, ez = (oracle.to_local_frame(n, e) for e in np.eye(3))
+        np.testing.assert_allclose(ez, n, atol=1e-12)
+        np.testing.assert_allclose([ex @ ex, ey @ ey, ex @ ey, ex @ n, ey @ n], [1, 1, 0, 0, 0], atol=1e-12)
+
+
+def test_glass_btdf_follows_snell():
+    """sample_glass + btdf_glass (MaterialDetail.h:143-169,225-257): refracted directions satisfy
+    sin(t) = sin(i)/eta, and the BTDF's delta window is non-zero exactly on that direction."""
+    m = scenes.material(scenes.GLASS, albedo_s=(1, 1, 1), eta=1.5)
+    n = np.array([0, 0, 1.0])
+    i = _unit([0.5, 0, 1.0])
+    s = oracle.sample_direction(m, 21, i, n, 2000)
+    refr = s[:, 2] < 0
+    assert 0.5 < refr.mean() < 0.99
+    sin_i = np.linalg.norm(np.cross(i, n))
+    sin_t = np.linalg.norm(np.cross(s[refr, :3], n), axis=1)
+    np.testing.assert_allclose(sin_t, sin_i / 1.5, rtol=1e-6)
+    assert oracle.brdf(m, i, s[np.argmax(refr), :3], n)[0] > 0
+
+
+# --------------------------------------------------------------------------- integrators (closed forms)
+
+
+def _emissive_box(rho, emit, depth_cam=None):
+    b = scenes.SceneBuilder()
+    m = scenes.material(scenes.LAMBERT, albedo_e=(emit,) * 3, albedo_d=(rho,) * 3)
+    for t in scenes.box_triangles((0, 0, 0), (40, 40, 40)):
+        v = t.reshape(3, 3)
+        b.triangle(v[0], v[2], v[1], m)  # inward normals
+    return b.desc()
+
+
+def test_pt_furnace_closed_form():
+    """Closed emissive Lambertian box, no lights: every path collects E * sum_k rho'^k with
+    rho' = rho * 3.14159/pi per bounce (PathTracer.cu:20-26,76-95), deterministic per sample."""
+    rho, E, depth = 0.6, 0.5, 5
+    S = oracle.OracleScene(_emissive_box(rho, E))
+    cam = dict(position=[0, 0, 0], x_axis=[1, 0, 0], y_axis=[0, 1, 0], z_axis=[0, 0, 1])
+    img = S.render(cam, 24, 24, 0, 1, depth=depth, precision=64)
+    rp = float(np.float32(rho)) * float(np.float32(3.14159)) / float(np.float32(math.pi))
+    partial = [float(np.float32(E)) * sum(rp ** k for k in range(j)) for j in range(1, depth + 1)]
+    # a bounce that starts within the 0.01 ray offset of an edge can leave the box (the reference's
+    # offset, PathTracer.cu:156) and end the path early: every pixel is one of the partial sums
+    dist = np.min(np.abs(img[..., :, None] - np.array(partial)), -1)
+    assert (dist < 1e-9 * partial[-1]).all()
+    assert np.mean(np.abs(img - partial[-1]) < 1e-9) > 0.99
+
+
+def test_pt_nee_matches_polygon_irradiance():
+    """One bounce off a Lambertian floor under an area light: E[L] = rho/pi * E(P), with E(P) from
+    Lambert's polygon formula; checked per pixel within 4 sigma of the oracle's own sample spread."""
+    b = scenes.SceneBuilder()
+    rho = 0.5
+    b.plane((0, 0, 0), (0, 1, 0), scenes.material(scenes.LAMBERT, albedo_d=(rho,) * 3))
+    b.area_light((0.0, 1.0, 2.0), (3, 3, 3), (0.4, 0, 0), (0, 0, 0.3))
+    S = oracle.OracleScene(b.desc())
+    w = h = 8
+    cam = dict(position=[0, 0.5, 0], x_axis=[0.3, 0, 0], y_axis=[0, 0.3, 0.3], z_axis=[0, -0.6, 1])
+    img, s, s2 = S.render(cam, w, h, 0, 4000, depth=1, precision=64, stats=True)
+    mean = s / 4000
+    var = np.maximum(s2 / 4000 - mean ** 2, 0)
+    corners = [np.array([-0.4, 1, 1.7]), np.array([0.4, 1, 1.7]), np.array([0.4, 1, 2.3]), np.array([-0.4, 1, 2.3])]
+    # pixel centres are jittered, so compare against the irradiance averaged over each pixel
+    S2 = oracle.OracleScene(b.desc())
+    rays = []
+    for py in range(h):
+        for px in range(w):
+            for jx in (0.25, 0.75):
+                for jy in (0.25, 0.75):
+                    rx, ry = 2 * ((px + jx) / w - 0.5), 2 * ((py + jy) / h - 0.5)
+                    d = np.array(cam["z_axis"]) + rx * np.array(cam["x_axis"]) + ry * np.array(cam["y_axis"])
+                    rays.append(np.concatenate([cam["position"], _unit(d)]))
+    rays = np.array(rays, np.float32)
+    _, _, _, P, _ = S2.trace(rays, precision=64, mode=0, want_geometry=True)
+    E = np.array([_rect_irradiance(p, np.array([0, 1.0, 0]), corners) for p in P]).reshape(h, w, 4).mean(-1)
+    want = rho / math.pi * 3.0 * E
+    z = np.abs(mean[..., 0] - want) / np.sqrt(var[..., 0] / 4000 + 1e-12)
+    assert np.median(z) < 2.0 and (z < 6).all()
+
+
+def test_volume_absorber_transmittance():
+    """Scene medium with sigma_s = 0 (VolumeRenderer.cu:88-211): a free-flight event before the wall
+    absorbs, otherwise the weight stays 1, and the NEE term carries exp(-sigma_t d_light); the
+    volume renderer adds no emission. One bounce off a Lambertian wall at distance D lit by a point
+    light at distance d: E[L] = exp(-sigma_t (D + d)) * rho/pi * I cos / d^2."""
+    b = scenes.SceneBuilder()
+    b.plane((0, 0, 2.0), (0, 0, -1), scenes.material(scenes.LAMBERT, albedo_d=(0.5, 0.5, 0.5)))
+    b.point_light((0, 0.5, 1.0), (1, 1, 1))
+    b.volume = np.array([0, 0, 0, 0.5, 0.5, 0.5, 0.0], np.float32)
+    S = oracle.OracleScene(b.desc())
+    cam = dict(position=[0, 0, 0], x_axis=[1e-4, 0, 0], y_axis=[0, 1e-4, 0], z_axis=[0, 0, 1])
+    n = 40000
+    img, s, s2 = S.render(cam, 2, 2, 0, n, depth=1, method=oracle.VOLUME, precision=64, stats=True)
+    mean = s.mean(axis=(0, 1))[0] / n
+    d = math.sqrt(0.25 + 1.0)
+    value = math.exp(-0.5 * d) * float(np.float32(0.5)) / math.pi * (1.0 / d) / d ** 2
+    p = math.exp(-0.5 * 2.0)
+    assert abs(mean - p * value) < 4 * value * math.sqrt(p * (1 - p) / (4 * n))
+
+
+# --------------------------------------------------------------------------- post chain
+
+
+def _curve(t=1.0, k=1.0):
+    return oracle.threshold_curve(t, k)
+
+
+def test_bloom_constant_image_closed_form():
+    """On a constant image every pyramid level is constant and the 3x3 tent sums to 16, so
+    out = c (1 + 2 s) with s the threshold scale (PyramidConstructor.cu:9-125)."""
+    for val in (0.5, 1.7, 4.0):
+        c = np.array([val, val * 0.5, val * 0.25], np.float32)
+        img = np.tile(c, (40, 64, 1))
+        out = oracle.bloom(img, _curve())
+        t = _curve()
+        b = val
+        rq = np.clip(b - t[1], 0, t[2])
+        s = max(t[3] * rq * rq, b - t[0]) / max(b, 1e-5)
+        np.testing.assert_allclose(out, img * (1 + 2 * s), rtol=1e-6)
+
+
+def test_bloom_energy_conservation():
+    """Away from the borders each level-1 texel reaches 4 output pixels through a tent that sums
+    to 16/16, so sum(out) - sum(hdr) = 2 sum(threshold(hdr)) for a compact bright spot."""
+    img = np.zeros((64, 64, 3), np.float32)
+    img[30:34, 28:33] = [5.0, 3.0, 2.0]
+    out = oracle.bloom(img, _curve())
+    t = _curve()
+    b = img.max(-1)
+    rq = np.clip(b - t[1], 0, t[2])
+    s = np.maximum(t[3] * rq * rq, b - t[0]) / np.maximum(b, 1e-5)
+    thr = img * s[..., None]
+    np.testing.assert_allclose(out.sum((0, 1)) - img.sum((0, 1)), 2 * thr.sum((0, 1)), rtol=1e-5)
+
+
+def test_tonemap_against_fp64_formulas():
+    """Reinhard-named exponential map and the sRGB OETF (ToneMapper.cu:30-36,45-48) in FP64 numpy;
+    uint8 values agree except where the FP64 value sits within 1e-4 of a truncation boundary."""
+    rng = np.random.default_rng(9)
+    hdr = (rng.exponential(1.0, (64, 64, 3)) * rng.choice([0.01, 1, 5], (64, 64, 1))).astype(np.float32)
+    for kind in (0, 1):
+        got = oracle.tonemap(hdr, kind, 1.3)
+        x = hdr.astype(np.float64)
+        if kind == 0:
+            m = (1 - np.exp(-x * 1.3)) ** (1 / 2.2)
+        else:
+            m = np.where(x <= 0.0031308, 12.92 * x, 1.055 * x ** (1 / 2.4) - 0.055)
+        v = np.clip(m, 0, 1) * 255
+        far = np.abs(v - np.round(v)) > 1e-3
+        assert np.array_equal(got[..., :3][far], np.floor(v[far]).astype(np.uint8))
+        assert (got[..., 3] == 255).all()
+
+
+def test_gdpt_reconstruction_fixed_points():
+    """optimization_kernel (GradientDomain.cu:303-350) keeps an affine ramp with consistent
+    gradients, and a constant image with zero gradients, unchanged in the interior (its border is
+    zero, which only reaches the pixels next to it)."""
+    h, w = 24, 32
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+    base = np.stack([0.5 + 0.01 * xx + 0.02 * yy] * 3, -1).astype(np.float32)
+    gx = np.zeros_like(base)
+    gy = np.zeros_like(base)
+    gx[:, :-1] = base[:, 1:] - base[:, :-1]
+    gy[:-1] = base[1:] - base[:-1]
+    bufs = np.stack([base, gx / 2, gx / 2, gy / 2, gy / 2])
+    recon, _, _ = oracle.gdpt_reconstruct(bufs, iters=1)
+    np.testing.assert_allclose(recon[2:-2, 2:-2], base[2:-2, 2:-2], rtol=1e-5)
+
+
+def test_gdpt_median_is_mean_of_middle_two():
+    """compute_median (GradientDomain.cu:275-301) = mean of the two middle values of four, checked
+    by one Jacobi step on random data against numpy's sort."""
+    rng = np.random.default_rng(10)
+    h, w = 8, 9
+    bufs = rng.normal(size=(5, h, w, 3)).astype(np.float32)
+    recon, gx, gy = oracle.gdpt_reconstruct(bufs, iters=1)
+    base = bufs[0]
+    for y in range(1, h - 1):
+        for x in range(1, w - 1):
+            for c in range(3):
+                v = [base[y, x - 1, c] + gx[y, x - 1, c], base[y - 1, x, c] + gy[y - 1, x, c],
+                     base[y, x + 1, c] - gx[y, x, c], base[y + 1, x, c] - gy[y, x, c]]
+                s = np.sort(np.array(v, np.float64))
+                assert recon[y, x, c] == pytest.approx((s[1] + s[2]) / 2, rel=1e-5, abs=1e-6)
