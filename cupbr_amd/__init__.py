@@ -126,6 +126,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise CupbrError(-100, f"{LIB_PATH} is missing: build it with `python -m cupbr_amd.build`")
+        # PyTorch ships its own HIP runtime under the same soname (libamdhip64.so.7). Loading torch
+        # first lets libcupbr bind to that copy, so a process that uses both has one HIP runtime
+        # (two runtimes in one process cannot share the GPU).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         P, U32, I32, SZ = C.c_void_p, C.c_uint32, C.c_int32, C.c_size_t
         sig = {
@@ -143,6 +150,7 @@ def lib():
             "cupbr_scene_get_lights": (C.c_int, [P, P, U32]),
             "cupbr_scene_get_volume": (C.c_int, [P, P]),
             "cupbr_scene_get_bvh_stats": (C.c_int, [P, P]),
+            "cupbr_scene_get_environment": (C.c_int, [P, P, SZ, P, P]),
             "cupbr_scene_get_reference_order": (C.c_int, [P, P, U32]),
             "cupbr_trace_closest": (C.c_int, [P, P, SZ, P, P, P]),
             "cupbr_trace_closest_device": (C.c_int, [P, P, SZ, P, P, P, P]),
@@ -348,6 +356,16 @@ class Scene:
         v = _Volume()
         _check(lib().cupbr_scene_get_volume(self.handle, C.byref(v)))
         return np.array(list(v.sigma_s) + list(v.sigma_a) + [v.g], np.float32)
+
+    def environment(self):
+        """The environment map as float32 [h, w, 3] (row 0 first as in the file), or None."""
+        w, h = C.c_uint32(), C.c_uint32()
+        _check(lib().cupbr_scene_get_environment(self.handle, None, 0, C.byref(w), C.byref(h)))
+        if w.value == 0:
+            return None
+        out = np.zeros((h.value, w.value, 3), np.float32)
+        _check(lib().cupbr_scene_get_environment(self.handle, _ptr(out), out.size, None, None))
+        return out
 
     def bvh_stats(self) -> dict:
         s = _BvhStats()

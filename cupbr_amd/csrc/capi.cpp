@@ -192,6 +192,17 @@ int cupbr_scene_get_volume(const cupbr_scene* s, cupbr_volume* out) {
     return CUPBR_OK;
 }
 
+int cupbr_scene_get_environment(const cupbr_scene* s, float* out, size_t n, uint32_t* w, uint32_t* h) {
+    CHECK_ARG(s, "null scene");
+    if (w) *w = s->env.empty() ? 0 : s->env_w;
+    if (h) *h = s->env.empty() ? 0 : s->env_h;
+    if (out) {
+        CHECK_ARG(n >= s->env.size(), "buffer too small");
+        if (!s->env.empty()) std::memcpy(out, s->env.data(), s->env.size() * sizeof(float));
+    }
+    return CUPBR_OK;
+}
+
 int cupbr_scene_get_bvh_stats(const cupbr_scene* s, cupbr_bvh_stats* out) {
     CHECK_ARG(s && out, "null argument");
     *out = s->stats;

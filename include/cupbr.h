@@ -204,6 +204,10 @@ int cupbr_scene_get_objects(const cupbr_scene* scene, cupbr_object* out, uint32_
 int cupbr_scene_get_triangles(const cupbr_scene* scene, float* out, uint32_t n_triangles);
 int cupbr_scene_get_lights(const cupbr_scene* scene, cupbr_light* out, uint32_t n);
 int cupbr_scene_get_volume(const cupbr_scene* scene, cupbr_volume* out);
+/* Scene::environment (Scene.h:26): width/height (0 without a map) and, when out is not NULL,
+ * the FP32 RGB texels, row 0 first as stored in the file */
+int cupbr_scene_get_environment(const cupbr_scene* scene, float* out, size_t n_floats, uint32_t* width,
+                                uint32_t* height);
 int cupbr_scene_get_bvh_stats(const cupbr_scene* scene, cupbr_bvh_stats* out);
 /* reference BVH leaf order of the objects (BoundingVolumeHierarchy.cpp:46-136); n >= n_objects */
 int cupbr_scene_get_reference_order(const cupbr_scene* scene, int32_t* order, uint32_t n);
