@@ -40,12 +40,15 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run three device-to-device copies of MIB MiB (a plain copy kernel of known "
+                         "size, for scaling the HBM byte counters of a --pmc run)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this renderer core runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sample sharding is described in DESIGN.md, not implemented)"}))
+                          "(sharding samples over GPUs is not implemented)"}))
         sys.exit(2)
 
     import numpy as np
@@ -54,6 +57,17 @@ def main():
     from cupbr_amd import scenes
 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    if args.copy_calibration > 0:
+        import torch
+
+        n = args.copy_calibration << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads n bytes, writes n bytes
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 elementwise copies of {n} bytes")
+        del src, dst
     W, H, SPP = args.width, args.height, args.spp
     t0 = time.time()
     desc, cam_args = scenes.million_triangles(W, H)

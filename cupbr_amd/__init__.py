@@ -167,6 +167,7 @@ def lib():
             "cupbr_renderer_set_max_trace_depth": (C.c_int, [P, U32]),
             "cupbr_renderer_get_max_trace_depth": (C.c_int, [P, P]),
             "cupbr_renderer_get_frame_index": (C.c_int, [P, P]),
+            "cupbr_renderer_get_output_size": (C.c_int, [P, P, P]),
             "cupbr_render": (C.c_int, [P, P, U32]),
             "cupbr_renderer_get_image": (C.c_int, [P, I32, P, SZ]),
             "cupbr_renderer_get_image_device": (C.c_int, [P, I32, P]),

@@ -364,6 +364,13 @@ int cupbr_renderer_get_frame_index(const cupbr_renderer* r, uint32_t* f) {
     return CUPBR_OK;
 }
 
+int cupbr_renderer_get_output_size(const cupbr_renderer* r, uint32_t* width, uint32_t* height) {
+    CHECK_ARG(r && width && height, "null argument");
+    *width = r->output_size_set ? r->width : 0;
+    *height = r->output_size_set ? r->height : 0;
+    return CUPBR_OK;
+}
+
 // PBRenderer::render, PBRenderer.cu:137-230
 int cupbr_render(cupbr_renderer* r, const cupbr_camera* cam, uint32_t spp) {
     CHECK_ARG(r && cam, "null argument");

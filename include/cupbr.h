@@ -238,6 +238,8 @@ int cupbr_renderer_set_russian_roulette(cupbr_renderer* r, int32_t enable); /* s
 int cupbr_renderer_set_max_trace_depth(cupbr_renderer* r, uint32_t depth);  /* setMaxTraceDepth */
 int cupbr_renderer_get_max_trace_depth(const cupbr_renderer* r, uint32_t* depth); /* getMaxTraceDepth */
 int cupbr_renderer_get_frame_index(const cupbr_renderer* r, uint32_t* frame_index);
+/* the size set by setOutputSize (0 x 0 before), for views of the renderer's images */
+int cupbr_renderer_get_output_size(const cupbr_renderer* r, uint32_t* width, uint32_t* height);
 /* render(Camera*), PBRenderer.cu:137-230, repeated spp times: each sample is one reference frame with
  * its own seed tea<4>(pixel, frameIndex) and the running mean of PathTracer.cu:162-168. */
 int cupbr_render(cupbr_renderer* r, const cupbr_camera* camera, uint32_t spp);

@@ -144,7 +144,7 @@ struct Tri {  // a mesh triangle (ObjLoader.cpp:72-76); its _id is never set (hi
     V3<R> v1, v2, v3, n;
 };
 template <typename R>
-struct Hit {  // LocalGeometry, Geometry.h:28-36; tri is this build's addition (index inside a mesh)
+struct Hit {  // LocalGeometry, Geometry.h:28-36; tri (index inside a mesh) is an addition
     R depth = std::numeric_limits<R>::infinity();
     V3<R> P, N;
     int obj = -1;

@@ -264,3 +264,46 @@ def test_host_side_entry_points():
     np.testing.assert_allclose(cb.threshold_curve(1.0, 1.0), [1, 0, 2, 0.25])  # Interactor.cpp:436-440
     with pytest.raises(cb.CupbrError):
         cb.Camera.look_at((0, 0, 0), (0, 0, 1), (0, 1, 0), 200.0, 10, 10)
+
+
+CORNELL_XML = """<header><scene_size>3</scene_size><light_count>1</light_count></header>
+<geometry>
+<geometry1><type>QUAD</type><position>0,-1,2</position><normal>0,1,0</normal><extend1>1,0,0</extend1>
+<extend2>0,0,1</extend2><material><type>LAMBERT</type><albedo_d>0.7,0.7,0.7</albedo_d></material></geometry1>
+<geometry2><type>SPHERE</type><position>0,-0.5,2</position><radius>0.5</radius>
+<material><type>GGX</type><albedo_d>0.2,0.5,0.2</albedo_d><albedo_s>0.9,0.9,0.9</albedo_s>
+<roughness>0.2</roughness></material></geometry2>
+<geometry3><type>TRIANGLE</type><vertex1>-1,-1,3</vertex1><vertex2>1,-1,3</vertex2><vertex3>0,1,3</vertex3>
+<material><type>LAMBERT</type><albedo_d>0.7,0.2,0.2</albedo_d></material></geometry3>
+</geometry>
+<lights>
+<light1><type>AREA</type><position>0,0.9,1.5</position><intensity>0,0,0</intensity><radiance>8,8,8</radiance>
+<extend1>0.3,0,0</extend1><extend2>0,0,0.3</extend2></light1>
+</lights>
+"""
+
+
+def build_cpp_example(tmp_path):
+    """Compile tests/cpp/render_example.cpp (a caller of the reference's class interface)
+    against include/cupbr.hpp and libcupbr.so."""
+    exe = tmp_path / "render_example"
+    lib_dir = os.path.dirname(cb.LIB_PATH)
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", f"-I{ROOT}/include",
+           os.path.join(ROOT, "tests", "cpp", "render_example.cpp"), f"-L{lib_dir}", "-lcupbr",
+           f"-Wl,-rpath,{lib_dir}", "-o", str(exe)]
+    import subprocess
+
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    xml = tmp_path / "scene.xml"
+    xml.write_text(CORNELL_XML)
+    return exe, xml
+
+
+def test_cpp_wrapper_compiles_and_loads_a_scene(tmp_path):
+    import subprocess
+
+    exe, xml = build_cpp_example(tmp_path)
+    out = subprocess.run([str(exe), str(xml), str(tmp_path / "o.bmp"), "--load-only"], capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert "3 objects, 0 triangles, 1 lights" in out.stdout  # mesh triangles only
