@@ -107,6 +107,12 @@ class _RenderStats(C.Structure):
                 ("shadow_node_visits", C.c_uint64), ("shadow_prim_tests", C.c_uint64)]
 
 
+class _GridMedium(C.Structure):
+    _fields_ = [("bmin", C.c_float * 3), ("bmax", C.c_float * 3), ("nx", C.c_uint32), ("ny", C.c_uint32),
+                ("nz", C.c_uint32), ("sigma_t_scale", C.c_float), ("majorant", C.c_float),
+                ("scattering_albedo", C.c_float), ("g", C.c_float), ("min_collision_probability", C.c_float)]
+
+
 # cupbr_object / cupbr_light as numpy records (144 and 64 bytes, no padding)
 OBJECT_DTYPE = np.dtype([
     ("type", "<i4"), ("params", "<f4", 12), ("first_triangle", "<u4"), ("triangle_count", "<u4"),
@@ -193,6 +199,11 @@ def lib():
             "cupbr_tonemapper_get_type": (C.c_int, [P, P]),
             "cupbr_tonemapper_set_exposure": (C.c_int, [P, C.c_float]),
             "cupbr_tonemapper_save_to_file": (C.c_int, [P, C.c_char_p]),
+            "cupbr_grid_surface_count": (C.c_int, [P, P]),
+            "cupbr_grid_render_device": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P]),
+            "cupbr_grid_adjoint_device": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P, P, P]),
+            "cupbr_grid_render": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P]),
+            "cupbr_grid_adjoint": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -419,6 +430,106 @@ class Scene:
         vis = np.zeros(len(rays), np.uint8)
         _check(lib().cupbr_trace_visibility(self.handle, _ptr(rays), _ptr(dist), len(rays), _ptr(vis)))
         return vis.astype(bool)
+
+
+class GridMedium:
+    """cupbr_grid_medium: a box medium whose extinction is sigma_t_scale * density(x), density
+    trilinear between the centres of an [nz, ny, nx] voxel grid (no reference counterpart)."""
+
+    def __init__(self, shape, bmin=(0.0, 0.0, 0.0), bmax=(1.0, 1.0, 1.0), sigma_t_scale=10.0, majorant=10.0,
+                 scattering_albedo=0.9, g=0.6, min_collision_probability=1e-3):
+        nz, ny, nx = shape
+        self.shape = (int(nz), int(ny), int(nx))
+        self._c = _GridMedium((C.c_float * 3)(*bmin), (C.c_float * 3)(*bmax), int(nx), int(ny), int(nz),
+                              sigma_t_scale, majorant, scattering_albedo, g, min_collision_probability)
+
+    @classmethod
+    def from_dict(cls, shape, d):
+        return cls(shape, **d)
+
+
+class GridRenderer:
+    """Heterogeneous-medium path tracer and its path-replay adjoint (cupbr_grid_*).
+
+    numpy arguments go through host memory; torch GPU tensors stay on the device and run on
+    torch's current stream. Images are [h, w, 3] (row 0 = bottom), density [nz, ny, nx],
+    albedo [n_surfaces, 3]."""
+
+    def __init__(self, scene: Scene, camera: Camera, medium: GridMedium, width: int, height: int, spp: int = 1,
+                 max_depth: int = 6, frame0: int = 0):
+        self.scene, self.camera, self.medium = scene, camera, medium
+        self.width, self.height, self.spp, self.max_depth, self.frame0 = width, height, spp, max_depth, frame0
+
+    def surface_count(self) -> int:
+        n = C.c_uint32()
+        _check(lib().cupbr_grid_surface_count(self.scene.handle, C.byref(n)))
+        return n.value
+
+    def _args(self):
+        return (self.scene.handle, C.byref(self.camera._c), C.byref(self.medium._c))
+
+    def _check_sizes(self, density, albedo, proposal=None, d_image=None):
+        """The C ABI takes plain pointers: sizes are checked here before any copy or launch."""
+        nv = self.medium.shape[0] * self.medium.shape[1] * self.medium.shape[2]
+        if _numel(density) != nv or (proposal is not None and _numel(proposal) != nv):
+            raise CupbrError(-13, f"density/proposal must have {nv} values (grid {self.medium.shape})")
+        if _numel(albedo) != 3 * self.surface_count():
+            raise CupbrError(-13, f"albedo must have 3 x {self.surface_count()} values")
+        if d_image is not None and _numel(d_image) != 3 * self.width * self.height:
+            raise CupbrError(-13, f"d_image must have {3 * self.width * self.height} values")
+
+    def _tail(self):
+        return (self.width, self.height, self.frame0, self.spp, self.max_depth)
+
+    def render(self, density, albedo, proposal=None):
+        self._check_sizes(density, albedo, proposal)
+        if _is_torch_gpu(density):
+            import torch
+            rho = density.detach().contiguous().float()
+            alb = albedo.detach().contiguous().float()
+            prop = None if proposal is None else proposal.detach().contiguous().float()
+            img = torch.empty(self.height, self.width, 3, dtype=torch.float32, device=rho.device)
+            stream = torch.cuda.current_stream(rho.device).cuda_stream
+            _check(lib().cupbr_grid_render_device(*self._args(), rho.data_ptr(), None if prop is None else prop.data_ptr(),
+                                                  alb.data_ptr(), *self._tail(), img.data_ptr(), stream))
+            return img
+        rho = np.ascontiguousarray(density, np.float32)
+        alb = np.ascontiguousarray(albedo, np.float32)
+        prop = None if proposal is None else np.ascontiguousarray(proposal, np.float32)
+        img = np.zeros((self.height, self.width, 3), np.float32)
+        _check(lib().cupbr_grid_render(*self._args(), _ptr(rho), _ptr(prop), _ptr(alb), *self._tail(), _ptr(img)))
+        return img
+
+    def adjoint(self, density, albedo, d_image, proposal=None, grad_density=None, grad_albedo=None):
+        """dLoss/d(density), dLoss/d(albedo) for the image cotangent d_image. With torch GPU tensors
+        the gradients are added to grad_density / grad_albedo when given (else new zero tensors)."""
+        self._check_sizes(density, albedo, proposal, d_image)
+        if _is_torch_gpu(density):
+            import torch
+            rho = density.detach().contiguous().float()
+            alb = albedo.detach().contiguous().float()
+            prop = None if proposal is None else proposal.detach().contiguous().float()
+            di = d_image.detach().contiguous().float()
+            gr = torch.zeros_like(rho) if grad_density is None else grad_density
+            ga = torch.zeros_like(alb) if grad_albedo is None else grad_albedo
+            stream = torch.cuda.current_stream(rho.device).cuda_stream
+            _check(lib().cupbr_grid_adjoint_device(*self._args(), rho.data_ptr(),
+                                                   None if prop is None else prop.data_ptr(), alb.data_ptr(),
+                                                   *self._tail(), di.data_ptr(), gr.data_ptr(), ga.data_ptr(), stream))
+            return gr, ga
+        rho = np.ascontiguousarray(density, np.float32)
+        alb = np.ascontiguousarray(albedo, np.float32)
+        prop = None if proposal is None else np.ascontiguousarray(proposal, np.float32)
+        di = np.ascontiguousarray(d_image, np.float32)
+        gr = np.zeros_like(rho)
+        ga = np.zeros_like(alb)
+        _check(lib().cupbr_grid_adjoint(*self._args(), _ptr(rho), _ptr(prop), _ptr(alb), *self._tail(), _ptr(di),
+                                        _ptr(gr), _ptr(ga)))
+        return gr, ga
+
+
+def _numel(x) -> int:
+    return int(x.numel()) if hasattr(x, "numel") and callable(x.numel) else int(np.size(x))
 
 
 def _is_torch_gpu(x) -> bool:

@@ -1,4 +1,5 @@
 // extern "C" entry points (include/cupbr.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -646,3 +647,97 @@ int cupbr_tonemapper_save_to_file(const cupbr_tonemapper* t, const char* path) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ heterogeneous medium + adjoint
+static size_t grid_voxels(const cupbr_grid_medium* m) { return (size_t)m->nx * m->ny * m->nz; }
+static size_t grid_surfaces(const cupbr_scene* s) {
+    size_t n = s->objects.size() + s->triangles.size() / 9;
+    return n;
+}
+
+int cupbr_grid_surface_count(const cupbr_scene* s, uint32_t* n) {
+    CHECK_ARG(s && n, "null argument");
+    *n = (uint32_t)grid_surfaces(s);
+    return CUPBR_OK;
+}
+
+int cupbr_grid_render_device(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m,
+                             const float* density, const float* proposal, const float* albedo, uint32_t w, uint32_t h,
+                             uint32_t frame0, uint32_t spp, uint32_t max_depth, float* image, void* stream) {
+    if (int rc = grid_validate(s, cam, m, w, h, spp, max_depth)) return rc;
+    CHECK_ARG(density && albedo && image, "null buffer");
+    return grid_render_device(s, cam, m, density, proposal, albedo, w, h, frame0, spp, max_depth, image,
+                              (hipStream_t)stream);
+}
+
+int cupbr_grid_adjoint_device(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m,
+                              const float* density, const float* proposal, const float* albedo, uint32_t w,
+                              uint32_t h, uint32_t frame0, uint32_t spp, uint32_t max_depth, const float* d_image,
+                              float* grad_density, float* grad_albedo, void* stream) {
+    if (int rc = grid_validate(s, cam, m, w, h, spp, max_depth)) return rc;
+    CHECK_ARG(density && albedo && d_image && grad_density && grad_albedo, "null buffer");
+    return grid_adjoint_device(s, cam, m, density, proposal, albedo, w, h, frame0, spp, max_depth, d_image,
+                               grad_density, grad_albedo, (hipStream_t)stream);
+}
+
+namespace {
+struct GridHostBufs {
+    DevBuf rho, prop, alb;
+    int upload(const cupbr_scene* s, const cupbr_grid_medium* m, const float* density, const float* proposal,
+               const float* albedo) {
+        const size_t nv = grid_voxels(m) * 4, na = grid_surfaces(s) * 12;
+        int rc;
+        if ((rc = rho.ensure(nv)) || (rc = alb.ensure(std::max<size_t>(na, 4)))) return rc;
+        CUPBR_HIP_CHECK(hipMemcpy(rho.ptr, density, nv, hipMemcpyHostToDevice));
+        if (na) CUPBR_HIP_CHECK(hipMemcpy(alb.ptr, albedo, na, hipMemcpyHostToDevice));
+        if (proposal) {
+            if ((rc = prop.ensure(nv))) return rc;
+            CUPBR_HIP_CHECK(hipMemcpy(prop.ptr, proposal, nv, hipMemcpyHostToDevice));
+        }
+        return CUPBR_OK;
+    }
+};
+}  // namespace
+
+int cupbr_grid_render(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m, const float* density,
+                      const float* proposal, const float* albedo, uint32_t w, uint32_t h, uint32_t frame0,
+                      uint32_t spp, uint32_t max_depth, float* image) {
+    if (int rc = grid_validate(s, cam, m, w, h, spp, max_depth)) return rc;
+    CHECK_ARG(density && (albedo || grid_surfaces(s) == 0) && image, "null buffer");
+    GridHostBufs b;
+    DevBuf img;
+    int rc;
+    if ((rc = b.upload(s, m, density, proposal, albedo)) || (rc = img.ensure((size_t)w * h * 12))) return rc;
+    if ((rc = grid_render_device(s, cam, m, b.rho.as<float>(), proposal ? b.prop.as<float>() : nullptr,
+                                 b.alb.as<float>(), w, h, frame0, spp, max_depth, img.as<float>(), nullptr)))
+        return rc;
+    CUPBR_HIP_CHECK(hipMemcpy(image, img.ptr, (size_t)w * h * 12, hipMemcpyDeviceToHost));
+    return CUPBR_OK;
+}
+
+int cupbr_grid_adjoint(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m, const float* density,
+                       const float* proposal, const float* albedo, uint32_t w, uint32_t h, uint32_t frame0,
+                       uint32_t spp, uint32_t max_depth, const float* d_image, float* grad_density,
+                       float* grad_albedo) {
+    if (int rc = grid_validate(s, cam, m, w, h, spp, max_depth)) return rc;
+    CHECK_ARG(density && (albedo || grid_surfaces(s) == 0) && d_image && grad_density &&
+                  (grad_albedo || grid_surfaces(s) == 0),
+              "null buffer");
+    GridHostBufs b;
+    DevBuf di, gr, ga;
+    const size_t nv = grid_voxels(m) * 4, na = grid_surfaces(s) * 12, ni = (size_t)w * h * 12;
+    int rc;
+    if ((rc = b.upload(s, m, density, proposal, albedo)) || (rc = di.ensure(ni)) || (rc = gr.ensure(nv)) ||
+        (rc = ga.ensure(std::max<size_t>(na, 4))))
+        return rc;
+    CUPBR_HIP_CHECK(hipMemcpy(di.ptr, d_image, ni, hipMemcpyHostToDevice));
+    CUPBR_HIP_CHECK(hipMemset(gr.ptr, 0, nv));
+    CUPBR_HIP_CHECK(hipMemset(ga.ptr, 0, std::max<size_t>(na, 4)));
+    if ((rc = grid_adjoint_device(s, cam, m, b.rho.as<float>(), proposal ? b.prop.as<float>() : nullptr,
+                                  b.alb.as<float>(), w, h, frame0, spp, max_depth, di.as<float>(), gr.as<float>(),
+                                  ga.as<float>(), nullptr)))
+        return rc;
+    CUPBR_HIP_CHECK(hipMemcpy(grad_density, gr.ptr, nv, hipMemcpyDeviceToHost));
+    if (na) CUPBR_HIP_CHECK(hipMemcpy(grad_albedo, ga.ptr, na, hipMemcpyDeviceToHost));
+    return CUPBR_OK;
+}

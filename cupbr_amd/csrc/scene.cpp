@@ -162,6 +162,7 @@ int scene_finalize(cupbr_scene* s) {
         for (int k = 0; k < 3; ++k) { oi.bmin[k] = mn[k]; oi.bmax[k] = mx[k]; }
         oi.type = s->objects[i].type;
         oi.finite_box = fin ? 1u : 0u;
+        oi.first_triangle = s->objects[i].type == CUPBR_MESH ? s->objects[i].first_triangle : 0u;
     }
     s->ref_order.clear();
     std::vector<int32_t> all(n_obj);

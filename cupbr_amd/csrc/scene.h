@@ -35,6 +35,8 @@ struct cupbr_scene {
     std::vector<void*> allocations;
     int device = 0;
     cupbr::DevBuf spill;               // traversal stack spill for the batch query kernels
+    cupbr::DevBuf grid_spill;          // traversal stack spill of the grid-medium kernels
+    cupbr::DevBuf grid_samples;        // per-sample radiance of the grid-medium forward pass
 };
 
 namespace cupbr {
@@ -58,5 +60,16 @@ int trace_closest_device(const cupbr_scene* s, const float* rays, size_t n, int3
                          hipStream_t stream);
 int trace_visibility_device(const cupbr_scene* s, const float* rays, const float* dist, size_t n, uint8_t* vis,
                             hipStream_t stream);
+
+// heterogeneous medium + adjoint (grid.hip)
+int grid_validate(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m, uint32_t w, uint32_t h,
+                  uint32_t spp, uint32_t max_depth);
+int grid_render_device(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m, const float* rho,
+                       const float* prop, const float* alb, uint32_t w, uint32_t h, uint32_t frame0, uint32_t spp,
+                       uint32_t max_depth, float* image, hipStream_t stream);
+int grid_adjoint_device(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m, const float* rho,
+                        const float* prop, const float* alb, uint32_t w, uint32_t h, uint32_t frame0, uint32_t spp,
+                        uint32_t max_depth, const float* d_image, float* grad_rho, float* grad_alb,
+                        hipStream_t stream);
 
 }  // namespace cupbr

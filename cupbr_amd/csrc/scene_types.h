@@ -65,7 +65,8 @@ struct ObjInfo {
     uint32_t rank_base;
     uint32_t finite_box;
     uint32_t rank_count;  // primitives of the object (triangles of a mesh, else 1)
-    uint32_t pad[2];
+    uint32_t first_triangle;  // mesh: index of its first triangle in the scene's triangle array
+    uint32_t pad;
 };
 
 // Everything a kernel needs, passed by value.

@@ -324,3 +324,44 @@ def random_mesh_scene(n_meshes=50, tris_per_mesh=200, seed=0, extent=10.0):
             b.sphere(rng.uniform(-extent, extent, 3), rng.uniform(0.5, 2.0), material(GGX, albedo_s=(0.5, 0.5, 0.5), roughness=0.3))
     b.point_light((0, extent * 2, 0), (100, 100, 100))
     return b.desc()
+
+
+# ------------------------------------------------------------------ heterogeneous medium (adjoint workload)
+def grid_medium_scene(res=64, width=256, height=256, seed=3, target_seed=4, target_sigma=0.05):
+    """Unit-cube medium in an open room of 10 Lambertian triangles, lit by one area light.
+
+    density: fBm (seed 3, 4 octaves at 4 cells per unit) normalised to [0, 1] on a res^3 grid;
+    sigma_t = 10 * density, single-scattering albedo 0.9, HG g = 0.6, majorant 10.
+    target: the same grid plus N(0, 0.05^2) noise (seed 4), clipped at 0 — the density an
+    inverse-rendering test tries to recover. Surface albedos are the 10 walls' (the parameters
+    of the per-surface gradient). Returns a dict with desc, camera args, medium, density,
+    target_density and albedo ([n_surfaces, 3]: one row per scene object)."""
+    b = SceneBuilder()
+    P = np.array
+    lo, hi = -0.6, 1.6
+    walls = [  # (corner, u, v): normal u x v points into the room
+        (P([lo, lo, lo]), P([0, 0, hi - lo]), P([hi - lo, 0, 0])),   # floor
+        (P([lo, hi, lo]), P([hi - lo, 0, 0]), P([0, 0, hi - lo])),   # ceiling
+        (P([lo, lo, hi]), P([0, hi - lo, 0]), P([hi - lo, 0, 0])),   # back
+        (P([lo, lo, lo]), P([0, hi - lo, 0]), P([0, 0, hi - lo])),   # left
+        (P([hi, lo, lo]), P([0, 0, hi - lo]), P([0, hi - lo, 0])),   # right
+    ]
+    base = [(0.75, 0.75, 0.75), (0.75, 0.75, 0.75), (0.6, 0.6, 0.6), (0.65, 0.12, 0.1), (0.12, 0.5, 0.15)]
+    albedo = []
+    for (c, u, v), a in zip(walls, base):
+        m = material(LAMBERT, albedo_d=a)
+        b.triangle(c, c + u, c + u + v, m)
+        b.triangle(c, c + u + v, c + v, m)
+        albedo += [a, a]
+    b.area_light((0.5, hi - 0.15, 0.5), (12, 12, 12), (0.35, 0, 0), (0, 0, 0.35))
+    g = (np.arange(res) + 0.5) / res * 4.0
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    d = fbm3(np.stack([x, y, z], -1), seed)
+    d = (d - d.min()) / (d.max() - d.min())
+    rng = np.random.default_rng(target_seed)
+    target = np.maximum(d + rng.normal(0.0, target_sigma, d.shape), 0.0)
+    medium = dict(bmin=(0.0, 0.0, 0.0), bmax=(1.0, 1.0, 1.0), sigma_t_scale=10.0, majorant=10.0,
+                  scattering_albedo=0.9, g=0.6, min_collision_probability=1e-3)
+    cam = dict(eye=(0.5, 0.5, -1.9), target=(0.5, 0.5, 0.5), up=(0, 1, 0), vfov=45.0, width=width, height=height)
+    return dict(desc=b.desc(), camera=cam, medium=medium, density=d.astype(np.float32),
+                target_density=target.astype(np.float32), albedo=np.array(albedo, np.float32))

@@ -279,6 +279,49 @@ int cupbr_tonemapper_set_exposure(cupbr_tonemapper* t, float exposure);
 /* saveToFile(path): 32-bit BMP, rows flipped so row 0 of the image is the bottom (ToneMapper.cu:183-198) */
 int cupbr_tonemapper_save_to_file(const cupbr_tonemapper* t, const char* path);
 
+/* ------------------------------------------------------------------ heterogeneous medium + adjoint
+ * No counterpart in the reference (its volumes are homogeneous: Volume.h:15-21,
+ * VolumeRenderer.cu:97-124). A path tracer for scenes whose primitives are two-sided Lambertian
+ * surfaces with one RGB albedo parameter each, inside and around a box medium with a voxel density
+ * grid, plus its gradient by path replay: the adjoint re-runs every path from its seed instead of
+ * storing it. Camera, seeding (tea<4>(pixel, frame0 + sample) + LCG), light sampling and HG phase
+ * follow the reference's definitions.
+ *
+ * Surface index: object id for spheres/planes/quads/triangles; n_objects + global triangle index
+ * for a mesh triangle (cupbr_grid_surface_count gives n_objects + n_triangles).
+ * Images are the mean over spp samples, FP32 RGB, row 0 = bottom. */
+typedef struct cupbr_grid_medium {
+    float bmin[3], bmax[3];     /* medium box */
+    uint32_t nx, ny, nz;        /* voxels; density(x) is trilinear between voxel centres, x fastest */
+    float sigma_t_scale;        /* sigma_t(x) = sigma_t_scale * density(x) */
+    float majorant;             /* tracking rate; sigma_t above it gives negative null weights */
+    float scattering_albedo;    /* sigma_s / sigma_t */
+    float g;                    /* Henyey-Greenstein asymmetry, (-1, 1) */
+    float min_collision_probability; /* real-collision probability is clamped to [p, 1 - p], p in (0, 0.5] */
+} cupbr_grid_medium;
+
+int cupbr_grid_surface_count(const cupbr_scene* scene, uint32_t* n_surfaces);
+/* Forward render. density / proposal: nx*ny*nz floats; proposal NULL = density (the fixed grid that
+ * drives real/null decisions); surface_albedo: 3 floats per surface; image: width*height*3 floats. */
+int cupbr_grid_render_device(const cupbr_scene* scene, const cupbr_camera* camera, const cupbr_grid_medium* medium,
+                             const float* density, const float* proposal, const float* surface_albedo,
+                             uint32_t width, uint32_t height, uint32_t frame0, uint32_t spp, uint32_t max_depth,
+                             float* image, void* hip_stream);
+/* Adjoint: ADDS dLoss/d(density) and dLoss/d(surface_albedo) for the image's cotangent d_image
+ * (width*height*3 floats) to grad_density and grad_albedo (same sizes as density and albedo). */
+int cupbr_grid_adjoint_device(const cupbr_scene* scene, const cupbr_camera* camera, const cupbr_grid_medium* medium,
+                              const float* density, const float* proposal, const float* surface_albedo,
+                              uint32_t width, uint32_t height, uint32_t frame0, uint32_t spp, uint32_t max_depth,
+                              const float* d_image, float* grad_density, float* grad_albedo, void* hip_stream);
+/* host-pointer variants (copy through device memory; the adjoint OVERWRITES the gradient outputs) */
+int cupbr_grid_render(const cupbr_scene* scene, const cupbr_camera* camera, const cupbr_grid_medium* medium,
+                      const float* density, const float* proposal, const float* surface_albedo, uint32_t width,
+                      uint32_t height, uint32_t frame0, uint32_t spp, uint32_t max_depth, float* image);
+int cupbr_grid_adjoint(const cupbr_scene* scene, const cupbr_camera* camera, const cupbr_grid_medium* medium,
+                       const float* density, const float* proposal, const float* surface_albedo, uint32_t width,
+                       uint32_t height, uint32_t frame0, uint32_t spp, uint32_t max_depth, const float* d_image,
+                       float* grad_density, float* grad_albedo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,8 @@ def lib():
         L.orc_gdpt.argtypes = [P, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
                                C.c_int, P]
         L.orc_gdpt_reconstruct.argtypes = [C.c_uint32, C.c_uint32, P, C.c_int, P, P, P]
+        L.orc_grid.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                               C.c_uint32, C.c_int, P, P, P, P]
         L.orc_bloom.argtypes = [P, C.c_uint32, C.c_uint32, P, P]
         L.orc_tonemap.argtypes = [P, C.c_uint64, C.c_int, C.c_float, P]
         L.orc_brdf.argtypes = [P, P, P, P, P]
@@ -178,6 +180,36 @@ class OracleScene:
         camf = camera_array(cam)
         lib().orc_gdpt(self.h, precision, _p(camf), w, h, frame0, nframes, depth, mode, threads, _p(bufs))
         return bufs
+
+    def grid(self, cam, medium, density, albedo, w, h, frame0=0, spp=1, depth=6, proposal=None, d_image=None,
+             precision=64, mode=2, threads=0):
+        """Heterogeneous-medium integrator (restating cupbr_amd/csrc/grid.hip).
+
+        medium: dict with bmin, bmax, sigma_t_scale, majorant, scattering_albedo, g,
+        min_collision_probability; density [nz, ny, nx]; albedo [n_surfaces, 3].
+        Returns the image [h, w, 3] (mean of frames frame0..frame0+spp-1), or with d_image
+        (dLoss/dImage) the gradients (d_density [nz, ny, nx], d_albedo [n_surfaces, 3])."""
+        self._need(precision)
+        rho = _c(density, np.float64)
+        nz, ny, nx = rho.shape
+        med = np.array(list(medium["bmin"]) + list(medium["bmax"]) +
+                       [medium["sigma_t_scale"], medium["majorant"], medium["scattering_albedo"], medium["g"],
+                        medium["min_collision_probability"]], np.float32)
+        dims = np.array([nx, ny, nz], np.uint32)
+        prop = None if proposal is None else _c(proposal, np.float64)
+        alb = _c(albedo, np.float64)
+        camf = camera_array(cam)
+        if d_image is None:
+            img = np.zeros((h, w, 3), np.float64)
+            lib().orc_grid(self.h, precision, mode, _p(camf), _p(med), _p(dims), _p(rho), _p(prop), _p(alb), w, h,
+                           frame0, spp, depth, threads, _p(img), None, None, None)
+            return img
+        di = _c(d_image, np.float64)
+        g_rho = np.zeros_like(rho)
+        g_alb = np.zeros_like(alb)
+        lib().orc_grid(self.h, precision, mode, _p(camf), _p(med), _p(dims), _p(rho), _p(prop), _p(alb), w, h,
+                       frame0, spp, depth, threads, None, _p(di), _p(g_rho), _p(g_alb))
+        return g_rho, g_alb
 
 
 def gdpt_reconstruct(bufs5, iters=50):

@@ -16,6 +16,7 @@
 #include <atomic>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <thread>
 
 namespace orc {
@@ -882,6 +883,239 @@ static SceneT<R>* make_scene(const int32_t* types, const float* params, const ui
     return S;
 }
 
+
+// ------------------------------------------------------------------ heterogeneous medium + adjoint
+// The reference has no heterogeneous medium (Volume.h:15-21); this restates the renderer core's
+// grid integrator (cupbr_amd/csrc/grid.hip) from its definition: weighted delta tracking with a
+// fixed proposal grid, ratio-tracking NEE, Lambertian surfaces with per-surface albedo, and the
+// path-replay adjoint. Same operation order as the HIP code so that R=float agrees path for path.
+template <typename R>
+struct GridT {
+    const double* rho;
+    const double* prop;
+    const double* alb;
+    R bmin[3], ext[3];
+    uint32_t n[3];
+    R scale, maj, alpha, g, eps;
+    uint32_t n_objs;
+};
+
+static const uint32_t kGridMaxSteps = 1u << 22;  // tentative collisions per segment (path ends there)
+
+struct GridAcc {  // per-thread gradient accumulators
+    std::vector<double> rho, alb;
+};
+
+template <typename R>
+struct StencilT {
+    uint32_t idx[8];
+    R w[8];
+};
+
+template <typename R>
+static inline void grid_stencil(const GridT<R>& G, const V3<R>& x, StencilT<R>& s) {
+    uint32_t i0[3], i1[3];
+    R f[3];
+    for (int a = 0; a < 3; ++a) {
+        const R u = (x[a] - G.bmin[a]) / G.ext[a] * (R)(float)G.n[a] - (R)0.5f;
+        const R fl = std::floor(u);
+        f[a] = u - fl;
+        const int lo = (int)fl, hi_max = (int)G.n[a] - 1;
+        i0[a] = (uint32_t)std::min(std::max(lo, 0), hi_max);
+        i1[a] = (uint32_t)std::min(std::max(lo + 1, 0), hi_max);
+    }
+    for (int k = 0; k < 8; ++k) {
+        const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+        const uint32_t ix = dx ? i1[0] : i0[0], iy = dy ? i1[1] : i0[1], iz = dz ? i1[2] : i0[2];
+        s.idx[k] = (iz * G.n[1] + iy) * G.n[0] + ix;
+        s.w[k] = ((dx ? f[0] : (R)1 - f[0]) * (dy ? f[1] : (R)1 - f[1])) * (dz ? f[2] : (R)1 - f[2]);
+    }
+}
+
+template <typename R>
+static inline R grid_value(const double* g, const StencilT<R>& s) {
+    R v = 0;
+    for (int k = 0; k < 8; ++k) v += s.w[k] * (R)g[s.idx[k]];
+    return v;
+}
+
+template <typename R>
+static inline void grid_scatter(std::vector<double>& grad, const StencilT<R>& s, R c) {
+    if (c == (R)0 || !(std::abs(c) < std::numeric_limits<R>::infinity())) return;
+    for (int k = 0; k < 8; ++k)
+        if (s.w[k] != (R)0) grad[s.idx[k]] += (double)(c * s.w[k]);
+}
+
+template <typename R>
+static inline bool grid_clip(const GridT<R>& G, const V3<R>& o, const V3<R>& d, R tmax, R& t0, R& t1) {
+    t0 = 0;
+    t1 = tmax;
+    for (int a = 0; a < 3; ++a) {
+        const R lo = G.bmin[a], hi = G.bmin[a] + G.ext[a];
+        if (d[a] == (R)0) {
+            if (o[a] < lo || o[a] > hi) return false;
+            continue;
+        }
+        R ta = (lo - o[a]) / d[a], tb = (hi - o[a]) / d[a];
+        if (ta > tb) std::swap(ta, tb);
+        t0 = std::fmax(t0, ta);
+        t1 = std::fmin(t1, tb);
+    }
+    return t0 < t1;
+}
+
+template <typename R>
+static inline R grid_step(const GridT<R>& G, R t, uint32_t& seed) {
+    return t - std::log((R)1.0f - (R)rnd(seed)) / G.maj;
+}
+
+template <typename R>
+static R grid_ratio_track(const GridT<R>& G, const V3<R>& o, const V3<R>& d, R dist, uint32_t& seed,
+                          std::vector<double>* grad, R coef) {
+    R t0, t1;
+    if (!grid_clip(G, o, d, dist, t0, t1)) return (R)1;
+    const R len = t1 - t0;
+    R T = 1, u = 0;
+    for (uint32_t it = 0;; ++it) {
+        if (it == kGridMaxSteps) return (R)0;
+        u = grid_step(G, u, seed);
+        if (!(u < len)) break;
+        StencilT<R> s;
+        grid_stencil(G, o + (t0 + u) * d, s);
+        const R sigma = G.scale * grid_value(G.rho, s);
+        if (grad) grid_scatter(*grad, s, coef * ((R)-1.0f / (G.maj - sigma)) * G.scale);
+        else T = T * ((R)1.0f - sigma / G.maj);
+    }
+    return T;
+}
+
+template <typename R>
+struct GridAdj {
+    V3<R> delta, rem;
+    GridAcc* acc;
+};
+
+template <typename R>
+static bool grid_nee(const SceneT<R>& S, const GridT<R>& G, const Tracer<R>& T, const V3<R>& x, uint32_t& seed,
+                     V3<R>& ldir, V3<R>& le, R& tr, uint32_t& tseed, R& dist, bool surface, const V3<R>& N) {
+    const uint32_t nl = (uint32_t)S.lights.size();
+    if (nl == 0) return false;
+    uint32_t k = (uint32_t)(rnd(seed) * (float)nl);
+    k = std::min(k, nl - 1);
+    le = light_sample(S.lights[k], seed, x, ldir, dist) * (R)(float)nl;
+    if (surface && !(dot(N, ldir) > (R)0)) return false;
+    const V3<R> so = surface ? x + (R)1e-4f * N : x;
+    if (!T.visible(so, ldir, dist)) return false;
+    tseed = seed;
+    tr = grid_ratio_track(G, so, ldir, dist, seed, nullptr, (R)0);
+    return true;
+}
+
+template <typename R, bool kAdj>
+static V3<R> grid_path(const SceneT<R>& S, const GridT<R>& G, const Tracer<R>& T, const Camera<R>& cam, uint32_t px,
+                       uint32_t py, uint32_t w, uint32_t h, uint32_t frame, uint32_t max_depth, GridAdj<R>* A) {
+    const R pi = (R)kPiD;
+    const R inf = std::numeric_limits<R>::infinity();
+    uint32_t seed = tea4(py * w + px, frame);
+    V3<R> o, d;
+    launch_ray(px, py, w, h, cam, seed, o, d);
+    V3<R> L(0), beta(1);
+    for (uint32_t depth = 0; depth < max_depth; ++depth) {
+        Hit<R> hr = T.closest(o, d);
+        const bool hit = hr.obj >= 0;
+        const R ts = hit ? hr.depth : inf;
+        bool scattered = false;
+        V3<R> x;
+        R t0, t1;
+        if (grid_clip(G, o, d, ts, t0, t1)) {
+            const R len = t1 - t0;
+            R u = 0;
+            for (uint32_t it = 0;; ++it) {
+                if (it == kGridMaxSteps) return L;
+                u = grid_step(G, u, seed);
+                if (!(u < len)) break;
+                x = o + (t0 + u) * d;
+                StencilT<R> s;
+                grid_stencil(G, x, s);
+                const R sigma = G.scale * grid_value(G.rho, s);
+                const R P = clampf(G.scale * grid_value(G.prop, s) / G.maj, G.eps, (R)1.0f - G.eps);
+                R f, dlog;
+                const bool real = (R)rnd(seed) < P;
+                if (real) {
+                    f = G.alpha * sigma / (G.maj * P);
+                    dlog = (R)1.0f / sigma;
+                } else {
+                    f = ((R)1.0f - sigma / G.maj) / ((R)1.0f - P);
+                    dlog = (R)-1.0f / (G.maj - sigma);
+                }
+                if (f == (R)0) return L;
+                beta = beta * f;
+                if (kAdj) grid_scatter(A->acc->rho, s, dot(A->delta, A->rem) * dlog * G.scale);
+                if (real) {
+                    scattered = true;
+                    break;
+                }
+            }
+        }
+        if (scattered) {
+            V3<R> ldir, le;
+            R tr, dist;
+            uint32_t tseed;
+            if (grid_nee(S, G, T, x, seed, ldir, le, tr, tseed, dist, false, V3<R>(0))) {
+                const R ph = hg_phase(G.g, dot(d, ldir));
+                const V3<R> C = beta * (le * (ph * tr));
+                L += C;
+                if (kAdj) {
+                    const R c = dot(A->delta, C);
+                    if (c != (R)0) grid_ratio_track(G, x, ldir, dist, tseed, &A->acc->rho, c);
+                    A->rem = A->rem - C;
+                }
+            }
+            if (depth + 1 == max_depth) break;
+            V4<R> hs = hg_sample(G.g, d, seed);
+            o = x;
+            d = hs.v;
+            continue;
+        }
+        if (!hit) break;
+        const Object<R>& ob = S.objs[hr.obj];
+        const uint32_t surf = ob.type == MESH ? G.n_objs + ob.first + (uint32_t)hr.tri : (uint32_t)hr.obj;
+        V3<R> N = hr.N;
+        if (dot(N, d) > (R)0) N = (R)-1.0f * N;
+        const V3<R> P = hr.P;
+        const V3<R> a((R)G.alb[3 * surf], (R)G.alb[3 * surf + 1], (R)G.alb[3 * surf + 2]);
+        V3<R> ldir, le;
+        R tr, dist;
+        uint32_t tseed;
+        if (grid_nee(S, G, T, P, seed, ldir, le, tr, tseed, dist, true, N)) {
+            const V3<R> E0 = le * (dot(N, ldir) * tr / pi);
+            const V3<R> C = beta * (a * E0);
+            L += C;
+            if (kAdj) {
+                const V3<R> ga = A->delta * beta * E0;
+                for (int c = 0; c < 3; ++c) A->acc->alb[3 * surf + c] += (double)ga[c];
+                const R cc = dot(A->delta, C);
+                if (cc != (R)0) grid_ratio_track(G, P + (R)1e-4f * N, ldir, dist, tseed, &A->acc->rho, cc);
+                A->rem = A->rem - C;
+            }
+        }
+        if (depth + 1 == max_depth) break;
+        const R u1 = (R)rnd(seed), u2 = (R)rnd(seed);
+        const R r = std::sqrt(u1), phi = (R)2.0f * pi * u2;
+        const V3<R> nd = normalize(toLocalFrame(N, V3<R>(r * std::cos(phi), r * std::sin(phi), std::sqrt(std::fmax((R)0, (R)1.0f - u1)))));
+        if (a.x == (R)0 && a.y == (R)0 && a.z == (R)0) break;
+        beta = beta * a;
+        if (kAdj) {
+            for (int c = 0; c < 3; ++c)
+                if (a[c] > (R)0 && A->delta[c] * A->rem[c] != (R)0)
+                    A->acc->alb[3 * surf + c] += (double)(A->delta[c] * A->rem[c] / a[c]);
+        }
+        o = P + (R)1e-4f * N;
+        d = nd;
+    }
+    return L;
+}
+
 }  // namespace orc
 
 // ======================================================================= C ABI
@@ -990,6 +1224,72 @@ static void gdpt_impl(const SceneT<R>& S, const float* camf, uint32_t w, uint32_
     });
 }
 
+// grid medium: image = mean over frames frame0..frame0+spp-1 (summed in order, then divided)
+template <typename R>
+static GridT<R> read_grid(const SceneT<R>& S, const float* med, const uint32_t* dims, const double* rho,
+                          const double* prop, const double* alb) {
+    GridT<R> G;
+    G.rho = rho;
+    G.prop = prop ? prop : rho;
+    G.alb = alb;
+    for (int a = 0; a < 3; ++a) {
+        G.bmin[a] = (R)med[a];
+        G.ext[a] = (R)med[3 + a] - (R)med[a];
+        G.n[a] = dims[a];
+    }
+    G.scale = (R)med[6];
+    G.maj = (R)med[7];
+    G.alpha = (R)med[8];
+    G.g = (R)med[9];
+    G.eps = (R)med[10];
+    G.n_objs = (uint32_t)S.objs.size();
+    return G;
+}
+
+template <typename R>
+static void grid_impl(const SceneT<R>& S, int mode, const float* camf, const float* med, const uint32_t* dims,
+                      const double* rho, const double* prop, const double* alb, uint32_t w, uint32_t h, uint32_t frame0,
+                      uint32_t spp, uint32_t depth, int threads, double* img, const double* d_img, double* g_rho,
+                      double* g_alb, size_t n_surf) {
+    Tracer<R> T{S, mode};
+    Camera<R> cam = read_cam<R>(camf);
+    const GridT<R> G = read_grid<R>(S, med, dims, rho, prop, alb);
+    const size_t n = (size_t)w * h, nvox = (size_t)dims[0] * dims[1] * dims[2];
+    std::mutex mu;
+    parallel_for(n, threads, [&](size_t lo, size_t hi) {
+        GridAcc acc;
+        if (d_img) {
+            acc.rho.assign(nvox, 0.0);
+            acc.alb.assign(3 * n_surf, 0.0);
+        }
+        for (size_t i = lo; i < hi; ++i) {
+            const uint32_t px = (uint32_t)(i % w), py = (uint32_t)(i / w);
+            V3<R> sum(0);
+            for (uint32_t s = 0; s < spp; ++s) {
+                V3<R> L = grid_path<R, false>(S, G, T, cam, px, py, w, h, frame0 + s, depth, nullptr);
+                sum += L;
+                if (d_img) {
+                    GridAdj<R> A;
+                    A.delta = V3<R>((R)d_img[3 * i], (R)d_img[3 * i + 1], (R)d_img[3 * i + 2]) / (R)(float)spp;
+                    A.rem = L;
+                    A.acc = &acc;
+                    if (A.delta.x != (R)0 || A.delta.y != (R)0 || A.delta.z != (R)0)
+                        grid_path<R, true>(S, G, T, cam, px, py, w, h, frame0 + s, depth, &A);
+                }
+            }
+            if (img) {
+                const V3<R> m = sum / (R)(float)spp;
+                for (int c = 0; c < 3; ++c) img[3 * i + c] = (double)m[c];
+            }
+        }
+        if (d_img) {
+            std::lock_guard<std::mutex> lk(mu);
+            for (size_t k = 0; k < nvox; ++k) g_rho[k] += acc.rho[k];
+            for (size_t k = 0; k < 3 * n_surf; ++k) g_alb[k] += acc.alb[k];
+        }
+    });
+}
+
 extern "C" {
 
 // precisions: bit 0 builds the FP32 scene, bit 1 the FP64 scene
@@ -1058,6 +1358,22 @@ void orc_gdpt(void* p, int precision, const float* cam, uint32_t w, uint32_t h, 
     auto* s = (OrcScene*)p;
     if (precision == 64) gdpt_impl(*s->d, cam, w, h, frame0, nframes, depth, mode, threads, bufs);
     else gdpt_impl(*s->f, cam, w, h, frame0, nframes, depth, mode, threads, bufs);
+}
+
+// grid medium forward (img) and adjoint (d_img -> g_rho, g_alb; accumulated). med: bmin[3], bmax[3],
+// sigma_t_scale, majorant, scattering_albedo, g, min_collision_probability; dims: nx, ny, nz.
+void orc_grid(void* p, int precision, int mode, const float* cam, const float* med, const uint32_t* dims,
+              const double* rho, const double* prop, const double* alb, uint32_t w, uint32_t h, uint32_t frame0,
+              uint32_t spp, uint32_t depth, int threads, double* img, const double* d_img, double* g_rho,
+              double* g_alb) {
+    auto* s = (OrcScene*)p;
+    if (precision == 64) {
+        size_t ns = s->d->objs.size() + s->d->tris.size();
+        grid_impl(*s->d, mode, cam, med, dims, rho, prop, alb, w, h, frame0, spp, depth, threads, img, d_img, g_rho, g_alb, ns);
+    } else {
+        size_t ns = s->f->objs.size() + s->f->tris.size();
+        grid_impl(*s->f, mode, cam, med, dims, rho, prop, alb, w, h, frame0, spp, depth, threads, img, d_img, g_rho, g_alb, ns);
+    }
 }
 
 // ---- probes of single functions (FP64), for the closed-form checks of the tests
