@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["pt", "grid"], default="pt",
+                    help="pt: the headline PT+NEE frame; grid: one gradient evaluation (forward + path-replay "
+                         "adjoint) of the 64^3 heterogeneous medium at 256x256, 64 spp")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run three device-to-device copies of MIB MiB (a plain copy kernel of known "
                          "size, for scaling the HBM byte counters of a --pmc run)")
@@ -68,6 +71,8 @@ def main():
         torch.cuda.synchronize()
         log(f"copy calibration: 3 elementwise copies of {n} bytes")
         del src, dst
+    if args.workload == "grid":
+        return bench_grid(args, log)
     W, H, SPP = args.width, args.height, args.spp
     t0 = time.time()
     desc, cam_args = scenes.million_triangles(W, H)
@@ -189,6 +194,112 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_pt_extend (closest-hit BVH4 traversal)",
                      "bytes_per_ray": round(bytes_per_ext_ray, 1), "avg_launch_ms": round(ext_launch_ms, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_grid(args, log):
+    """One step = forward render of the 64^3 medium at 256x256, 64 spp, and the path-replay adjoint of
+    L = sum (I - T)^2 against a target rendered from the perturbed density (untimed)."""
+    import numpy as np
+    import torch
+
+    import cupbr_amd as cb
+    from cupbr_amd import scenes
+
+    W = H = 256
+    SPP, DEPTH, RES = 64, 6, 64
+    G = scenes.grid_medium_scene(res=RES, width=W, height=H)
+    ca = G["camera"]
+    cam = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], W, H)
+    scene = cb.Scene.from_desc(G["desc"])
+    scene.upload()
+    R = cb.GridRenderer(scene, cam, cb.GridMedium.from_dict(G["density"].shape, G["medium"]), W, H, spp=SPP,
+                        max_depth=DEPTH)
+    rho = torch.from_numpy(G["density"]).cuda()
+    alb = torch.from_numpy(G["albedo"]).cuda()
+    target = R.render(torch.from_numpy(G["target_density"]).cuda(), alb)
+
+    def step():
+        img = R.render(rho, alb)
+        return R.adjoint(rho, alb, 2 * (img - target))
+
+    R.set_counting(True)
+    step()
+    cnt = R.counts()
+    R.set_counting(False)
+    # per adjoint launch: the replayed paths read the grid in the forward pass and the replay
+    R.set_counting(True)
+    img = R.render(rho, alb)
+    fwd = R.counts()
+    R.set_counting(False)
+    adj_reads = cnt["grid_reads"] - fwd["grid_reads"]
+    adj_bytes = adj_reads * 32 + cnt["atomics"] * 8  # 8 voxels x 4 B per read; read-modify-write per atomic
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    adj_ms = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        img = R.render(rho, alb)
+        d = 2 * (img - target)
+        ev[k][0].record()
+        R.adjoint(rho, alb, d)
+        ev[k][1].record()
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    adj_ms = [a.elapsed_time(b) for a, b in ev]
+    adj_launch_ms = float(np.mean(adj_ms))
+    samples = W * H * SPP * args.steps
+    value = samples / t_total / 1e6
+    achieved = adj_bytes / (adj_launch_ms * 1e-3) / 1e9
+    log(f"grid: {value:.1f} Msamples/s (forward + adjoint), {t_total / args.steps * 1e3:.1f} ms/step, adjoint "
+        f"{adj_launch_ms:.2f} ms; per step {cnt['closest_rays'] / 1e6:.1f}M closest + {cnt['shadow_rays'] / 1e6:.1f}M "
+        f"shadow rays, {cnt['grid_reads'] / 1e6:.1f}M grid reads, {cnt['atomics'] / 1e6:.1f}M atomics")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        S = oracle.OracleScene(G["desc"], precisions=(32,))
+        sw = sh = 256  # the full frame
+        cam_s = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], sw, sh)
+        t0 = time.perf_counter()
+        im = S.grid(cam_s.as_dict(), G["medium"], G["density"], G["albedo"], sw, sh, spp=SPP, depth=DEPTH,
+                    precision=32, threads=threads)
+        S.grid(cam_s.as_dict(), G["medium"], G["density"], G["albedo"], sw, sh, spp=SPP, depth=DEPTH,
+               precision=32, threads=threads, d_image=2 * im)  # cotangent of sum I^2: same work as any loss
+        t_cpu = time.perf_counter() - t0
+        cpu_value = sw * sh * SPP / t_cpu / 1e6
+        cpu = {"value": round(cpu_value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+               "sample": f"forward + adjoint of the same medium at {sw}x{sh}, {SPP} spp on the C++ oracle "
+                         f"(FP32), {threads} threads, {t_cpu:.1f}s"}
+        log(f"CPU oracle: {cpu_value:.3f} Msamples/s on {threads} threads; GPU/CPU = {value / cpu_value:.0f}x")
+
+    out = {
+        "metric": "Msamples/s (grid medium forward + adjoint)",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (fBm density, cupbr_amd.scenes.grid_medium_scene)",
+        "config": {"workload": f"heterogeneous medium {RES}^3 in a 10-triangle room, {W}x{H}, {SPP} spp, depth "
+                               f"{DEPTH}: forward render + path-replay adjoint of an L2 image loss",
+                   "width": W, "height": H, "spp": SPP, "grid": RES, "max_depth": DEPTH},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_grid<true> (path-replay adjoint)",
+                     "bytes_per_launch": int(adj_bytes), "avg_launch_ms": round(adj_launch_ms, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

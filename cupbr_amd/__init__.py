@@ -200,6 +200,8 @@ def lib():
             "cupbr_tonemapper_set_exposure": (C.c_int, [P, C.c_float]),
             "cupbr_tonemapper_save_to_file": (C.c_int, [P, C.c_char_p]),
             "cupbr_grid_surface_count": (C.c_int, [P, P]),
+            "cupbr_grid_set_counting": (C.c_int, [P, I32]),
+            "cupbr_grid_get_counts": (C.c_int, [P, P]),
             "cupbr_grid_render_device": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P]),
             "cupbr_grid_adjoint_device": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P, P, P]),
             "cupbr_grid_render": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P]),
@@ -459,6 +461,14 @@ class GridRenderer:
                  max_depth: int = 6, frame0: int = 0):
         self.scene, self.camera, self.medium = scene, camera, medium
         self.width, self.height, self.spp, self.max_depth, self.frame0 = width, height, spp, max_depth, frame0
+
+    def set_counting(self, enable: bool):
+        _check(lib().cupbr_grid_set_counting(self.scene.handle, int(enable)))
+
+    def counts(self) -> dict:
+        c = (C.c_uint64 * 4)()
+        _check(lib().cupbr_grid_get_counts(self.scene.handle, c))
+        return dict(closest_rays=c[0], shadow_rays=c[1], grid_reads=c[2], atomics=c[3])
 
     def surface_count(self) -> int:
         n = C.c_uint32()

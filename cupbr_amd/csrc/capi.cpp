@@ -661,6 +661,16 @@ int cupbr_grid_surface_count(const cupbr_scene* s, uint32_t* n) {
     return CUPBR_OK;
 }
 
+int cupbr_grid_set_counting(cupbr_scene* s, int32_t enable) {
+    CHECK_ARG(s, "null scene");
+    return grid_set_counting(s, enable != 0);
+}
+
+int cupbr_grid_get_counts(const cupbr_scene* s, uint64_t counts[4]) {
+    CHECK_ARG(s && counts, "null argument");
+    return grid_get_counts(s, counts);
+}
+
 int cupbr_grid_render_device(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m,
                              const float* density, const float* proposal, const float* albedo, uint32_t w, uint32_t h,
                              uint32_t frame0, uint32_t spp, uint32_t max_depth, float* image, void* stream) {

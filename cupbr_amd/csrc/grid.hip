@@ -95,10 +95,15 @@ __device__ __forceinline__ float sample_grid(const float* g, const Stencil& s) {
     return v;
 }
 
-__device__ __forceinline__ void scatter_grad(float* grad, const Stencil& s, float c) {
-    if (c == 0.0f || !(fabsf(c) < __builtin_inff())) return;
+__device__ __forceinline__ uint32_t scatter_grad(float* grad, const Stencil& s, float c) {
+    if (c == 0.0f || !(fabsf(c) < __builtin_inff())) return 0;
+    uint32_t n = 0;
     for (int k = 0; k < 8; ++k)
-        if (s.w[k] != 0.0f) atomicAdd(grad + s.idx[k], c * s.w[k]);
+        if (s.w[k] != 0.0f) {
+            atomicAdd(grad + s.idx[k], c * s.w[k]);
+            ++n;
+        }
+    return n;
 }
 
 // parametric range [t0, t1] of o + t d inside the medium box, clipped to [0, tmax]
@@ -135,18 +140,22 @@ struct Adj {
 struct GridTracer {
     const DevScene& S;
     LaneStack<kGridLds>& st;
+    // work of this lane: closest-hit rays, shadow rays, grid stencil reads, gradient atomics
+    uint32_t n_ext = 0, n_shadow = 0, n_reads = 0, n_atomics = 0;
     __device__ __forceinline__ bool closest(v3 o, v3 d, TraceResult& r) {
+        ++n_ext;
         traverse<false>(S, o, d, 0.0f, st, r);
         return r.prim != kNoHit;
     }
     __device__ __forceinline__ bool visible(v3 o, v3 d, float dist) {
+        ++n_shadow;
         TraceResult r;
         return !traverse<true>(S, o, d, dist, st, r);
     }
 };
 
 // shadow-ray transmittance through the medium by ratio tracking
-__device__ __forceinline__ float ratio_track(const GridDev& G, v3 o, v3 d, float dist, uint32_t& seed) {
+__device__ __forceinline__ float ratio_track(const GridDev& G, GridTracer& Tr, v3 o, v3 d, float dist, uint32_t& seed) {
     float t0, t1;
     if (!clip_box(G, o, d, dist, t0, t1)) return 1.0f;
     const float len = t1 - t0;
@@ -158,14 +167,15 @@ __device__ __forceinline__ float ratio_track(const GridDev& G, v3 o, v3 d, float
         Stencil s;
         stencil(G, o + (t0 + u) * d, s);
         const float sigma = G.scale * sample_grid(G.rho, s);
+        ++Tr.n_reads;
         T = T * (1.0f - sigma / G.maj);
     }
     return T;
 }
 
 // the same tentative points again, adding coef * dlog(1 - sigma/m)/dsigma * scale to the voxels
-__device__ __forceinline__ void ratio_track_grad(const GridDev& G, v3 o, v3 d, float dist, uint32_t seed, float coef,
-                                                 float* grad) {
+__device__ __forceinline__ void ratio_track_grad(const GridDev& G, GridTracer& Tr, v3 o, v3 d, float dist,
+                                                 uint32_t seed, float coef, float* grad) {
     float t0, t1;
     if (!clip_box(G, o, d, dist, t0, t1)) return;
     const float len = t1 - t0;
@@ -176,7 +186,8 @@ __device__ __forceinline__ void ratio_track_grad(const GridDev& G, v3 o, v3 d, f
         Stencil s;
         stencil(G, o + (t0 + u) * d, s);
         const float sigma = G.scale * sample_grid(G.rho, s);
-        scatter_grad(grad, s, coef * (-1.0f / (G.maj - sigma)) * G.scale);
+        ++Tr.n_reads;
+        Tr.n_atomics += scatter_grad(grad, s, coef * (-1.0f / (G.maj - sigma)) * G.scale);
     }
 }
 
@@ -195,7 +206,7 @@ __device__ __forceinline__ bool nee(const GridDev& G, const DevScene& S, GridTra
     const v3 so = surface ? x + kSurfaceOffset * N : x;
     if (!T.visible(so, ldir, dist)) return false;
     track_seed = seed;
-    tr = ratio_track(G, so, ldir, dist, seed);
+    tr = ratio_track(G, T, so, ldir, dist, seed);
     return true;
 }
 
@@ -227,6 +238,7 @@ __device__ v3 grid_path(const DevScene& S, const GridDev& G, GridTracer& T, cons
                 stencil(G, x, s);
                 const float sigma = G.scale * sample_grid(G.rho, s);
                 const float P = clampf(G.scale * sample_grid(G.prop, s) / G.maj, G.eps, 1.0f - G.eps);
+                T.n_reads += 2;
                 float f, dlog;
                 const bool real = rnd(seed) < P;
                 if (real) {
@@ -238,7 +250,7 @@ __device__ v3 grid_path(const DevScene& S, const GridDev& G, GridTracer& T, cons
                 }
                 if (f == 0.0f) return L;
                 beta = beta * f;
-                if (kAdj) scatter_grad(A->grad_rho, s, dot3(A->delta, A->rem) * dlog * G.scale);
+                if (kAdj) T.n_atomics += scatter_grad(A->grad_rho, s, dot3(A->delta, A->rem) * dlog * G.scale);
                 if (real) {
                     scattered = true;
                     break;
@@ -255,7 +267,7 @@ __device__ v3 grid_path(const DevScene& S, const GridDev& G, GridTracer& T, cons
                 L += C;
                 if (kAdj) {
                     const float c = dot3(A->delta, C);
-                    if (c != 0.0f) ratio_track_grad(G, x, ldir, dist, tseed, c, A->grad_rho);
+                    if (c != 0.0f) ratio_track_grad(G, T, x, ldir, dist, tseed, c, A->grad_rho);
                     A->rem = A->rem - C;
                 }
             }
@@ -301,8 +313,9 @@ __device__ v3 grid_path(const DevScene& S, const GridDev& G, GridTracer& T, cons
                 if (ga.x != 0.0f) atomicAdd(gp + 0, ga.x);
                 if (ga.y != 0.0f) atomicAdd(gp + 1, ga.y);
                 if (ga.z != 0.0f) atomicAdd(gp + 2, ga.z);
+                T.n_atomics += 3;
                 const float c = dot3(A->delta, C);
-                if (c != 0.0f) ratio_track_grad(G, P + kSurfaceOffset * N, ldir, dist, tseed, c, A->grad_rho);
+                if (c != 0.0f) ratio_track_grad(G, T, P + kSurfaceOffset * N, ldir, dist, tseed, c, A->grad_rho);
                 A->rem = A->rem - C;
             }
         }
@@ -330,7 +343,8 @@ __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupb
                                                      uint32_t spp, uint32_t frame0, uint32_t s_begin, uint32_t s_count,
                                                      uint32_t max_depth, float* __restrict__ samples,
                                                      const float* __restrict__ d_image, float* grad_rho,
-                                                     float* grad_alb, uint32_t* spill) {
+                                                     float* grad_alb, uint32_t* spill,
+                                                     unsigned long long* counts) {
     __shared__ uint32_t lds[kGridLds * kGridBlock];
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -352,6 +366,13 @@ __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupb
             A.grad_alb = grad_alb;
             if (A.delta.x != 0.0f || A.delta.y != 0.0f || A.delta.z != 0.0f)
                 grid_path<true>(S, G, T, cam, px, py, w, h, frame0 + s, max_depth, &A);
+        }
+    }
+    if (counts) {
+        unsigned long long c[4] = {T.n_ext, T.n_shadow, T.n_reads, T.n_atomics};
+        for (int k = 0; k < 4; ++k) {
+            for (int off = 32; off > 0; off >>= 1) c[k] += __shfl_xor(c[k], off);
+            if (__lane_id() == 0) atomicAdd(counts + k, c[k]);
         }
     }
 }
@@ -430,12 +451,29 @@ int grid_launch_shape(cupbr_scene* s, size_t n, int* blocks, uint32_t** spill) {
 
 }  // namespace
 
+int grid_set_counting(cupbr_scene* s, bool enable) {
+    s->grid_counting = enable;
+    if (!enable) return CUPBR_OK;
+    if (s->grid_counts.ensure(4 * sizeof(unsigned long long))) return CUPBR_ERR_OUT_OF_MEMORY;
+    CUPBR_HIP_CHECK(hipMemset(s->grid_counts.ptr, 0, 4 * sizeof(unsigned long long)));
+    return CUPBR_OK;
+}
+
+int grid_get_counts(const cupbr_scene* s, uint64_t out[4]) {
+    for (int k = 0; k < 4; ++k) out[k] = 0;
+    if (!s->grid_counting || !s->grid_counts.ptr) return CUPBR_OK;
+    CUPBR_HIP_CHECK(hipDeviceSynchronize());
+    CUPBR_HIP_CHECK(hipMemcpy(out, s->grid_counts.ptr, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return CUPBR_OK;
+}
+
 int grid_render_device(const cupbr_scene* cs, const cupbr_camera* cam, const cupbr_grid_medium* m, const float* rho,
                        const float* prop, const float* alb, uint32_t w, uint32_t h, uint32_t frame0, uint32_t spp,
                        uint32_t max_depth, float* image, hipStream_t stream) {
     cupbr_scene* s = const_cast<cupbr_scene*>(cs);
     if (int rc = scene_upload(s)) return rc;
     const GridDev G = make_grid(s, m, rho, prop, alb);
+    unsigned long long* counts = s->grid_counting ? s->grid_counts.as<unsigned long long>() : nullptr;
     const uint32_t n_px = w * h;
     const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(spp, kMaxGridSamples / n_px));
     if (s->grid_samples.ensure((size_t)n_px * chunk * 12)) return CUPBR_ERR_OUT_OF_MEMORY;
@@ -445,7 +483,7 @@ int grid_render_device(const cupbr_scene* cs, const cupbr_camera* cam, const cup
         uint32_t* spill;
         if (grid_launch_shape(s, (size_t)n_px * cnt, &blocks, &spill)) return CUPBR_ERR_OUT_OF_MEMORY;
         hipLaunchKernelGGL(k_grid<false>, dim3(blocks), dim3(kGridBlock), 0, stream, s->dev, G, *cam, w, h, spp, frame0,
-                           s0, cnt, max_depth, s->grid_samples.as<float>(), nullptr, nullptr, nullptr, spill);
+                           s0, cnt, max_depth, s->grid_samples.as<float>(), nullptr, nullptr, nullptr, spill, counts);
         CUPBR_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_grid_accumulate, dim3((n_px + 255) / 256), dim3(256), 0, stream, image,
                            s->grid_samples.as<float>(), n_px, cnt, s0 == 0 ? 1 : 0, s0 + cnt == spp ? 1 : 0, spp);
@@ -461,11 +499,12 @@ int grid_adjoint_device(const cupbr_scene* cs, const cupbr_camera* cam, const cu
     cupbr_scene* s = const_cast<cupbr_scene*>(cs);
     if (int rc = scene_upload(s)) return rc;
     const GridDev G = make_grid(s, m, rho, prop, alb);
+    unsigned long long* counts = s->grid_counting ? s->grid_counts.as<unsigned long long>() : nullptr;
     int blocks;
     uint32_t* spill;
     if (grid_launch_shape(s, (size_t)w * h * spp, &blocks, &spill)) return CUPBR_ERR_OUT_OF_MEMORY;
     hipLaunchKernelGGL(k_grid<true>, dim3(blocks), dim3(kGridBlock), 0, stream, s->dev, G, *cam, w, h, spp, frame0, 0u,
-                       spp, max_depth, nullptr, d_image, grad_rho, grad_alb, spill);
+                       spp, max_depth, nullptr, d_image, grad_rho, grad_alb, spill, counts);
     CUPBR_HIP_CHECK(hipGetLastError());
     return CUPBR_OK;
 }

@@ -37,6 +37,8 @@ struct cupbr_scene {
     cupbr::DevBuf spill;               // traversal stack spill for the batch query kernels
     cupbr::DevBuf grid_spill;          // traversal stack spill of the grid-medium kernels
     cupbr::DevBuf grid_samples;        // per-sample radiance of the grid-medium forward pass
+    cupbr::DevBuf grid_counts;         // 4 x u64 work counters of the grid-medium kernels
+    bool grid_counting = false;
 };
 
 namespace cupbr {
@@ -62,6 +64,8 @@ int trace_visibility_device(const cupbr_scene* s, const float* rays, const float
                             hipStream_t stream);
 
 // heterogeneous medium + adjoint (grid.hip)
+int grid_set_counting(cupbr_scene* s, bool enable);
+int grid_get_counts(const cupbr_scene* s, uint64_t out[4]);
 int grid_validate(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m, uint32_t w, uint32_t h,
                   uint32_t spp, uint32_t max_depth);
 int grid_render_device(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m, const float* rho,

@@ -313,6 +313,11 @@ int cupbr_grid_adjoint_device(const cupbr_scene* scene, const cupbr_camera* came
                               const float* density, const float* proposal, const float* surface_albedo,
                               uint32_t width, uint32_t height, uint32_t frame0, uint32_t spp, uint32_t max_depth,
                               const float* d_image, float* grad_density, float* grad_albedo, void* hip_stream);
+/* instrumentation: with counting on, the grid kernels add their work to four counters of the
+ * scene: closest-hit rays, shadow rays, grid stencil reads (8 voxels each) and gradient atomics.
+ * Enabling resets them; cupbr_grid_get_counts synchronizes the device and reads them. */
+int cupbr_grid_set_counting(cupbr_scene* scene, int32_t enable);
+int cupbr_grid_get_counts(const cupbr_scene* scene, uint64_t counts[4]);
 /* host-pointer variants (copy through device memory; the adjoint OVERWRITES the gradient outputs) */
 int cupbr_grid_render(const cupbr_scene* scene, const cupbr_camera* camera, const cupbr_grid_medium* medium,
                       const float* density, const float* proposal, const float* surface_albedo, uint32_t width,
