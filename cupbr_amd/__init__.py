@@ -104,7 +104,8 @@ class _RenderStats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("milliseconds", C.c_double), ("stage_ms", C.c_double * 4), ("stage_launches", C.c_uint32 * 4),
                 ("extension_node_visits", C.c_uint64), ("extension_prim_tests", C.c_uint64),
-                ("shadow_node_visits", C.c_uint64), ("shadow_prim_tests", C.c_uint64)]
+                ("shadow_node_visits", C.c_uint64), ("shadow_prim_tests", C.c_uint64),
+                ("cache_queries", C.c_uint64), ("cache_training_rows", C.c_uint64)]
 
 
 class _GridMedium(C.Structure):
@@ -200,6 +201,14 @@ def lib():
             "cupbr_tonemapper_set_exposure": (C.c_int, [P, C.c_float]),
             "cupbr_tonemapper_save_to_file": (C.c_int, [P, C.c_char_p]),
             "cupbr_grid_surface_count": (C.c_int, [P, P]),
+            "cupbr_nrc_create": (C.c_int, [P, P, U32, P]),
+            "cupbr_nrc_destroy": (C.c_int, [P]),
+            "cupbr_nrc_get_weights": (C.c_int, [P, P]),
+            "cupbr_nrc_set_weights": (C.c_int, [P, P]),
+            "cupbr_nrc_infer_device": (C.c_int, [P, P, U32, P, P]),
+            "cupbr_nrc_train_device": (C.c_int, [P, P, U32, U32, U32, C.c_float, P, P]),
+            "cupbr_nrc_matrix_core_product": (C.c_int, [P, P, P]),
+            "cupbr_renderer_set_radiance_cache": (C.c_int, [P, P, U32, U32, U32, C.c_float]),
             "cupbr_grid_set_counting": (C.c_int, [P, I32]),
             "cupbr_grid_get_counts": (C.c_int, [P, P]),
             "cupbr_grid_render_device": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P]),
@@ -434,6 +443,82 @@ class Scene:
         return vis.astype(bool)
 
 
+NRC_PARAMS = 16576
+NRC_ROW_FLOATS = 20
+
+
+class RadianceCache:
+    """cupbr_nrc: the neural radiance cache (64 -> 4x64 ReLU -> 3 MLP, BF16 matrix cores).
+
+    Inputs per query (16 floats): position[3], outgoing direction[3], normal[3], roughness,
+    diffuse albedo[3], specular albedo[3]. Device arrays are torch CUDA tensors."""
+
+    def __init__(self, box_min, box_max, seed: int = 0):
+        lo = (C.c_float * 3)(*box_min)
+        hi = (C.c_float * 3)(*box_max)
+        h = C.c_void_p()
+        _check(lib().cupbr_nrc_create(lo, hi, seed, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def destroy(self):
+        if self._h:
+            lib().cupbr_nrc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(NRC_PARAMS, np.float32)
+        _check(lib().cupbr_nrc_get_weights(self._h, _ptr(w)))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32).reshape(-1)
+        if w.size != NRC_PARAMS:
+            raise CupbrError(-13, f"expected {NRC_PARAMS} weights")
+        _check(lib().cupbr_nrc_set_weights(self._h, _ptr(w)))
+
+    def infer(self, inputs):
+        """inputs: CUDA tensor [n, 16] -> predictions [n, 3] (on torch's current stream)."""
+        import torch
+        x = inputs.contiguous().float()
+        n = x.shape[0]
+        out = torch.empty(n, 3, dtype=torch.float32, device=x.device)
+        stream = torch.cuda.current_stream(x.device).cuda_stream
+        _check(lib().cupbr_nrc_infer_device(self._h, x.data_ptr(), n, out.data_ptr(), stream))
+        return out
+
+    def train(self, rows, batch: int, steps: int, lr: float = 1e-3, return_loss: bool = False):
+        """rows: CUDA tensor [n, 20] (16 inputs, 3 targets, 1 unused). Returns the per-step summed
+        batch loss when return_loss."""
+        import torch
+        r = rows.contiguous().float()
+        if r.shape[-1] != NRC_ROW_FLOATS:
+            raise CupbrError(-13, "rows must have 20 floats each")
+        loss = torch.zeros(steps, dtype=torch.float32, device=r.device) if return_loss else None
+        stream = torch.cuda.current_stream(r.device).cuda_stream
+        _check(lib().cupbr_nrc_train_device(self._h, r.data_ptr(), r.shape[0], batch, steps, lr,
+                                            None if loss is None else loss.data_ptr(), stream))
+        return loss
+
+    @staticmethod
+    def matrix_core_product(a, b) -> np.ndarray:
+        a = np.ascontiguousarray(a, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        assert a.shape == (16, 32) and b.shape == (32, 16)
+        d = np.zeros((16, 16), np.float32)
+        _check(lib().cupbr_nrc_matrix_core_product(_ptr(a), _ptr(b), _ptr(d)))
+        return d
+
+
 class GridMedium:
     """cupbr_grid_medium: a box medium whose extinction is sigma_t_scale * density(x), density
     trilinear between the centres of an [nz, ny, nx] voxel grid (no reference counterpart)."""
@@ -648,6 +733,13 @@ class PBRenderer:
     def set_profiling(self, enable: bool):
         """Per-stage device time (HIP events on the renderer's stream), reported by stats()."""
         _check(lib().cupbr_renderer_set_profiling(self._h, int(bool(enable))))
+
+    def set_radiance_cache(self, cache: "RadianceCache | None", training_paths: int = 1 << 16, train_steps: int = 16,
+                           batch: int = 1 << 14, learning_rate: float = 1e-3):
+        """Attach a RadianceCache (None = plain PT); see cupbr_renderer_set_radiance_cache."""
+        self._cache = cache
+        _check(lib().cupbr_renderer_set_radiance_cache(self._h, None if cache is None else cache.handle,
+                                                       training_paths, train_steps, batch, learning_rate))
 
     def set_traversal_counting(self, enable: bool):
         """Counts BVH4 node visits and primitive tests (reported by stats())."""

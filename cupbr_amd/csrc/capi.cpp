@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "nrc.h"
 #include "post.h"
 #include "renderer.h"
 #include "scene.h"
@@ -387,6 +388,7 @@ int cupbr_render(cupbr_renderer* r, const cupbr_camera* cam, uint32_t spp) {
     if (int rc = scene_upload(r->scene)) return rc;
     if (cam->moved) r->frame_index = 0;  // :174-177
     r->stats = cupbr_render_stats{};
+    r->nrc_queries_total = r->nrc_rows_total = 0;
     r->timer.used = 0;
     CUPBR_HIP_CHECK(hipEventRecord(r->ev_start, r->stream));
     int rc = CUPBR_OK;
@@ -427,6 +429,8 @@ int cupbr_renderer_get_stats(const cupbr_renderer* cr, cupbr_render_stats* out) 
         r->stats.shadow_node_visits = c.shadow_nodes;
         r->stats.shadow_prim_tests = c.shadow_prims;
     }
+    r->stats.cache_queries = r->nrc_queries_total;
+    r->stats.cache_training_rows = r->nrc_rows_total;
     for (int k = 0; k < 4; ++k) { r->stats.stage_ms[k] = 0.0; r->stats.stage_launches[k] = 0; }
     for (size_t i = 1; i < r->timer.used; ++i) {
         const int st = r->timer.stage[i];
@@ -749,5 +753,65 @@ int cupbr_grid_adjoint(const cupbr_scene* s, const cupbr_camera* cam, const cupb
         return rc;
     CUPBR_HIP_CHECK(hipMemcpy(grad_density, gr.ptr, nv, hipMemcpyDeviceToHost));
     if (na) CUPBR_HIP_CHECK(hipMemcpy(grad_albedo, ga.ptr, na, hipMemcpyDeviceToHost));
+    return CUPBR_OK;
+}
+
+// ------------------------------------------------------------------ neural radiance cache
+int cupbr_nrc_create(const float lo[3], const float hi[3], uint32_t seed, cupbr_nrc** out) {
+    CHECK_ARG(lo && hi && out, "null argument");
+    for (int a = 0; a < 3; ++a) CHECK_ARG(std::isfinite(lo[a]) && std::isfinite(hi[a]) && hi[a] > lo[a], "invalid box");
+    return nrc_create(lo, hi, seed, out);
+}
+
+int cupbr_nrc_destroy(cupbr_nrc* c) {
+    delete c;
+    return CUPBR_OK;
+}
+
+int cupbr_nrc_get_weights(const cupbr_nrc* c, float* w) {
+    CHECK_ARG(c && w, "null argument");
+    CUPBR_HIP_CHECK(hipDeviceSynchronize());
+    CUPBR_HIP_CHECK(hipMemcpy(w, c->w.ptr, kNrcParams * 4, hipMemcpyDeviceToHost));
+    return CUPBR_OK;
+}
+
+int cupbr_nrc_set_weights(cupbr_nrc* c, const float* w) {
+    CHECK_ARG(c && w, "null argument");
+    CUPBR_HIP_CHECK(hipDeviceSynchronize());
+    int rc = nrc_set_weights(c, w);
+    if (rc) return rc;
+    CUPBR_HIP_CHECK(hipDeviceSynchronize());
+    return CUPBR_OK;
+}
+
+int cupbr_nrc_infer_device(cupbr_nrc* c, const float* inputs, uint32_t n, float* out, void* stream) {
+    CHECK_ARG(c && (n == 0 || (inputs && out)), "null argument");
+    return nrc_infer(c, inputs, n, nullptr, kNrcInputs, out, false, (hipStream_t)stream);
+}
+
+int cupbr_nrc_train_device(cupbr_nrc* c, const float* rows, uint32_t n_rows, uint32_t batch, uint32_t steps, float lr,
+                           float* loss, void* stream) {
+    CHECK_ARG(c && (n_rows == 0 || rows), "null argument");
+    CHECK_ARG(batch >= 512 && batch % 512 == 0 && batch <= (1u << 22), "batch must be a multiple of 512, <= 2^22");
+    CHECK_ARG(std::isfinite(lr) && lr >= 0.0f, "invalid learning rate");
+    return nrc_train(c, rows, n_rows, batch, steps, lr, loss, (hipStream_t)stream);
+}
+
+int cupbr_nrc_matrix_core_product(const float* a, const float* b, float* d) {
+    CHECK_ARG(a && b && d, "null argument");
+    return nrc_mfma_product(a, b, d);
+}
+
+int cupbr_renderer_set_radiance_cache(cupbr_renderer* r, cupbr_nrc* cache, uint32_t training_paths,
+                                      uint32_t train_steps, uint32_t batch, float lr) {
+    CHECK_ARG(r, "null renderer");
+    CHECK_ARG(batch >= 512 && batch % 512 == 0 && batch <= (1u << 22), "batch must be a multiple of 512, <= 2^22");
+    CHECK_ARG(std::isfinite(lr) && lr >= 0.0f, "invalid learning rate");
+    CUPBR_HIP_CHECK(hipStreamSynchronize(r->stream));
+    r->nrc = cache;
+    r->nrc_train_paths = training_paths;
+    r->nrc_steps = train_steps;
+    r->nrc_batch = batch;
+    r->nrc_lr = lr;
     return CUPBR_OK;
 }

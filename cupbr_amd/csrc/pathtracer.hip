@@ -69,14 +69,31 @@ __device__ __forceinline__ void wave_add_counts(const TravCount& tc, unsigned lo
     }
 }
 
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// radiance-cache training paths: pixel/frame pairs whose hash falls below the threshold
+__device__ __forceinline__ bool is_training_path(uint32_t pix, uint32_t frame, uint32_t thresh) {
+    return thresh != 0u && hash32(pix * 0x9E3779B1u ^ hash32(frame + 0x632BE5ABu)) < thresh;
+}
+
+// rad[p].w carries the path's flags: bit 0 = radiance-cache training path
 __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0,
                                                         uint32_t npath, float4* __restrict__ q_o,
                                                         float4* __restrict__ q_d, float4* __restrict__ weight,
-                                                        float4* __restrict__ rad, Counters* c) {
+                                                        float4* __restrict__ rad, Counters* c, uint32_t train_thresh) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
         c->queue_count[0].v = npath;
         c->work_extend.v = 0;
+        c->query_count.v = 0;
+        c->row_count.v = 0;
     }
     if (p >= npath) return;
     const uint32_t npix = w * h;
@@ -87,7 +104,8 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
     q_o[p] = make_float4(o.x, o.y, o.z, 0.0f);
     q_d[p] = make_float4(d.x, d.y, d.z, __uint_as_float(p));
     weight[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(seed));
-    rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t flags = is_training_path(pix, frame0 + s, train_thresh) ? 1u : 0u;
+    rad[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(flags));
 }
 
 template <bool kCount>
@@ -137,21 +155,32 @@ __device__ __forceinline__ void hit_geometry(const DevScene& S, uint32_t prim, v
     }
 }
 
+// radiance-cache records written by shade: a query (inputs 16, throughput 3, path) ends a rendering
+// path at its first diffuse/glossy vertex after the camera hit; a training path leaves a pending
+// row (inputs 16, radiance so far 3, throughput 3, path, pad) at each such vertex
+struct NrcShade {
+    float4* queries;  // 5 float4 per query
+    float4* pending;  // 6 float4 per row
+    uint32_t max_rows;
+};
+
+template <bool kNrc>
 __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int rr,
                                                        const float4* __restrict__ q_o, const float4* __restrict__ q_d,
                                                        const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
                                                        float4* __restrict__ nq_d, float4* __restrict__ s_o,
                                                        float4* __restrict__ s_d, float4* __restrict__ s_c,
                                                        float4* __restrict__ weight, float4* __restrict__ rad,
-                                                       Counters* c) {
+                                                       Counters* c, NrcShade nrc) {
     __shared__ uint32_t s_append[kPtWaves + 1];
     const uint32_t n = c->queue_count[cur].v;
     if (blockIdx.x == 0 && threadIdx.x == 0) c->work_extend.v = 0;  // extend of the next bounce starts at 0
     const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
     for (uint32_t base = blockIdx.x * kPtBlock; base < n; base += gridDim.x * kPtBlock) {
         const uint32_t i = base + threadIdx.x;
-        bool want_next = false, want_shadow = false;
+        bool want_next = false, want_shadow = false, want_query = false, want_row = false;
         float4 no = {}, nd = {}, so = {}, sd = {}, sc = {};
+        float4 rec[6] = {};  // cache query or pending row
         if (i < n) {
             const float4 o4 = q_o[i], d4 = q_d[i];
             const float2 hh = q_hit[i];
@@ -174,7 +203,26 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
                 const v3 inc = -1.0f * d;
                 L += w3 * m.albedo_e;  // emissiveIllumintation :20-26
                 bool alive = true;
-                if (m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f) {  // :36-41
+                bool cached = false;
+                if (kNrc && bounce >= 1 && (m.type == CUPBR_LAMBERT || m.type == CUPBR_PHONG || m.type == CUPBR_GGX) &&
+                    !(m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f)) {
+                    rec[0] = make_float4(P.x, P.y, P.z, inc.x);
+                    rec[1] = make_float4(inc.y, inc.z, N.x, N.y);
+                    rec[2] = make_float4(N.z, m.roughness, m.albedo_d.x, m.albedo_d.y);
+                    rec[3] = make_float4(m.albedo_d.z, m.albedo_s.x, m.albedo_s.y, m.albedo_s.z);
+                    if (__float_as_uint(R.w) & 1u) {  // training path: pending row, the path goes on
+                        want_row = true;
+                        rec[4] = make_float4(L.x, L.y, L.z, w3.x);
+                        rec[5] = make_float4(w3.y, w3.z, __uint_as_float(p), 0.0f);
+                    } else {  // rendering path: ends here with throughput * cache
+                        want_query = true;
+                        cached = true;
+                        rec[4] = make_float4(w3.x, w3.y, w3.z, __uint_as_float(p));
+                    }
+                }
+                if (cached) {
+                    alive = false;
+                } else if (m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f) {  // :36-41
                     w3 = splat(0.0f);
                     alive = false;
                 } else {
@@ -219,7 +267,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
                 }
                 weight[p] = make_float4(w3.x, w3.y, w3.z, __uint_as_float(seed));
             }
-            rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
+            rad[p] = make_float4(L.x, L.y, L.z, R.w);
         }
         const uint32_t ns = block_append(want_next, &c->queue_count[cur ^ 1].v, s_append);
         if (want_next) {
@@ -232,7 +280,35 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
             s_d[ss] = sd;
             s_c[ss] = sc;
         }
+        if (kNrc) {
+            const uint32_t qs = block_append(want_query, &c->query_count.v, s_append);
+            if (want_query)
+                for (int k = 0; k < 5; ++k) nrc.queries[(size_t)qs * 5 + k] = rec[k];
+            const uint32_t rs = block_append(want_row, &c->row_count.v, s_append);
+            if (want_row && rs < nrc.max_rows)
+                for (int k = 0; k < 6; ++k) nrc.pending[(size_t)rs * 6 + k] = rec[k];
+        }
     }
+}
+
+// training rows from the pending rows of finished training paths: target = (final radiance -
+// radiance before the vertex's reflection) / throughput at the vertex (0 where the throughput is 0)
+__global__ __launch_bounds__(kPtBlock) void k_nrc_rows(const float4* __restrict__ pending, const Counters* c,
+                                                       uint32_t max_rows, const float4* __restrict__ rad,
+                                                       float4* __restrict__ rows) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = min(c->row_count.v, max_rows);
+    if (i >= n) return;
+    const float4* r = pending + (size_t)i * 6;
+    const float4 a = r[4], b = r[5];
+    const uint32_t p = __float_as_uint(b.z);
+    const float4 R = rad[p];
+    const float beta[3] = {a.w, b.x, b.y}, snap[3] = {a.x, a.y, a.z}, fin[3] = {R.x, R.y, R.z};
+    float tgt[3];
+    for (int k = 0; k < 3; ++k) tgt[k] = beta[k] > 0.0f ? (fin[k] - snap[k]) / beta[k] : 0.0f;
+    float4* o = rows + (size_t)i * 5;
+    for (int k = 0; k < 4; ++k) o[k] = r[k];
+    o[4] = make_float4(tgt[0], tgt[1], tgt[2], 0.0f);
 }
 
 template <bool kCount>
@@ -258,7 +334,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shadow(DevScene S, const float4
             const float4 R = rad[p];
             v3 L = mk(R.x, R.y, R.z);
             L += mk(cc.x, cc.y, cc.z);
-            rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
+            rad[p] = make_float4(L.x, L.y, L.z, R.w);
         }
     }
     if (kCount) wave_add_counts(tc, &c->shadow_nodes, &c->shadow_prims);
@@ -312,11 +388,14 @@ static int ensure_wavefront(cupbr_renderer* r, uint32_t paths, uint32_t grid_thr
 int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp) {
     const uint32_t w = r->width, h = r->height, npix = w * h;
     if (npix == 0 || spp == 0) return CUPBR_OK;
-    // paths in flight: enough to fill the GPU at the deepest bounce, bounded to ~8M
+    const bool nrc = r->nrc != nullptr;
+    // paths in flight: enough to fill the GPU at the deepest bounce, bounded to ~8M; with a
+    // radiance cache one frame per batch (the cache trains between frames)
     const uint32_t target = 1u << 22, cap = 1u << 23;
     uint32_t batch = std::max<uint32_t>(1, (target + npix - 1) / npix);
     batch = std::min(batch, std::max<uint32_t>(1, cap / npix));
     batch = std::min(batch, spp);
+    if (nrc) batch = 1;
     const int blocks = device_cu_count() * 8;
     const uint32_t grid_threads = (uint32_t)blocks * kPtBlock;
     int rc = ensure_wavefront(r, batch * npix, grid_threads);
@@ -330,33 +409,70 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     const uint32_t bounces = std::max<uint32_t>(1, r->max_trace_depth);  // do { } while (depth < max)
     auto extend = r->counting ? k_pt_extend<true> : k_pt_extend<false>;
     auto shadow = r->counting ? k_pt_shadow<true> : k_pt_shadow<false>;
+    auto shade = nrc ? k_pt_shade<true> : k_pt_shade<false>;
+    NrcShade ns{nullptr, nullptr, 0};
+    uint32_t train_thresh = 0;
+    if (nrc) {
+        if ((rc = r->nrc_queries.ensure((size_t)batch * npix * 80)) ||
+            (rc = r->nrc_pending.ensure((size_t)r->nrc_max_rows * 96)) ||
+            (rc = r->nrc_rows.ensure((size_t)r->nrc_max_rows * 80)))
+            return rc;
+        ns = NrcShade{r->nrc_queries.as<float4>(), r->nrc_pending.as<float4>(), r->nrc_max_rows};
+        const double frac = std::min(1.0, (double)r->nrc_train_paths / (double)npix);
+        train_thresh = frac >= 1.0 ? 0xFFFFFFFFu : (uint32_t)(frac * 4294967296.0);
+    }
     if ((rc = stage_mark(r, -1))) return rc;
     for (uint32_t s0 = 0; s0 < spp; s0 += batch) {
         const uint32_t nb = std::min(batch, spp - s0);
         const uint32_t npath = nb * npix;
         hipLaunchKernelGGL(k_pt_raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
                            h, frame0 + s0, npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
-                           wf.weight.as<float4>(), wf.radiance.as<float4>(), c);
+                           wf.weight.as<float4>(), wf.radiance.as<float4>(), c, train_thresh);
         if ((rc = stage_mark(r, 3))) return rc;
         for (uint32_t b = 0; b < bounces; ++b) {
             const int cur = (int)(b & 1);
             hipLaunchKernelGGL(extend, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
                                wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill, grid_threads);
             if ((rc = stage_mark(r, 0))) return rc;
-            hipLaunchKernelGGL(k_pt_shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
+            hipLaunchKernelGGL(shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
                                (int)r->russian_roulette, wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),
                                wf.q_hit.as<float2>(), wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(),
                                wf.s_o.as<float4>(), wf.s_d.as<float4>(), wf.s_c.as<float4>(), wf.weight.as<float4>(),
-                               wf.radiance.as<float4>(), c);
+                               wf.radiance.as<float4>(), c, ns);
             if ((rc = stage_mark(r, 1))) return rc;
             hipLaunchKernelGGL(shadow, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, wf.s_o.as<float4>(),
                                wf.s_d.as<float4>(), wf.s_c.as<float4>(), wf.radiance.as<float4>(), c, spill,
                                grid_threads);
             if ((rc = stage_mark(r, 2))) return rc;
         }
+        if (nrc) {
+            // throughput * cache for the paths that ended at a cache vertex, then the training rows
+            if ((rc = nrc_infer(r->nrc, r->nrc_queries.as<float>(), npath, &c->query_count.v, 20,
+                                wf.radiance.as<float>(), true, r->stream)))
+                return rc;
+            hipLaunchKernelGGL(k_nrc_rows, dim3((r->nrc_max_rows + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0,
+                               r->stream, r->nrc_pending.as<float4>(), c, r->nrc_max_rows, wf.radiance.as<float4>(),
+                               r->nrc_rows.as<float4>());
+            if ((rc = stage_mark(r, 3))) return rc;
+        }
         hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, img,
                            wf.radiance.as<float4>(), npix, frame0 + s0, nb);
         if ((rc = stage_mark(r, 3))) return rc;
+        if (nrc) {
+            uint32_t counts[2] = {0, 0};
+            CUPBR_HIP_CHECK(hipMemcpyAsync(&counts[0], &c->query_count.v, 4, hipMemcpyDeviceToHost, r->stream));
+            CUPBR_HIP_CHECK(hipMemcpyAsync(&counts[1], &c->row_count.v, 4, hipMemcpyDeviceToHost, r->stream));
+            CUPBR_HIP_CHECK(hipStreamSynchronize(r->stream));
+            const uint32_t n_rows = std::min(counts[1], r->nrc_max_rows);
+            r->nrc_queries_total += counts[0];
+            r->nrc_rows_total += n_rows;
+            if (n_rows > 0 && r->nrc_steps > 0) {
+                if ((rc = nrc_train(r->nrc, r->nrc_rows.as<float>(), n_rows, r->nrc_batch, r->nrc_steps, r->nrc_lr,
+                                    nullptr, r->stream)))
+                    return rc;
+            }
+            if ((rc = stage_mark(r, 3))) return rc;
+        }
     }
     CUPBR_HIP_CHECK(hipGetLastError());
     return CUPBR_OK;

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "device.h"
+#include "nrc.h"
 #include "scene.h"
 
 namespace cupbr {
@@ -23,6 +24,8 @@ struct Counters {
     CounterLine queue_count[2];  // extension queues (ping-pong by bounce parity)
     CounterLine shadow_count;
     CounterLine work_extend;
+    CounterLine query_count;  // radiance-cache queries of the batch
+    CounterLine row_count;    // radiance-cache training rows of the batch
     unsigned long long extension_rays;
     unsigned long long shadow_rays;
     unsigned long long ext_nodes, ext_prims, shadow_nodes, shadow_prims;  // traversal counting
@@ -82,6 +85,15 @@ struct cupbr_renderer {
     bool profiling = false;
     bool counting = false;
     cupbr::StageTimer timer;
+    // radiance cache (PATHTRACER only; nullptr = plain PT)
+    cupbr_nrc* nrc = nullptr;
+    uint32_t nrc_train_paths = 1u << 16;  // training paths per frame (full-length PT paths)
+    uint32_t nrc_steps = 16;              // Adam steps per frame
+    uint32_t nrc_batch = 1u << 14;        // rows per step
+    float nrc_lr = 1e-3f;
+    uint32_t nrc_max_rows = 1u << 18;     // training rows kept per frame
+    cupbr::DevBuf nrc_queries, nrc_pending, nrc_rows;
+    uint64_t nrc_queries_total = 0, nrc_rows_total = 0;  // of the last cupbr_render call
 };
 
 namespace cupbr {

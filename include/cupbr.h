@@ -165,6 +165,8 @@ typedef struct cupbr_render_stats {
      * tested by the extension (closest-hit) and shadow (any-hit) traversals */
     uint64_t extension_node_visits, extension_prim_tests;
     uint64_t shadow_node_visits, shadow_prim_tests;
+    /* with a radiance cache attached: cache queries answered and training rows collected */
+    uint64_t cache_queries, cache_training_rows;
 } cupbr_render_stats;
 
 typedef struct cupbr_scene cupbr_scene;
@@ -326,6 +328,39 @@ int cupbr_grid_adjoint(const cupbr_scene* scene, const cupbr_camera* camera, con
                        const float* density, const float* proposal, const float* surface_albedo, uint32_t width,
                        uint32_t height, uint32_t frame0, uint32_t spp, uint32_t max_depth, const float* d_image,
                        float* grad_density, float* grad_albedo);
+
+/* ------------------------------------------------------------------ neural radiance cache
+ * No counterpart in the reference. A 64 -> 4 x 64 (ReLU) -> 3 MLP without biases (16,576 weights)
+ * predicting outgoing radiance from 16 raw inputs per query: position[3] (world), outgoing
+ * direction[3], normal[3], roughness, diffuse albedo[3], specular albedo[3]. BF16 matrix-core
+ * products with FP32 accumulation; FP32 master weights trained with Adam on the relative L2 loss
+ * sum_c (y_c - t_c)^2 / (y_c^2 + 0.01). Positions are normalised to the box given at creation.
+ * Weights: W1[64][64] | W2 | W3 | W4 | W5[3][64], FP32, row = output. */
+typedef struct cupbr_nrc cupbr_nrc;
+#define CUPBR_NRC_PARAMS 16576
+#define CUPBR_NRC_ROW_FLOATS 20 /* training row: 16 inputs, 3 targets, 1 unused */
+int cupbr_nrc_create(const float box_min[3], const float box_max[3], uint32_t seed, cupbr_nrc** out);
+int cupbr_nrc_destroy(cupbr_nrc* cache);
+int cupbr_nrc_get_weights(const cupbr_nrc* cache, float* weights);   /* CUPBR_NRC_PARAMS floats */
+int cupbr_nrc_set_weights(cupbr_nrc* cache, const float* weights);   /* also resets Adam */
+/* predictions out[n][3] for inputs[n][16] (device pointers) */
+int cupbr_nrc_infer_device(cupbr_nrc* cache, const float* inputs, uint32_t n, float* out, void* hip_stream);
+/* `steps` Adam steps (learning rate lr, betas 0.9 / 0.99), each on `batch` rows (a multiple of 512)
+ * drawn with replacement from rows[n_rows][CUPBR_NRC_ROW_FLOATS] (device); when loss is not NULL,
+ * loss[s] += the summed relative L2 loss of step s's batch (device, caller zeroes) */
+int cupbr_nrc_train_device(cupbr_nrc* cache, const float* rows, uint32_t n_rows, uint32_t batch, uint32_t steps,
+                           float lr, float* loss, void* hip_stream);
+/* Attaches a cache to a PATHTRACER renderer (NULL detaches: plain PT). Each frame then traces
+ * `training_paths` pixels (chosen by a hash of pixel and frame) as full PT paths whose vertices
+ * become training rows (target = the path's radiance after the vertex divided by its throughput
+ * there), and ends every other path at its first diffuse or glossy vertex after the camera hit
+ * with throughput * cache prediction. After each frame the cache takes `train_steps` Adam steps of
+ * `batch` rows (a multiple of 512) at learning rate lr. Images stay the running mean of frames. */
+int cupbr_renderer_set_radiance_cache(cupbr_renderer* r, cupbr_nrc* cache, uint32_t training_paths,
+                                      uint32_t train_steps, uint32_t batch, float learning_rate);
+/* self-check of the matrix-core fragment layout: d[16][16] = a[16][32] b[32][16] (host, row-major,
+ * inputs rounded to BF16) through one instruction */
+int cupbr_nrc_matrix_core_product(const float* a, const float* b, float* d);
 
 #ifdef __cplusplus
 }
