@@ -40,9 +40,10 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["pt", "grid"], default="pt",
+    ap.add_argument("--workload", choices=["pt", "grid", "nrc"], default="pt",
                     help="pt: the headline PT+NEE frame; grid: one gradient evaluation (forward + path-replay "
-                         "adjoint) of the 64^3 heterogeneous medium at 256x256, 64 spp")
+                         "adjoint) of the 64^3 heterogeneous medium at 256x256, 64 spp; nrc: equal-time rel-MSE of "
+                         "1-spp frames with the neural radiance cache against plain PT on the 1080p/1M-triangle scene")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run three device-to-device copies of MIB MiB (a plain copy kernel of known "
                          "size, for scaling the HBM byte counters of a --pmc run)")
@@ -73,6 +74,8 @@ def main():
         del src, dst
     if args.workload == "grid":
         return bench_grid(args, log)
+    if args.workload == "nrc":
+        return bench_nrc(args, log)
     W, H, SPP = args.width, args.height, args.spp
     t0 = time.time()
     desc, cam_args = scenes.million_triangles(W, H)
@@ -301,6 +304,126 @@ def bench_grid(args, log):
                      "kernel": "k_grid<true> (path-replay adjoint)",
                      "bytes_per_launch": int(adj_bytes), "avg_launch_ms": round(adj_launch_ms, 4)},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_nrc(args, log):
+    """Equal-time comparison on the 1080p / 1M-triangle scene (depth 8).
+
+    reference  plain PT, 4096 spp (64 calls of 64 spp; the running mean over 4096 frames)
+    cache      online training over warm-up frames (successive frames, 2^16 training paths and 16
+               Adam steps of 2^14 rows per frame), then `steps` timed 1-spp frames from frame 0
+               (inference + training rows + the frame's training included)
+    plain PT   k spp with k = round(cache frame time / plain 1-spp frame time), frames from 0
+    value      rel-MSE(plain PT, k spp) / rel-MSE(cache, 1 spp); higher is better
+    roofline   the inference kernel (matrix cores): 2 * 16,576 flops per query / launch time"""
+    import numpy as np
+    import torch
+
+    import cupbr_amd as cb
+    from cupbr_amd import scenes
+
+    W, H, DEPTH = args.width, args.height, args.depth
+    desc, ca = scenes.million_triangles(W, H)
+    scene = cb.Scene.from_desc(desc)
+    scene.upload()
+    cam = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], W, H)
+
+    def renderer(cache=None):
+        r = cb.PBRenderer(cb.RenderingMethod.PATHTRACER)
+        r.set_output_size(W, H)
+        r.register_scene(scene)
+        r.set_max_trace_depth(DEPTH)
+        if cache is not None:
+            r.set_radiance_cache(cache, training_paths=1 << 16, train_steps=16, batch=1 << 14, learning_rate=1e-3)
+        return r
+
+    def rel_mse(x, ref):
+        return float(np.mean((x.astype(np.float64) - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+    def frame_ms(r, spp):
+        r.reset()
+        r.synchronize()
+        t0 = time.perf_counter()
+        r.render(cam, spp)
+        r.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    t0 = time.perf_counter()
+    R = renderer()
+    for _ in range(64):
+        R.render(cam, 64)
+    ref = R.get_output_image().astype(np.float64)
+    log(f"reference: 4096 spp in {time.perf_counter() - t0:.1f}s")
+    del R
+
+    P = renderer()
+    frame_ms(P, 1)
+    t_pt = float(np.median([frame_ms(P, 1) for _ in range(5)]))
+    cache = cb.RadianceCache((-10.0, -10.0, -10.0), (110.0, 110.0, 110.0), seed=1)
+    N = renderer(cache)
+    t0 = time.perf_counter()
+    warm = 300
+    for _ in range(warm):
+        N.render(cam, 1)
+    N.synchronize()
+    log(f"cache: {warm} training frames in {time.perf_counter() - t0:.1f}s")
+    times, errs = [], []
+    for _ in range(args.steps):
+        times.append(frame_ms(N, 1))
+        errs.append(rel_mse(N.get_output_image(), ref))
+    st = N.stats()
+    t_nrc, e_nrc = float(np.median(times)), float(np.mean(errs))
+    k = max(1, int(round(t_nrc / t_pt)))
+    P.reset()
+    P.render(cam, k)
+    e_pt = rel_mse(P.get_output_image(), ref)
+    P.reset()
+    P.render(cam, 1)
+    e_pt1 = rel_mse(P.get_output_image(), ref)
+    ratio = e_pt / e_nrc
+    log(f"plain PT: 1 spp {t_pt:.2f} ms rel-MSE {e_pt1:.4f}; {k} spp (equal time) rel-MSE {e_pt:.4f}")
+    log(f"cache: frame {t_nrc:.2f} ms rel-MSE {e_nrc:.4f} ({st['cache_queries']} queries, "
+        f"{st['cache_training_rows']} training rows); equal-time ratio {ratio:.2f}")
+
+    # inference kernel alone, timed with HIP events on torch's stream
+    q = int(st["cache_queries"])
+    x = torch.rand(max(q, 1 << 20), 16, device="cuda")
+    x[:, 3:6] = torch.nn.functional.normalize(x[:, 3:6] - 0.5, dim=1)
+    x[:, 6:9] = torch.nn.functional.normalize(x[:, 6:9] - 0.5, dim=1)
+    cache.infer(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        cache.infer(x)
+    e1.record()
+    torch.cuda.synchronize()
+    inf_ms = e0.elapsed_time(e1) / 10
+    tflops = 2 * 16576 * x.shape[0] / (inf_ms * 1e-3) / 1e12
+    log(f"inference: {x.shape[0]} queries in {inf_ms:.3f} ms = {tflops:.1f} TFLOP/s (BF16 matrix cores)")
+    out = {
+        "metric": "equal-time rel-MSE ratio, plain PT / neural radiance cache (1080p/1M-tri)",
+        "value": round(ratio, 3),
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": warm,
+        "ms_per_step": round(t_nrc, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded procedural scene, cupbr_amd.scenes.million_triangles)",
+        "config": {"workload": f"PT+NEE {W}x{H}, depth {DEPTH}, 1 spp per frame with a 64->4x64->3 radiance cache "
+                               f"trained online; plain PT at equal time; reference 4096 spp",
+                   "rel_mse_cache": round(e_nrc, 5), "rel_mse_pt_equal_time": round(e_pt, 5),
+                   "pt_spp_equal_time": k, "cache_frame_ms": round(t_nrc, 3), "pt_frame_ms": round(t_pt, 3),
+                   "rel_mse_pt_1spp": round(e_pt1, 5)},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tflops / 2500.0, 5), "traffic": None, "kernel": "k_nrc_infer<false>",
+                     "avg_launch_ms": round(inf_ms, 4), "queries_per_launch": int(x.shape[0])},
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

@@ -25,7 +25,16 @@ namespace cupbr {
 
 constexpr int kPtBlock = 256;
 constexpr int kPtWaves = kPtBlock / 64;
-constexpr int kPtLds = 32;
+#ifndef CUPBR_PT_LDS
+#define CUPBR_PT_LDS 32
+#endif
+#ifndef CUPBR_PT_KEYS
+#define CUPBR_PT_KEYS 0
+#endif
+// traversal stack entries per lane kept in LDS; with kPtKeys the extension kernel also keeps each
+// entry's entry distance (a second LDS array of the same size) to skip entries behind the best hit
+constexpr int kPtLds = CUPBR_PT_LDS;
+constexpr bool kPtKeys = CUPBR_PT_KEYS != 0;
 
 __device__ __forceinline__ uint32_t wave_fetch(uint32_t* counter) {
     uint32_t base = 0;
@@ -111,8 +120,9 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
 template <bool kCount>
 __global__ __launch_bounds__(kPtBlock) void k_pt_extend(DevScene S, int cur, const float4* __restrict__ q_o,
                                                         const float4* __restrict__ q_d, float2* __restrict__ q_hit,
-                                                        Counters* c, uint32_t* spill, uint32_t spill_stride) {
-    __shared__ uint32_t lds[kPtLds * kPtBlock];
+                                                        Counters* c, uint32_t* spill, uint32_t spill_stride,
+                                                        uint32_t spill_entries) {
+    __shared__ uint32_t lds[(kPtKeys ? 2 : 1) * kPtLds * kPtBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = c->queue_count[cur].v;
     if (gtid == 0) {
@@ -121,6 +131,9 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_extend(DevScene S, int cur, con
         c->extension_rays += n;
     }
     LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
+    // keys: second LDS half; their spill region follows the entries' (spill_entries * spill_stride)
+    LaneStack<kPtLds> ks{lds + kPtLds * kPtBlock + threadIdx.x, kPtBlock,
+                         spill ? spill + (size_t)spill_entries * spill_stride + gtid : nullptr, spill_stride};
     TravCount tc;
     for (;;) {
         const uint32_t base = wave_fetch(&c->work_extend.v);
@@ -129,7 +142,8 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_extend(DevScene S, int cur, con
         if (i < n) {
             const float4 o4 = q_o[i], d4 = q_d[i];
             TraceResult r;
-            traverse<false, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r, &tc);
+            traverse<false, kPtLds, kCount, kPtKeys>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r,
+                                                     &tc, &ks);
             q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
         }
     }
@@ -380,7 +394,7 @@ static int ensure_wavefront(cupbr_renderer* r, uint32_t paths, uint32_t grid_thr
     if ((rc = wf.counters.ensure(sizeof(Counters)))) return rc;
     const uint32_t entries = r->scene->stack_entries;
     if (entries > (uint32_t)kPtLds) {
-        if ((rc = wf.spill.ensure((size_t)(entries - kPtLds) * grid_threads * 4))) return rc;
+        if ((rc = wf.spill.ensure((size_t)(kPtKeys ? 2 : 1) * (entries - kPtLds) * grid_threads * 4))) return rc;
     }
     return CUPBR_OK;
 }
@@ -432,7 +446,8 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
         for (uint32_t b = 0; b < bounces; ++b) {
             const int cur = (int)(b & 1);
             hipLaunchKernelGGL(extend, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
-                               wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill, grid_threads);
+                               wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill, grid_threads,
+                               r->scene->stack_entries > (uint32_t)kPtLds ? r->scene->stack_entries - kPtLds : 0u);
             if ((rc = stage_mark(r, 0))) return rc;
             hipLaunchKernelGGL(shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
                                (int)r->russian_roulette, wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),

@@ -2,6 +2,7 @@
 against a plain PyTorch FP32 reference of the same network (with the kernel's BF16 roundings
 emulated), and the cache inside the path tracer."""
 import math
+import time
 
 import numpy as np
 import pytest
@@ -153,12 +154,31 @@ def test_training_fits_a_smooth_function():
     rows[:, :16] = x
     rows[:, 16:19] = 0.5 + 0.3 * np.sin(4 * x[:, 0:3]) * x[:, 10:13]
     r = torch.from_numpy(rows).cuda()
-    losses = cache.train(r, batch=4096, steps=400, lr=3e-3, return_loss=True).cpu().numpy() / 4096
+    steps, lr = 400, 1e-2
+    losses = cache.train(r, batch=4096, steps=steps, lr=lr, return_loss=True).cpu().numpy() / 4096
     print(f"fit: relative L2 loss {losses[0]:.4f} -> {losses[-20:].mean():.5f}", flush=True)
-    assert losses[-20:].mean() < 0.05 * losses[0]
+    assert losses[-20:].mean() < 0.02 * losses[0]
     pred = cache.infer(r[:, :16]).cpu().numpy()
-    rel = np.sqrt(np.mean((pred - rows[:, 16:19]) ** 2)) / np.sqrt(np.mean(rows[:, 16:19] ** 2))
-    assert rel < 0.05, rel
+    y = rows[:, 16:19]
+    rel = np.sqrt(np.mean((pred - y) ** 2)) / np.sqrt(np.mean(y ** 2))
+    rel_const = np.sqrt(np.mean((y.mean(0) - y) ** 2)) / np.sqrt(np.mean(y ** 2))  # best constant predictor
+    # the same optimisation in PyTorch FP32 (Adam, same loss, batch and step count)
+    f_all = encode(r[:, :16], (0, 0, 0), (1, 1, 1))
+    g = torch.Generator(device="cuda").manual_seed(0)
+    W = torch.from_numpy(cb.RadianceCache((0, 0, 0), (1, 1, 1), seed=5).get_weights()).cuda().requires_grad_(True)
+    opt = torch.optim.Adam([W], lr=lr, betas=(0.9, 0.99), eps=1e-10)
+    for _ in range(steps):
+        idx = torch.randint(0, n, (4096,), device="cuda", generator=g)
+        o = mlp(W, f_all[idx], emulate_bf16=False)
+        loss = ((o - r[idx, 16:19]) ** 2 / (o.detach() ** 2 + 0.01)).sum() / 4096
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+    with torch.no_grad():
+        p_ref = mlp(W, f_all, emulate_bf16=False).cpu().numpy()
+    rel_ref = np.sqrt(np.mean((p_ref - y) ** 2)) / np.sqrt(np.mean(y ** 2))
+    print(f"fit: rms relative error {rel:.4f}; PyTorch FP32 {rel_ref:.4f}; best constant {rel_const:.4f}", flush=True)
+    assert rel < 0.5 * rel_const and rel < 1.5 * rel_ref
 
 
 def _cornell(w, h):
@@ -202,55 +222,49 @@ def rel_mse(x, ref):
 
 
 def test_cache_lowers_equal_time_error_on_the_cornell_box():
-    """After online training, a 1-spp frame with the cache has lower rel-MSE against a converged
-    image than plain PT given the same GPU time (frame time measured with HIP events)."""
-    w = h = 128
-    scene, cam = _cornell(w, h)
+    """After online training, a 1-spp frame with the cache has a lower rel-MSE against a converged
+    image than plain PT given the same GPU time. The light is lowered to y = 0.9: at 0.999 the
+    two-sided light next to the ceiling makes direct-lighting fireflies dominate every estimator's
+    error, which a cache of indirect light cannot change."""
+    w = h = 256
+    desc, ca = scenes.cornell_box()
+    desc["lights"][0, 2] = 0.9
+    scene = cb.Scene.from_desc(desc)
+    cam = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], w, h)
     ref = _pt(scene, cam, w, h, 2048).get_output_image().astype(np.float64)
     cache = cb.RadianceCache((0, 0, 0), (1, 1, 1), seed=1)
     r = cb.PBRenderer(cb.RenderingMethod.PATHTRACER)
     r.set_output_size(w, h)
     r.register_scene(scene)
     r.set_max_trace_depth(5)
-    r.set_radiance_cache(cache, training_paths=4096, train_steps=4, batch=4096, learning_rate=2e-3)
+    r.set_radiance_cache(cache, training_paths=8192, train_steps=4, batch=4096, learning_rate=2e-3)
     for _ in range(300):  # online training: successive frames (fresh seeds), one at a time
         r.render(cam, 1)
-    r.synchronize()
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    errs, times = [], []
-    for _ in range(8):
-        r.reset()
-        torch.cuda.synchronize()
-        t0.record()
-        r.render(cam, 1)
-        r.synchronize()
-        t1.record()
-        torch.cuda.synchronize()
-        times.append(t0.elapsed_time(t1))
-        errs.append(rel_mse(r.get_output_image(), ref))
-    t_nrc, e_nrc = float(np.median(times)), float(np.mean(errs))
+
+    def frame_ms(rd, spp):
+        ts = []
+        for _ in range(5):
+            rd.reset()
+            rd.synchronize()
+            t0 = time.perf_counter()
+            rd.render(cam, spp)
+            rd.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts))
+
+    t_nrc = frame_ms(r, 1)
+    r.reset()
+    r.render(cam, 1)
+    e_nrc = rel_mse(r.get_output_image(), ref)
     p = cb.PBRenderer(cb.RenderingMethod.PATHTRACER)
     p.set_output_size(w, h)
     p.register_scene(scene)
     p.set_max_trace_depth(5)
-    p.render(cam, 1)
-    p.synchronize()
-    tp = []
-    for _ in range(8):
-        p.reset()
-        torch.cuda.synchronize()
-        t0.record()
-        p.render(cam, 1)
-        p.synchronize()
-        t1.record()
-        torch.cuda.synchronize()
-        tp.append(t0.elapsed_time(t1))
-    t_pt = float(np.median(tp))
-    k = max(1, int(t_nrc / t_pt))
+    t_pt = frame_ms(p, 1)
+    k = max(1, round(t_nrc / t_pt))
     p.reset()
-    p.render(cam, k)  # frames 0..k-1: k spp in the NRC frame's time
+    p.render(cam, k)  # frames 0..k-1: the spp plain PT affords in the NRC frame's time
     e_pt = rel_mse(p.get_output_image(), ref)
-    print(f"cornell 128^2: NRC frame {t_nrc:.3f} ms rel-MSE {e_nrc:.4f}; PT 1 spp {t_pt:.3f} ms -> {k} spp at equal "
+    print(f"cornell 256^2: NRC frame {t_nrc:.3f} ms rel-MSE {e_nrc:.4f}; PT 1 spp {t_pt:.3f} ms -> {k} spp at equal "
           f"time, rel-MSE {e_pt:.4f}; ratio {e_pt / e_nrc:.2f}", flush=True)
-    assert e_nrc < e_pt
+    assert e_nrc * 1.5 < e_pt
