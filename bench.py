@@ -113,6 +113,8 @@ def main():
     r.set_traversal_counting(False)
     ext_rays = sc["extension_rays"]
     bytes_per_ext_ray = (sc["extension_node_visits"] * 128 + sc["extension_prim_tests"] * 48 + ext_rays * 40) / max(ext_rays, 1)
+    sh_rays = sc["shadow_rays"]
+    bytes_per_sh_ray = (sc["shadow_node_visits"] * 128 + sc["shadow_prim_tests"] * 48 + sh_rays * 48) / max(sh_rays, 1)
 
     for _ in range(args.warmup):
         step()
@@ -136,7 +138,8 @@ def main():
     value = samples / t_total / 1e6
     ms_per_step = t_total / args.steps * 1e3
     ext_launch_ms = stage_ms[0] / max(stage_n[0], 1)
-    ext_bytes_per_launch = bytes_per_ext_ray * rays[0] / max(stage_n[0], 1)
+    fused = stage_n[2] == 0  # one trace kernel per bounce takes extension and shadow rays
+    ext_bytes_per_launch = (bytes_per_ext_ray * rays[0] + (bytes_per_sh_ray * rays[1] if fused else 0.0)) / max(stage_n[0], 1)
     achieved = ext_bytes_per_launch / (ext_launch_ms * 1e-3) / 1e9
     img = r.get_output_image()
     log(f"GPU: {value:.1f} Msamples/s, {ms_per_step:.1f} ms/step, {(rays.sum()) / t_total / 1e6:.0f} Mrays/s "
@@ -145,7 +148,9 @@ def main():
         f"other {stage_ms[3] / args.steps:.1f}; image mean {img.mean():.4f} finite {bool(np.isfinite(img).all())}; "
         f"rgba mean {tm_buf[..., :3].mean():.1f}")
     log(f"extend: {bytes_per_ext_ray:.0f} algorithmic B/ray ({sc['extension_node_visits'] / max(ext_rays, 1):.1f} nodes, "
-        f"{sc['extension_prim_tests'] / max(ext_rays, 1):.1f} tris per ray), {ext_launch_ms:.3f} ms/launch, {achieved:.0f} GB/s")
+        f"{sc['extension_prim_tests'] / max(ext_rays, 1):.1f} tris per ray); shadow {bytes_per_sh_ray:.0f} B/ray "
+        f"({sc['shadow_node_visits'] / max(sh_rays, 1):.1f} nodes, {sc['shadow_prim_tests'] / max(sh_rays, 1):.1f} tris); "
+        f"{'trace' if fused else 'extend'} kernel {ext_launch_ms:.3f} ms/launch, {achieved:.0f} GB/s")
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "extend_traffic.json")
@@ -195,8 +200,10 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_pt_extend (closest-hit BVH4 traversal)",
-                     "bytes_per_ray": round(bytes_per_ext_ray, 1), "avg_launch_ms": round(ext_launch_ms, 4)},
+                     "kernel": "k_pt_trace (extension closest hit + shadow any hit, BVH4)" if fused else
+                               "k_pt_extend (closest-hit BVH4 traversal)",
+                     "bytes_per_launch": int(ext_bytes_per_launch), "bytes_per_ext_ray": round(bytes_per_ext_ray, 1),
+                     "bytes_per_shadow_ray": round(bytes_per_sh_ray, 1), "avg_launch_ms": round(ext_launch_ms, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -209,6 +209,8 @@ def lib():
             "cupbr_nrc_train_device": (C.c_int, [P, P, U32, U32, U32, C.c_float, P, P]),
             "cupbr_nrc_matrix_core_product": (C.c_int, [P, P, P]),
             "cupbr_renderer_set_radiance_cache": (C.c_int, [P, P, U32, U32, U32, C.c_float]),
+            "cupbr_renderer_save_pfm": (C.c_int, [P, I32, C.c_char_p]),
+            "cupbr_renderer_resume": (C.c_int, [P, P, SZ, U32]),
             "cupbr_grid_set_counting": (C.c_int, [P, I32]),
             "cupbr_grid_get_counts": (C.c_int, [P, P]),
             "cupbr_grid_render_device": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P]),
@@ -623,6 +625,17 @@ class GridRenderer:
         return gr, ga
 
 
+def read_pfm(path) -> np.ndarray:
+    """Reads a colour PFM as written by PBRenderer.save_pfm: (h, w, 3) float32, row 0 = bottom."""
+    with open(path, "rb") as f:
+        if f.readline().strip() != b"PF":
+            raise CupbrError(-5, f"{path}: not a colour PFM")
+        w, h = (int(v) for v in f.readline().split())
+        scale = float(f.readline())
+        data = np.frombuffer(f.read(w * h * 12), dtype="<f4" if scale < 0 else ">f4")
+    return data.reshape(h, w, 3).astype(np.float32)
+
+
 def _numel(x) -> int:
     return int(x.numel()) if hasattr(x, "numel") and callable(x.numel) else int(np.size(x))
 
@@ -733,6 +746,15 @@ class PBRenderer:
     def set_profiling(self, enable: bool):
         """Per-stage device time (HIP events on the renderer's stream), reported by stats()."""
         _check(lib().cupbr_renderer_set_profiling(self._h, int(bool(enable))))
+
+    def save_pfm(self, path: str, which: Image = Image.OUTPUT):
+        """FP32 dump of an image as a colour PFM (rows bottom-up)."""
+        _check(lib().cupbr_renderer_save_pfm(self._h, int(which), str(path).encode()))
+
+    def resume(self, image, frame_index: int):
+        """Continue the running mean of a saved run: output image (h, w, 3) and its frame count."""
+        img = np.ascontiguousarray(image, np.float32)
+        _check(lib().cupbr_renderer_resume(self._h, _ptr(img), img.size, int(frame_index)))
 
     def set_radiance_cache(self, cache: "RadianceCache | None", training_paths: int = 1 << 16, train_steps: int = 16,
                            batch: int = 1 << 14, learning_rate: float = 1e-3):

@@ -477,6 +477,36 @@ int cupbr_renderer_get_image(const cupbr_renderer* r, int32_t which, float* host
     return CUPBR_OK;
 }
 
+// PFM (Portable Float Map, "PF" colour): header "PF\n<w> <h>\n-1.0\n" (negative scale = little
+// endian), then rows from the bottom up, which is the renderer's own row order
+int cupbr_renderer_save_pfm(const cupbr_renderer* r, int32_t which, const char* path) {
+    CHECK_ARG(r && path, "null argument");
+    const size_t n = (size_t)r->width * r->height * 3;
+    std::vector<float> px(n);
+    if (int rc = cupbr_renderer_get_image(r, which, px.data(), n)) return rc;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return set_error(CUPBR_ERR_FILE, std::string("cannot write ") + path);
+    std::fprintf(f, "PF\n%u %u\n-1.0\n", r->width, r->height);
+    const size_t wrote = std::fwrite(px.data(), sizeof(float), n, f);
+    const int closed = std::fclose(f);
+    if (wrote != n || closed != 0) return set_error(CUPBR_ERR_FILE, std::string("write failed: ") + path);
+    return CUPBR_OK;
+}
+
+// restores an accumulation state: the output image (host FP32 RGB, width*height*3) and the frame
+// index, so that the next render continues the running mean where a saved run stopped
+int cupbr_renderer_resume(cupbr_renderer* r, const float* host_rgb, size_t n_floats, uint32_t frame_index) {
+    CHECK_ARG(r && host_rgb, "null argument");
+    if (!r->output_size_set)
+        return set_error(CUPBR_ERR_NO_OUTPUT_SIZE, "[PBRenderer]  No output size set. Use setOutputSize()", true);
+    const size_t need = (size_t)r->width * r->height * 3;
+    if (n_floats < need) return set_error(CUPBR_ERR_BUFFER_SIZE, "image buffer smaller than width*height*3");
+    CUPBR_HIP_CHECK(hipStreamSynchronize(r->stream));
+    CUPBR_HIP_CHECK(hipMemcpy(r->images[CUPBR_IMAGE_OUTPUT].ptr, host_rgb, need * sizeof(float), hipMemcpyHostToDevice));
+    r->frame_index = frame_index;
+    return CUPBR_OK;
+}
+
 // ------------------------------------------------------------------ post processing
 int cupbr_threshold_curve(float threshold, float knee, float curve[4]) {
     CHECK_ARG(curve, "null curve");

@@ -31,6 +31,12 @@ constexpr int kPtWaves = kPtBlock / 64;
 #ifndef CUPBR_PT_KEYS
 #define CUPBR_PT_KEYS 0
 #endif
+#ifndef CUPBR_PT_FUSED
+#define CUPBR_PT_FUSED 0
+#endif
+// fused: one persistent trace kernel per bounce takes the bounce's extension rays and the
+// previous bounce's shadow rays from one work counter (64-ray chunks, extension first)
+constexpr bool kPtFused = CUPBR_PT_FUSED != 0;
 // traversal stack entries per lane kept in LDS; with kPtKeys the extension kernel also keeps each
 // entry's entry distance (a second LDS array of the same size) to skip entries behind the best hit
 constexpr int kPtLds = CUPBR_PT_LDS;
@@ -101,6 +107,8 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
     if (p == 0) {
         c->queue_count[0].v = npath;
         c->work_extend.v = 0;
+        c->shadow_count[0].v = 0;
+        c->shadow_count[1].v = 0;
         c->query_count.v = 0;
         c->row_count.v = 0;
     }
@@ -127,7 +135,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_extend(DevScene S, int cur, con
     const uint32_t n = c->queue_count[cur].v;
     if (gtid == 0) {
         c->queue_count[cur ^ 1].v = 0;
-        c->shadow_count.v = 0;
+        c->shadow_count[cur].v = 0;
         c->extension_rays += n;
     }
     LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
@@ -288,7 +296,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
             nq_o[ns] = no;
             nq_d[ns] = nd;
         }
-        const uint32_t ss = block_append(want_shadow, &c->shadow_count.v, s_append);
+        const uint32_t ss = block_append(want_shadow, &c->shadow_count[cur].v, s_append);
         if (want_shadow) {
             s_o[ss] = so;
             s_d[ss] = sd;
@@ -325,14 +333,81 @@ __global__ __launch_bounds__(kPtBlock) void k_nrc_rows(const float4* __restrict_
     o[4] = make_float4(tgt[0], tgt[1], tgt[2], 0.0f);
 }
 
+// shadow ray i: visible -> radiance of its path += contribution (traceVisibility, :66-73)
+__device__ __forceinline__ void shadow_result(bool occluded, const float4 d4, const float4* __restrict__ s_c,
+                                              uint32_t i, float4* __restrict__ rad) {
+    if (occluded) return;
+    const uint32_t p = __float_as_uint(d4.w);
+    const float4 cc = s_c[i];
+    const float4 R = rad[p];
+    v3 L = mk(R.x, R.y, R.z);
+    L += mk(cc.x, cc.y, cc.z);
+    rad[p] = make_float4(L.x, L.y, L.z, R.w);
+}
+
+// one bounce's extension rays (queue cur) and the previous bounce's shadow rays (parity cur ^ 1)
+// under one work counter: chunk k < ceil(n_ext / 64) is extension rays [64k, 64k + 64), the rest
+// shadow rays; every wave's chunk is of one kind
 template <bool kCount>
-__global__ __launch_bounds__(kPtBlock) void k_pt_shadow(DevScene S, const float4* __restrict__ s_o,
+__global__ __launch_bounds__(kPtBlock) void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o,
+                                                       const float4* __restrict__ q_d, float2* __restrict__ q_hit,
+                                                       const float4* __restrict__ s_o, const float4* __restrict__ s_d,
+                                                       const float4* __restrict__ s_c, float4* __restrict__ rad,
+                                                       Counters* c, uint32_t* spill, uint32_t spill_stride,
+                                                       uint32_t spill_entries) {
+    __shared__ uint32_t lds[(kPtKeys ? 2 : 1) * kPtLds * kPtBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_ext = c->queue_count[cur].v;
+    const uint32_t n_sh = c->shadow_count[cur ^ 1].v;
+    if (gtid == 0) {
+        c->queue_count[cur ^ 1].v = 0;
+        c->shadow_count[cur].v = 0;
+        c->extension_rays += n_ext;
+        c->shadow_rays += n_sh;
+    }
+    LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
+    LaneStack<kPtLds> ks{lds + kPtLds * kPtBlock + threadIdx.x, kPtBlock,
+                         spill ? spill + (size_t)spill_entries * spill_stride + gtid : nullptr, spill_stride};
+    TravCount te, ts;
+    const uint32_t ext_span = (n_ext + 63u) & ~63u;
+    const uint32_t total = ext_span + n_sh;
+    for (;;) {
+        const uint32_t base = wave_fetch(&c->work_extend.v);
+        if (base >= total) break;
+        const uint32_t i = base + __lane_id();
+        if (base < ext_span) {
+            if (i < n_ext) {
+                const float4 o4 = q_o[i], d4 = q_d[i];
+                TraceResult r;
+                traverse<false, kPtLds, kCount, kPtKeys>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st,
+                                                         r, &te, &ks);
+                q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
+            }
+        } else {
+            const uint32_t j = i - ext_span;
+            if (j < n_sh) {
+                const float4 o4 = s_o[j], d4 = s_d[j];
+                TraceResult r;
+                const bool occluded =
+                    traverse<true, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r, &ts);
+                shadow_result(occluded, d4, s_c, j, rad);
+            }
+        }
+    }
+    if (kCount) {
+        wave_add_counts(te, &c->ext_nodes, &c->ext_prims);
+        wave_add_counts(ts, &c->shadow_nodes, &c->shadow_prims);
+    }
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kPtBlock) void k_pt_shadow(DevScene S, int par, const float4* __restrict__ s_o,
                                                         const float4* __restrict__ s_d, const float4* __restrict__ s_c,
                                                         float4* __restrict__ rad, Counters* c, uint32_t* spill,
                                                         uint32_t spill_stride) {
     __shared__ uint32_t lds[kPtLds * kPtBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = c->shadow_count.v;
+    const uint32_t n = c->shadow_count[par].v;
     if (gtid == 0) c->shadow_rays += n;
     LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
     TravCount tc;
@@ -342,14 +417,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shadow(DevScene S, const float4
         TraceResult r;
         const bool occluded =
             traverse<true, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r, &tc);
-        if (!occluded) {  // traceVisibility true: radiance += contribution (:66-73)
-            const uint32_t p = __float_as_uint(d4.w);
-            const float4 cc = s_c[i];
-            const float4 R = rad[p];
-            v3 L = mk(R.x, R.y, R.z);
-            L += mk(cc.x, cc.y, cc.z);
-            rad[p] = make_float4(L.x, L.y, L.z, R.w);
-        }
+        shadow_result(occluded, d4, s_c, i, rad);
     }
     if (kCount) wave_add_counts(tc, &c->shadow_nodes, &c->shadow_prims);
 }
@@ -386,9 +454,11 @@ static int ensure_wavefront(cupbr_renderer* r, uint32_t paths, uint32_t grid_thr
             if ((rc = wf.q_d[k].ensure(n * 16))) return rc;
         }
         if ((rc = wf.q_hit.ensure(n * 8))) return rc;
-        if ((rc = wf.s_o.ensure(n * 16))) return rc;
-        if ((rc = wf.s_d.ensure(n * 16))) return rc;
-        if ((rc = wf.s_c.ensure(n * 16))) return rc;
+        for (int k = 0; k < (kPtFused ? 2 : 1); ++k) {
+            if ((rc = wf.s_o[k].ensure(n * 16))) return rc;
+            if ((rc = wf.s_d[k].ensure(n * 16))) return rc;
+            if ((rc = wf.s_c[k].ensure(n * 16))) return rc;
+        }
         wf.capacity = paths;
     }
     if ((rc = wf.counters.ensure(sizeof(Counters)))) return rc;
@@ -423,6 +493,9 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     const uint32_t bounces = std::max<uint32_t>(1, r->max_trace_depth);  // do { } while (depth < max)
     auto extend = r->counting ? k_pt_extend<true> : k_pt_extend<false>;
     auto shadow = r->counting ? k_pt_shadow<true> : k_pt_shadow<false>;
+    auto trace = r->counting ? k_pt_trace<true> : k_pt_trace<false>;
+    const uint32_t spill_entries =
+        r->scene->stack_entries > (uint32_t)kPtLds ? r->scene->stack_entries - kPtLds : 0u;
     auto shade = nrc ? k_pt_shade<true> : k_pt_shade<false>;
     NrcShade ns{nullptr, nullptr, 0};
     uint32_t train_thresh = 0;
@@ -445,20 +518,38 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
         if ((rc = stage_mark(r, 3))) return rc;
         for (uint32_t b = 0; b < bounces; ++b) {
             const int cur = (int)(b & 1);
-            hipLaunchKernelGGL(extend, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
-                               wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill, grid_threads,
-                               r->scene->stack_entries > (uint32_t)kPtLds ? r->scene->stack_entries - kPtLds : 0u);
+            const int sp = kPtFused ? cur : 0;  // shadow queue shade(b) appends to
+            if (kPtFused) {
+                hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
+                                   wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
+                                   wf.s_d[cur ^ 1].as<float4>(), wf.s_c[cur ^ 1].as<float4>(),
+                                   wf.radiance.as<float4>(), c, spill, grid_threads, spill_entries);
+            } else {
+                hipLaunchKernelGGL(extend, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
+                                   wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill, grid_threads,
+                                   spill_entries);
+            }
             if ((rc = stage_mark(r, 0))) return rc;
             hipLaunchKernelGGL(shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
                                (int)r->russian_roulette, wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),
                                wf.q_hit.as<float2>(), wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(),
-                               wf.s_o.as<float4>(), wf.s_d.as<float4>(), wf.s_c.as<float4>(), wf.weight.as<float4>(),
-                               wf.radiance.as<float4>(), c, ns);
+                               wf.s_o[sp].as<float4>(), wf.s_d[sp].as<float4>(), wf.s_c[sp].as<float4>(),
+                               wf.weight.as<float4>(), wf.radiance.as<float4>(), c, ns);
             if ((rc = stage_mark(r, 1))) return rc;
-            hipLaunchKernelGGL(shadow, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, wf.s_o.as<float4>(),
-                               wf.s_d.as<float4>(), wf.s_c.as<float4>(), wf.radiance.as<float4>(), c, spill,
-                               grid_threads);
-            if ((rc = stage_mark(r, 2))) return rc;
+            if (!kPtFused) {
+                hipLaunchKernelGGL(shadow, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.s_o[0].as<float4>(),
+                                   wf.s_d[0].as<float4>(), wf.s_c[0].as<float4>(), wf.radiance.as<float4>(), c, spill,
+                                   grid_threads);
+                if ((rc = stage_mark(r, 2))) return rc;
+            }
+        }
+        if (kPtFused) {  // the last bounce's shadow rays (its extension queue is empty)
+            const int cur = (int)(bounces & 1);
+            hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
+                               wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
+                               wf.s_d[cur ^ 1].as<float4>(), wf.s_c[cur ^ 1].as<float4>(), wf.radiance.as<float4>(),
+                               c, spill, grid_threads, spill_entries);
+            if ((rc = stage_mark(r, 0))) return rc;
         }
         if (nrc) {
             // throughput * cache for the paths that ended at a cache vertex, then the training rows

@@ -22,7 +22,7 @@ struct alignas(128) CounterLine {
 };
 struct Counters {
     CounterLine queue_count[2];  // extension queues (ping-pong by bounce parity)
-    CounterLine shadow_count;
+    CounterLine shadow_count[2]; // shadow queues (bounce parity)
     CounterLine work_extend;
     CounterLine query_count;  // radiance-cache queries of the batch
     CounterLine row_count;    // radiance-cache training rows of the batch
@@ -60,7 +60,8 @@ struct Wavefront {
     DevBuf radiance;        // float4 per path
     DevBuf q_o[2], q_d[2];  // float4 per entry: origin | direction.xyz, path (bits)
     DevBuf q_hit;           // float2 per entry: t, prim (bits)
-    DevBuf s_o, s_d, s_c;   // shadow queue: origin.xyz+distance | direction.xyz+path | contribution
+    DevBuf s_o[2], s_d[2], s_c[2];  // shadow queues by bounce parity: origin.xyz+distance |
+                                    // direction.xyz+path | contribution
     DevBuf counters;
     DevBuf spill;
     uint32_t spill_entries = 0;

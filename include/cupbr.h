@@ -248,6 +248,11 @@ int cupbr_render(cupbr_renderer* r, const cupbr_camera* camera, uint32_t spp);
 int cupbr_renderer_get_image(const cupbr_renderer* r, int32_t which, float* host_rgb, size_t n_floats);
 int cupbr_renderer_get_image_device(const cupbr_renderer* r, int32_t which, float** device_rgb);
 int cupbr_renderer_get_stats(const cupbr_renderer* r, cupbr_render_stats* out);
+/* extension: FP32 dumps and resume. save_pfm writes image `which` as a colour PFM (rows bottom-up,
+ * little endian); resume loads an output image and frame index so that render continues the
+ * running mean of a saved run (PBRenderer.cu:172-186 accumulates by frame index) */
+int cupbr_renderer_save_pfm(const cupbr_renderer* r, int32_t which, const char* path);
+int cupbr_renderer_resume(cupbr_renderer* r, const float* host_rgb, size_t n_floats, uint32_t frame_index);
 /* extension (no reference counterpart): instrumentation for throughput and roofline reports */
 int cupbr_renderer_set_profiling(cupbr_renderer* r, int32_t enable);
 int cupbr_renderer_set_traversal_counting(cupbr_renderer* r, int32_t enable);

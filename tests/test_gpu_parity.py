@@ -412,3 +412,21 @@ def test_cpp_class_interface_renders_like_the_c_abi(tmp_path):
     assert abs(r.get_output_image().astype(np.float64).mean() - mean) <= 1e-6 * max(1.0, abs(mean))
     data = bmp.read_bytes()
     assert data[:2] == b"BM" and len(data) == 54 + 96 * 64 * 4
+
+
+def test_pfm_dump_and_resume_continue_the_running_mean(tmp_path):
+    """save_pfm writes the exact FP32 image; resume(image, 8) followed by 8 frames equals one
+    16-frame render (the running mean continues by frame index, PBRenderer.cu:172-186)."""
+    desc, ca = scenes.cornell_box()
+    cam = _camera(ca, 48, 32)
+    scene = cb.Scene.from_desc(desc)
+    a = _render(scene, cam, 48, 32, 8)
+    path = tmp_path / "acc.pfm"
+    a.save_pfm(str(path))
+    saved = cb.read_pfm(str(path))
+    assert np.array_equal(saved, a.get_output_image())
+    b = _render(scene, cam, 48, 32, 0)
+    b.resume(saved, 8)
+    b.render(cam, 8)
+    assert b.frame_index() == 16
+    assert np.array_equal(b.get_output_image(), _render(scene, cam, 48, 32, 16).get_output_image())

@@ -1,6 +1,7 @@
 """Host-side checks (no GPU): scene loading, the RGBE reader, reference ordering, the BVH build,
 and the C ABI surface of libcupbr.so."""
 import ctypes
+import json
 import os
 import re
 import shutil
@@ -307,3 +308,46 @@ def test_cpp_wrapper_compiles_and_loads_a_scene(tmp_path):
                          text=True, timeout=60)
     assert out.returncode == 0, out.stderr
     assert "3 objects, 0 triangles, 1 lights" in out.stdout  # mesh triangles only
+
+
+def _golden_scenes():
+    with open(os.path.join(ROOT, "tests", "golden", "reference_scenes.json")) as f:
+        return json.load(f)
+
+
+def test_golden_scene_fixtures_are_complete():
+    g = _golden_scenes()
+    assert len(g) == 12
+    assert g["GGX_Test.xml"]["types"] and len(g["GGX_Test.xml"]["types"]) == 10
+    assert g["Suzanne.xml"]["n_tris"] == 968
+
+
+@pytest.mark.parametrize("name", _reference_scenes() or ["(reference absent)"])
+def test_product_loader_matches_golden_scene_fixtures(tmp_path, name):
+    """The product loader reads each shipped scene to the values recorded in
+    tests/golden/reference_scenes.json (written by tests/golden/make_scene_goldens.py)."""
+    if name == "(reference absent)":
+        pytest.skip("reference scenes not mounted")
+    g = _golden_scenes()[name]
+    work = tmp_path / "ref"
+    shutil.copytree(os.path.join(REF, "res", "Scenes"), work / "res" / "Scenes")
+    shutil.copytree(os.path.join(REF, "res", "Meshes"), work / "res" / "Meshes")
+    write_hdr(work / "res" / "hdrtest.hdr", np.full((8, 16, 3), 0.5, np.float32))
+    cwd = os.getcwd()
+    os.chdir(work)
+    try:
+        s = cb.SceneLoader.load_from_file(f"res/Scenes/{name}")
+    finally:
+        os.chdir(cwd)
+    objs = s.objects()
+    assert objs["type"].tolist() == g["types"]
+    np.testing.assert_array_equal(objs["params"], np.array(g["params"], np.float32).reshape(-1, 12))
+    assert objs["first_triangle"].tolist() == g["first"] and objs["triangle_count"].tolist() == g["count"]
+    m = np.array(g["mats"], np.float32).reshape(-1, 21)
+    np.testing.assert_array_equal(objs["albedo_d"], m[:, 3:6])
+    np.testing.assert_array_equal(objs["roughness"], m[:, 10])
+    tris = s.triangles().astype(np.float64)
+    assert len(tris) == g["n_tris"]
+    assert tris.sum() == pytest.approx(g["tris_sum"], rel=1e-12, abs=1e-9)
+    assert np.abs(tris).sum() == pytest.approx(g["tris_abs_sum"], rel=1e-12)
+    np.testing.assert_array_equal(s.volume(), np.array(g["volume"], np.float32))
