@@ -120,6 +120,11 @@ OBJECT_DTYPE = np.dtype([
     ("albedo_e", "<f4", 3), ("albedo_d", "<f4", 3), ("albedo_s", "<f4", 3), ("shininess", "<f4"),
     ("roughness", "<f4"), ("eta", "<f4"), ("sigma_s", "<f4", 3), ("sigma_a", "<f4", 3), ("g", "<f4"),
     ("interface_type", "<i4"), ("material_type", "<i4")])
+# cupbr_material (84 bytes, Material.h:35-43)
+MATERIAL_DTYPE = np.dtype([
+    ("albedo_e", "<f4", 3), ("albedo_d", "<f4", 3), ("albedo_s", "<f4", 3), ("shininess", "<f4"),
+    ("roughness", "<f4"), ("eta", "<f4"), ("sigma_s", "<f4", 3), ("sigma_a", "<f4", 3), ("g", "<f4"),
+    ("interface_type", "<i4"), ("material_type", "<i4")])
 LIGHT_DTYPE = np.dtype([("type", "<i4"), ("position", "<f4", 3), ("intensity", "<f4", 3), ("radiance", "<f4", 3),
                         ("half_extend1", "<f4", 3), ("half_extend2", "<f4", 3)])
 assert OBJECT_DTYPE.itemsize == 144 and LIGHT_DTYPE.itemsize == 64
@@ -210,6 +215,8 @@ def lib():
             "cupbr_nrc_matrix_core_product": (C.c_int, [P, P, P]),
             "cupbr_renderer_set_radiance_cache": (C.c_int, [P, P, U32, U32, U32, C.c_float]),
             "cupbr_renderer_save_pfm": (C.c_int, [P, I32, C.c_char_p]),
+            "cupbr_pick": (C.c_int, [P, P, U32, U32, U32, U32, P, P]),
+            "cupbr_scene_update_material": (C.c_int, [P, I32, P]),
             "cupbr_renderer_resume": (C.c_int, [P, P, SZ, U32]),
             "cupbr_grid_set_counting": (C.c_int, [P, I32]),
             "cupbr_grid_get_counts": (C.c_int, [P, P]),
@@ -402,6 +409,20 @@ class Scene:
         out = np.zeros(max(n, 1), np.int32)
         _check(lib().cupbr_scene_get_reference_order(self.handle, _ptr(out), max(n, 1)))
         return out[:n]
+
+    def pick(self, camera: "Camera", x: int, y: int, width: int, height: int):
+        """Interaction::pickMouse: (scene_index, material record) of the object seen through pixel
+        (x, y), or None on a miss."""
+        m = np.zeros(1, MATERIAL_DTYPE)
+        idx = C.c_int32(-1)
+        _check(lib().cupbr_pick(self.handle, C.byref(camera._c), x, y, width, height, _ptr(m), C.byref(idx)))
+        return None if idx.value < 0 else (idx.value, m[0])
+
+    def update_material(self, scene_index: int, material):
+        """Interaction::updateMaterial: material is a MATERIAL_DTYPE record (e.g. from pick)."""
+        m = np.zeros(1, MATERIAL_DTYPE)
+        m[0] = material
+        _check(lib().cupbr_scene_update_material(self.handle, int(scene_index), _ptr(m)))
 
     def trace_closest(self, rays):
         """Closest hit for n x 6 rays (origin, normalised direction): (object_id, triangle_id, t).

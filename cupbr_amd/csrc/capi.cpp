@@ -264,6 +264,48 @@ int cupbr_trace_visibility(const cupbr_scene* s, const float* rays, const float*
     return CUPBR_OK;
 }
 
+// ------------------------------------------------------------------ picking (Interaction, MousePicker.cu)
+// pickMouse (MousePicker.cu:8-35,54-73): the unjittered ray through pixel (x, y) (TracingDetail.h:34-45);
+// on a hit the object's material and scene index are written, on a miss both are left unchanged
+int cupbr_pick(const cupbr_scene* s, const cupbr_camera* cam, uint32_t x, uint32_t y, uint32_t width,
+               uint32_t height, cupbr_material* out_material, int32_t* out_scene_index) {
+    CHECK_ARG(s && cam && out_material && out_scene_index, "null argument");
+    CHECK_ARG(width && height, "invalid size");
+    const float rx = 2.0f * (((float)x + 0.0f) / (float)width - 0.5f);
+    const float ry = 2.0f * (((float)y + 0.0f) / (float)height - 0.5f);
+    float wp[3], d[3], n2 = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        wp[k] = cam->position[k] + cam->z_axis[k] + rx * cam->x_axis[k] + ry * cam->y_axis[k];
+        d[k] = wp[k] - cam->position[k];
+    }
+    n2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const float nrm = std::sqrt(n2);
+    const float ray[6] = {cam->position[0], cam->position[1], cam->position[2], d[0] / nrm, d[1] / nrm, d[2] / nrm};
+    int32_t obj = -1, tri = -1;
+    float t = 0.0f;
+    if (int rc = cupbr_trace_closest(s, ray, 1, &obj, &tri, &t)) return rc;
+    if (obj >= 0) {
+        *out_material = s->objects[obj].material;
+        *out_scene_index = obj;
+    }
+    return CUPBR_OK;
+}
+
+// updateMaterial (MousePicker.cu:37-51,76-84): replaces the material of object scene_index; a
+// renderer accumulating this scene should be reset by the caller (as the reference's UI does)
+int cupbr_scene_update_material(cupbr_scene* s, int32_t scene_index, const cupbr_material* m) {
+    CHECK_ARG(s && m, "null argument");
+    CHECK_ARG(scene_index >= 0 && (size_t)scene_index < s->objects.size(), "scene index out of range");
+    s->objects[scene_index].material = *m;
+    s->h_mats[scene_index] = *m;
+    if (s->uploaded) {
+        CUPBR_HIP_CHECK(hipDeviceSynchronize());
+        CUPBR_HIP_CHECK(hipMemcpy(const_cast<cupbr_material*>(s->dev.mats) + scene_index, m, sizeof(cupbr_material),
+                                  hipMemcpyHostToDevice));
+    }
+    return CUPBR_OK;
+}
+
 // ------------------------------------------------------------------ renderer
 static bool valid_method(int32_t m) { return m >= CUPBR_RAYTRACER && m <= CUPBR_VOLUME; }
 

@@ -228,6 +228,14 @@ int cupbr_trace_visibility(const cupbr_scene* scene, const float* rays, const fl
 int cupbr_trace_visibility_device(const cupbr_scene* scene, const float* rays, const float* distance, size_t n,
                                   uint8_t* visible, void* hip_stream);
 
+/* ------------------------------------------------------------------ picking (Interaction/MousePicker.h:13-33) */
+/* Interaction::pickMouse: closest hit of the unjittered ray through pixel (x, y); on a hit writes
+ * the object's material and scene index, on a miss leaves both unchanged (MousePicker.cu:23-34) */
+int cupbr_pick(const cupbr_scene* scene, const cupbr_camera* camera, uint32_t x, uint32_t y, uint32_t width,
+               uint32_t height, cupbr_material* out_material, int32_t* out_scene_index);
+/* Interaction::updateMaterial: replaces the material of object scene_index (MousePicker.cu:37-51) */
+int cupbr_scene_update_material(cupbr_scene* scene, int32_t scene_index, const cupbr_material* material);
+
 /* ------------------------------------------------------------------ renderer (PBRenderer.h:27-116) */
 int cupbr_renderer_create(int32_t method, cupbr_renderer** out);       /* PBRenderer(RenderingMethod) */
 int cupbr_renderer_destroy(cupbr_renderer* r);                          /* ~PBRenderer */

@@ -430,3 +430,33 @@ def test_pfm_dump_and_resume_continue_the_running_mean(tmp_path):
     b.render(cam, 8)
     assert b.frame_index() == 16
     assert np.array_equal(b.get_output_image(), _render(scene, cam, 48, 32, 16).get_output_image())
+
+
+def test_pick_and_update_material():
+    """pickMouse returns the object the unjittered centre ray of a pixel hits (the same ray through
+    trace_closest), and updateMaterial changes what the renderer sees."""
+    desc, ca = scenes.cornell_box()
+    w, h = 64, 64
+    cam = _camera(ca, w, h)
+    scene = cb.Scene.from_desc(desc)
+    picks = 0
+    for (x, y) in [(32, 32), (5, 40), (60, 10), (32, 60), (20, 5)]:
+        got = scene.pick(cam, x, y, w, h)
+        rx, ry = 2.0 * (x / w - 0.5), 2.0 * (y / h - 0.5)
+        d = cam.z_axis + rx * cam.x_axis + ry * cam.y_axis
+        d = d / np.linalg.norm(d)
+        obj, _, t = scene.trace_closest(np.concatenate([cam.position, d])[None].astype(np.float32))
+        if obj[0] < 0:
+            assert got is None
+            continue
+        picks += 1
+        assert got[0] == obj[0]
+        assert np.allclose(got[1]["albedo_d"], desc["mats"][obj[0], 3:6])
+    assert picks >= 4
+    before = _render(scene, cam, w, h, 4).get_output_image()
+    idx, mat = scene.pick(cam, 32, 32, w, h)
+    mat["albedo_e"] = (2.0, 2.0, 2.0)
+    scene.update_material(idx, mat)
+    assert scene.pick(cam, 32, 32, w, h)[1]["albedo_e"].tolist() == [2.0, 2.0, 2.0]
+    after = _render(scene, cam, w, h, 4).get_output_image()
+    assert after.mean() > before.mean()

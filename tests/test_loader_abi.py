@@ -351,3 +351,18 @@ def test_product_loader_matches_golden_scene_fixtures(tmp_path, name):
     assert tris.sum() == pytest.approx(g["tris_sum"], rel=1e-12, abs=1e-9)
     assert np.abs(tris).sum() == pytest.approx(g["tris_abs_sum"], rel=1e-12)
     np.testing.assert_array_equal(s.volume(), np.array(g["volume"], np.float32))
+
+
+def test_pick_and_update_material_validate_arguments():
+    desc, ca = scenes.cornell_box()
+    s = cb.Scene.from_desc(desc)
+    cam = cb.Camera(16, 16)
+    with pytest.raises(cb.CupbrError) as e:
+        s.pick(cam, 1, 1, 0, 16)
+    assert e.value.code == -1
+    m = np.zeros(1, cb.MATERIAL_DTYPE)[0]
+    with pytest.raises(cb.CupbrError) as e:
+        s.update_material(len(desc["types"]), m)
+    assert e.value.code == -1
+    s.update_material(0, m)  # host-side update of an unuploaded scene
+    assert s.objects()["albedo_d"][0].tolist() == [0.0, 0.0, 0.0]
