@@ -158,6 +158,169 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_extend(DevScene S, int cur, con
     if (kCount) wave_add_counts(tc, &c->ext_nodes, &c->ext_prims);
 }
 
+#ifndef CUPBR_PT_REFILL
+#define CUPBR_PT_REFILL 0
+#endif
+// refill: the extension kernel keeps every lane busy: a lane whose ray finished takes a new one
+// as soon as kRefillMin lanes of its wave are idle (one atomic per refill), instead of idling
+// until the wave's longest ray of a 64-ray chunk is done
+constexpr bool kPtRefill = CUPBR_PT_REFILL != 0;
+constexpr int kRefillMin = 16;
+
+#define CUPBR_CSWAP(a, b)                                             \
+    if (key[b] < key[a]) {                                             \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk;               \
+        uint32_t tv = val[a]; val[a] = val[b]; val[b] = tv;            \
+    }
+// closest hit, one node or leaf per step, with per-lane ray refill (same result as traverse<false>)
+template <bool kCount>
+__global__ __launch_bounds__(kPtBlock) void k_pt_extend_refill(DevScene S, int cur_q, const float4* __restrict__ q_o,
+                                                               const float4* __restrict__ q_d,
+                                                               float2* __restrict__ q_hit, Counters* c,
+                                                               uint32_t* spill, uint32_t spill_stride) {
+    __shared__ uint32_t lds[kPtLds * kPtBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = c->queue_count[cur_q].v;
+    if (gtid == 0) {
+        c->queue_count[cur_q ^ 1].v = 0;
+        c->shadow_count[cur_q].v = 0;
+        c->extension_rays += n;
+    }
+    LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
+    TravCount tc;
+    const float inf = __builtin_inff();
+    const uint32_t lane = __lane_id();
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t ray = kNone, cur = 0;
+    int sp = 0;
+    v3 o = splat(0.0f), d = splat(0.0f);
+    float ix = 0.0f, iy = 0.0f, iz = 0.0f;
+    bool sx = true, sy = true, sz = true;
+    TraceResult best{inf, kNoHit, 0xFFFFFFFFu};
+    bool exhausted = false;  // wave-uniform: the queue has no rays left to hand out
+    for (;;) {
+        const unsigned long long idle = __ballot(ray == kNone);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (!exhausted && (n_idle >= (uint32_t)kRefillMin || n_idle == 64u)) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&c->work_extend.v, n_idle);
+            base = __shfl(base, 0);
+            if (base + n_idle >= n) exhausted = true;
+            if (ray == kNone) {
+                const uint32_t my = base + (uint32_t)__popcll(idle & lt_mask);
+                if (my < n) {
+                    ray = my;
+                    const float4 o4 = q_o[my], d4 = q_d[my];
+                    o = mk(o4.x, o4.y, o4.z);
+                    d = mk(d4.x, d4.y, d4.z);
+                    best = TraceResult{inf, kNoHit, 0xFFFFFFFFu};
+                    for (uint32_t k = 0; k < S.n_unb; ++k) {  // planes and quads first, as traverse()
+                        const UnbRec u = S.unb[k];
+                        float t;
+                        if (!plane_quad_test(u, o, d, t)) continue;
+                        const uint32_t rank = __float_as_uint(u.n.w);
+                        if (t < best.t || (t == best.t && rank < best.rank)) {
+                            best.t = t;
+                            best.prim = kUnbBit | k;
+                            best.rank = rank;
+                        }
+                    }
+                    ix = 1.0f / d.x;
+                    iy = 1.0f / d.y;
+                    iz = 1.0f / d.z;
+                    sx = ix >= 0.0f;
+                    sy = iy >= 0.0f;
+                    sz = iz >= 0.0f;
+                    sp = 0;
+                    cur = 0;
+                    if (S.n_nodes == 0) {  // nothing but planes and quads
+                        q_hit[ray] = make_float2(best.t, __uint_as_float(best.prim));
+                        ray = kNone;
+                    }
+                }
+            }
+        }
+        if (__ballot(ray != kNone) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (ray == kNone) continue;
+        bool pop = true;
+        if (cur & kLeafBit) {
+            const uint32_t first = leaf_first(cur), n_leaf = leaf_count(cur);
+            if (kCount) tc.prims += n_leaf;
+            for (uint32_t k = 0; k < n_leaf; ++k) {
+                const PrimRec pr = S.prims[first + k];
+                const uint32_t flags = __float_as_uint(pr.c.w);
+                float t;
+                bool h;
+                if ((flags & 3u) == kPrimTriangle)
+                    h = mt_test(mk(pr.a.x, pr.a.y, pr.a.z), mk(pr.a.w, pr.b.x, pr.b.y), mk(pr.b.z, pr.b.w, pr.c.x), o, d, t);
+                else
+                    h = sphere_test(mk(pr.a.x, pr.a.y, pr.a.z), pr.a.w, o, d, t);
+                if (!h) continue;
+                const uint32_t rank = __float_as_uint(pr.c.y);
+                if (t < best.t || (t == best.t && rank < best.rank)) {
+                    if (flags & kPrimNeedsBoxCheck) {
+                        const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
+                        if (!ref_aabb_hit(ob.bmin, ob.bmax, o, d)) continue;
+                    }
+                    best.t = t;
+                    best.prim = first + k;
+                    best.rank = rank;
+                }
+            }
+        } else {
+            if (kCount) tc.nodes += 1;
+            const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
+            const float4 lx = nf[0], ly = nf[1], lz = nf[2], hx = nf[3], hy = nf[4], hz = nf[5];
+            const uint4 ch = *reinterpret_cast<const uint4*>(nf + 6);
+            const float4 nx = sx ? lx : hx, fx = sx ? hx : lx;
+            const float4 ny = sy ? ly : hy, fy = sy ? hy : ly;
+            const float4 nz = sz ? lz : hz, fz = sz ? hz : lz;
+            const float lim = best.t * 1.0001f + 1e-6f;
+            const float tlo = -1e-5f;
+            float key[4];
+            uint32_t val[4] = {ch.x, ch.y, ch.z, ch.w};
+#define CUPBR_CHILD(i, cc)                                                                              \
+    {                                                                                                   \
+        float tn = fmaxf(fmaxf((nx.cc - o.x) * ix, (ny.cc - o.y) * iy), fmaxf((nz.cc - o.z) * iz, tlo)); \
+        float tf = fminf(fminf((fx.cc - o.x) * ix, (fy.cc - o.y) * iy), fminf((fz.cc - o.z) * iz, lim)); \
+        bool ok = val[i] != kChildEmpty && tn <= tf * 1.000002f;                                       \
+        key[i] = ok ? tn : inf;                                                                         \
+    }
+            CUPBR_CHILD(0, x)
+            CUPBR_CHILD(1, y)
+            CUPBR_CHILD(2, z)
+            CUPBR_CHILD(3, w)
+#undef CUPBR_CHILD
+            CUPBR_CSWAP(0, 1)
+            CUPBR_CSWAP(2, 3)
+            CUPBR_CSWAP(0, 2)
+            CUPBR_CSWAP(1, 3)
+            CUPBR_CSWAP(1, 2)
+            if (key[0] < inf) {
+                if (key[3] < inf) st.put(sp++, val[3]);
+                if (key[2] < inf) st.put(sp++, val[2]);
+                if (key[1] < inf) st.put(sp++, val[1]);
+                cur = val[0];
+                pop = false;
+            }
+        }
+        if (pop) {
+            if (sp == 0) {
+                q_hit[ray] = make_float2(best.t, __uint_as_float(best.prim));
+                ray = kNone;
+            } else {
+                cur = st.get(--sp);
+            }
+        }
+    }
+    if (kCount) wave_add_counts(tc, &c->ext_nodes, &c->ext_prims);
+}
+#undef CUPBR_CSWAP
+
 // geometry of a closest-hit record: world normal and object id (LocalGeometry, Geometry.h:28-36)
 __device__ __forceinline__ void hit_geometry(const DevScene& S, uint32_t prim, v3 P, v3& N, uint32_t& obj) {
     if (prim & kUnbBit) {
@@ -492,6 +655,7 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     float* img = r->images[CUPBR_IMAGE_OUTPUT].as<float>();
     const uint32_t bounces = std::max<uint32_t>(1, r->max_trace_depth);  // do { } while (depth < max)
     auto extend = r->counting ? k_pt_extend<true> : k_pt_extend<false>;
+    auto extend_rf = r->counting ? k_pt_extend_refill<true> : k_pt_extend_refill<false>;
     auto shadow = r->counting ? k_pt_shadow<true> : k_pt_shadow<false>;
     auto trace = r->counting ? k_pt_trace<true> : k_pt_trace<false>;
     const uint32_t spill_entries =
@@ -524,6 +688,10 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
                                    wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
                                    wf.s_d[cur ^ 1].as<float4>(), wf.s_c[cur ^ 1].as<float4>(),
                                    wf.radiance.as<float4>(), c, spill, grid_threads, spill_entries);
+            } else if (kPtRefill) {
+                hipLaunchKernelGGL(extend_rf, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur,
+                                   wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill,
+                                   grid_threads);
             } else {
                 hipLaunchKernelGGL(extend, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
                                    wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill, grid_threads,
