@@ -7,11 +7,12 @@ by bloom (threshold 1, knee 1) and the Reinhard tone map, all on the GPU with
 the scene resident in HBM. Prints one JSON line:
 
   value        Msamples/s = 1920*1080*64*steps / timed seconds (samples = camera paths)
-  roofline     the extension (closest-hit) traversal kernel, the dominant stage:
-               algorithmic bytes per launch = BVH4 nodes visited x 128 B + triangles
-               tested x 48 B + 40 B of queue traffic per ray (visit counts from an
-               instrumented pass), divided by the kernel's average launch time measured
-               with HIP events on the renderer's stream during the timed steps
+  roofline     the trace kernel (closest hit for extension rays + any hit for shadow rays,
+               one launch per bounce), the dominant stage: algorithmic bytes per launch =
+               BVH4 nodes visited x 128 B + triangles tested x 48 B + 40 B (extension) or
+               48 B (shadow) of queue traffic per ray (visit counts from an instrumented
+               pass), divided by the kernel's average launch time measured with HIP events
+               on the renderer's stream during the timed steps
   cpu_baseline the CPU oracle (the reference estimator restated in C++, SAH BVH, FP32)
                timed on this host on a bounded sample of the same workload
 """
@@ -144,7 +145,7 @@ def main():
     img = r.get_output_image()
     log(f"GPU: {value:.1f} Msamples/s, {ms_per_step:.1f} ms/step, {(rays.sum()) / t_total / 1e6:.0f} Mrays/s "
         f"(ext {rays[0] / args.steps / 1e6:.1f}M + shadow {rays[1] / args.steps / 1e6:.1f}M per step); stage ms/step "
-        f"extend {stage_ms[0] / args.steps:.1f} shade {stage_ms[1] / args.steps:.1f} shadow {stage_ms[2] / args.steps:.1f} "
+        f"trace {stage_ms[0] / args.steps:.1f} shade {stage_ms[1] / args.steps:.1f} "
         f"other {stage_ms[3] / args.steps:.1f}; image mean {img.mean():.4f} finite {bool(np.isfinite(img).all())}; "
         f"rgba mean {tm_buf[..., :3].mean():.1f}")
     log(f"extend: {bytes_per_ext_ray:.0f} algorithmic B/ray ({sc['extension_node_visits'] / max(ext_rays, 1):.1f} nodes, "
@@ -153,7 +154,7 @@ def main():
         f"{'trace' if fused else 'extend'} kernel {ext_launch_ms:.3f} ms/launch, {achieved:.0f} GB/s")
 
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "extend_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "trace_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")

@@ -1,17 +1,20 @@
 // PT+NEE in wavefront form (pathtracer_kernel_nee, lib/src/Renderer/PathTracer.cu:97-169).
 //
 // Per batch of paths (one path = one pixel of one reference frame):
-//   raygen      camera rays into queue 0                       (PathTracer.cu:112-117)
-//   per bounce: extend   closest hit for every queued ray      (:126)
-//               shade    emission, NEE sample -> shadow queue,  (:127-158)
-//                        Russian roulette, BSDF sample -> next queue
-//               shadow   any-hit visibility; adds the NEE term  (:64-73)
-//   accumulate  the reference running mean, frame by frame      (:162-168)
-// Queue lengths live on the device, so no host round trip happens between
-// bounces. extend is a persistent grid whose waves take 64-ray chunks from a
-// work counter (traversal cost varies per ray); shade and shadow walk their
-// queues with a static grid stride, and shade appends to the next queues with one
-// atomic per 256-entry block (ballot + LDS prefix).
+//   raygen      camera rays into extension queue 0                    (PathTracer.cu:112-117)
+//   per bounce: trace    closest hit for the bounce's extension rays   (:126), and any-hit
+//                        visibility for the previous bounce's shadow rays, adding their
+//                        NEE terms (:64-73), under one work counter
+//               shade    emission, NEE sample -> shadow queue,         (:127-158)
+//                        Russian roulette, BSDF sample -> next extension queue
+//   final trace the last bounce's shadow rays
+//   accumulate  the reference running mean, frame by frame             (:162-168)
+// Queue lengths live on the device, so no host round trip happens between bounces.
+// trace is a persistent grid whose waves take 64-ray chunks from a work counter
+// (traversal cost varies per ray; one launch per bounce keeps the GPU full where the
+// shadow queue alone is short); shade walks its queue with a static grid stride and
+// appends to the next queues with one atomic per 256-entry block (ballot + LDS prefix).
+// Extension and shadow queues alternate by bounce parity.
 //
 // Differences from the reference that cannot change the image: a path whose
 // throughput became exactly zero (back-facing hit, :36-41) stops instead of
@@ -25,22 +28,9 @@ namespace cupbr {
 
 constexpr int kPtBlock = 256;
 constexpr int kPtWaves = kPtBlock / 64;
-#ifndef CUPBR_PT_LDS
-#define CUPBR_PT_LDS 32
-#endif
-#ifndef CUPBR_PT_KEYS
-#define CUPBR_PT_KEYS 0
-#endif
-#ifndef CUPBR_PT_FUSED
-#define CUPBR_PT_FUSED 0
-#endif
-// fused: one persistent trace kernel per bounce takes the bounce's extension rays and the
-// previous bounce's shadow rays from one work counter (64-ray chunks, extension first)
-constexpr bool kPtFused = CUPBR_PT_FUSED != 0;
-// traversal stack entries per lane kept in LDS; with kPtKeys the extension kernel also keeps each
-// entry's entry distance (a second LDS array of the same size) to skip entries behind the best hit
-constexpr int kPtLds = CUPBR_PT_LDS;
-constexpr bool kPtKeys = CUPBR_PT_KEYS != 0;
+// traversal stack entries per lane kept in LDS (16 KB per 256-thread block: the trace kernel's 61
+// VGPRs, not LDS, then bound its occupancy at 8 waves per SIMD; deeper entries spill to memory)
+constexpr int kPtLds = 16;
 
 __device__ __forceinline__ uint32_t wave_fetch(uint32_t* counter) {
     uint32_t base = 0;
@@ -125,201 +115,6 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
     rad[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(flags));
 }
 
-template <bool kCount>
-__global__ __launch_bounds__(kPtBlock) void k_pt_extend(DevScene S, int cur, const float4* __restrict__ q_o,
-                                                        const float4* __restrict__ q_d, float2* __restrict__ q_hit,
-                                                        Counters* c, uint32_t* spill, uint32_t spill_stride,
-                                                        uint32_t spill_entries) {
-    __shared__ uint32_t lds[(kPtKeys ? 2 : 1) * kPtLds * kPtBlock];
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = c->queue_count[cur].v;
-    if (gtid == 0) {
-        c->queue_count[cur ^ 1].v = 0;
-        c->shadow_count[cur].v = 0;
-        c->extension_rays += n;
-    }
-    LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
-    // keys: second LDS half; their spill region follows the entries' (spill_entries * spill_stride)
-    LaneStack<kPtLds> ks{lds + kPtLds * kPtBlock + threadIdx.x, kPtBlock,
-                         spill ? spill + (size_t)spill_entries * spill_stride + gtid : nullptr, spill_stride};
-    TravCount tc;
-    for (;;) {
-        const uint32_t base = wave_fetch(&c->work_extend.v);
-        if (base >= n) break;
-        const uint32_t i = base + __lane_id();
-        if (i < n) {
-            const float4 o4 = q_o[i], d4 = q_d[i];
-            TraceResult r;
-            traverse<false, kPtLds, kCount, kPtKeys>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r,
-                                                     &tc, &ks);
-            q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
-        }
-    }
-    if (kCount) wave_add_counts(tc, &c->ext_nodes, &c->ext_prims);
-}
-
-#ifndef CUPBR_PT_REFILL
-#define CUPBR_PT_REFILL 0
-#endif
-// refill: the extension kernel keeps every lane busy: a lane whose ray finished takes a new one
-// as soon as kRefillMin lanes of its wave are idle (one atomic per refill), instead of idling
-// until the wave's longest ray of a 64-ray chunk is done
-constexpr bool kPtRefill = CUPBR_PT_REFILL != 0;
-constexpr int kRefillMin = 16;
-
-#define CUPBR_CSWAP(a, b)                                             \
-    if (key[b] < key[a]) {                                             \
-        float tk = key[a]; key[a] = key[b]; key[b] = tk;               \
-        uint32_t tv = val[a]; val[a] = val[b]; val[b] = tv;            \
-    }
-// closest hit, one node or leaf per step, with per-lane ray refill (same result as traverse<false>)
-template <bool kCount>
-__global__ __launch_bounds__(kPtBlock) void k_pt_extend_refill(DevScene S, int cur_q, const float4* __restrict__ q_o,
-                                                               const float4* __restrict__ q_d,
-                                                               float2* __restrict__ q_hit, Counters* c,
-                                                               uint32_t* spill, uint32_t spill_stride) {
-    __shared__ uint32_t lds[kPtLds * kPtBlock];
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = c->queue_count[cur_q].v;
-    if (gtid == 0) {
-        c->queue_count[cur_q ^ 1].v = 0;
-        c->shadow_count[cur_q].v = 0;
-        c->extension_rays += n;
-    }
-    LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
-    TravCount tc;
-    const float inf = __builtin_inff();
-    const uint32_t lane = __lane_id();
-    const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    constexpr uint32_t kNone = 0xFFFFFFFFu;
-    uint32_t ray = kNone, cur = 0;
-    int sp = 0;
-    v3 o = splat(0.0f), d = splat(0.0f);
-    float ix = 0.0f, iy = 0.0f, iz = 0.0f;
-    bool sx = true, sy = true, sz = true;
-    TraceResult best{inf, kNoHit, 0xFFFFFFFFu};
-    bool exhausted = false;  // wave-uniform: the queue has no rays left to hand out
-    for (;;) {
-        const unsigned long long idle = __ballot(ray == kNone);
-        const uint32_t n_idle = (uint32_t)__popcll(idle);
-        if (!exhausted && (n_idle >= (uint32_t)kRefillMin || n_idle == 64u)) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&c->work_extend.v, n_idle);
-            base = __shfl(base, 0);
-            if (base + n_idle >= n) exhausted = true;
-            if (ray == kNone) {
-                const uint32_t my = base + (uint32_t)__popcll(idle & lt_mask);
-                if (my < n) {
-                    ray = my;
-                    const float4 o4 = q_o[my], d4 = q_d[my];
-                    o = mk(o4.x, o4.y, o4.z);
-                    d = mk(d4.x, d4.y, d4.z);
-                    best = TraceResult{inf, kNoHit, 0xFFFFFFFFu};
-                    for (uint32_t k = 0; k < S.n_unb; ++k) {  // planes and quads first, as traverse()
-                        const UnbRec u = S.unb[k];
-                        float t;
-                        if (!plane_quad_test(u, o, d, t)) continue;
-                        const uint32_t rank = __float_as_uint(u.n.w);
-                        if (t < best.t || (t == best.t && rank < best.rank)) {
-                            best.t = t;
-                            best.prim = kUnbBit | k;
-                            best.rank = rank;
-                        }
-                    }
-                    ix = 1.0f / d.x;
-                    iy = 1.0f / d.y;
-                    iz = 1.0f / d.z;
-                    sx = ix >= 0.0f;
-                    sy = iy >= 0.0f;
-                    sz = iz >= 0.0f;
-                    sp = 0;
-                    cur = 0;
-                    if (S.n_nodes == 0) {  // nothing but planes and quads
-                        q_hit[ray] = make_float2(best.t, __uint_as_float(best.prim));
-                        ray = kNone;
-                    }
-                }
-            }
-        }
-        if (__ballot(ray != kNone) == 0ull) {
-            if (exhausted) break;
-            continue;
-        }
-        if (ray == kNone) continue;
-        bool pop = true;
-        if (cur & kLeafBit) {
-            const uint32_t first = leaf_first(cur), n_leaf = leaf_count(cur);
-            if (kCount) tc.prims += n_leaf;
-            for (uint32_t k = 0; k < n_leaf; ++k) {
-                const PrimRec pr = S.prims[first + k];
-                const uint32_t flags = __float_as_uint(pr.c.w);
-                float t;
-                bool h;
-                if ((flags & 3u) == kPrimTriangle)
-                    h = mt_test(mk(pr.a.x, pr.a.y, pr.a.z), mk(pr.a.w, pr.b.x, pr.b.y), mk(pr.b.z, pr.b.w, pr.c.x), o, d, t);
-                else
-                    h = sphere_test(mk(pr.a.x, pr.a.y, pr.a.z), pr.a.w, o, d, t);
-                if (!h) continue;
-                const uint32_t rank = __float_as_uint(pr.c.y);
-                if (t < best.t || (t == best.t && rank < best.rank)) {
-                    if (flags & kPrimNeedsBoxCheck) {
-                        const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
-                        if (!ref_aabb_hit(ob.bmin, ob.bmax, o, d)) continue;
-                    }
-                    best.t = t;
-                    best.prim = first + k;
-                    best.rank = rank;
-                }
-            }
-        } else {
-            if (kCount) tc.nodes += 1;
-            const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
-            const float4 lx = nf[0], ly = nf[1], lz = nf[2], hx = nf[3], hy = nf[4], hz = nf[5];
-            const uint4 ch = *reinterpret_cast<const uint4*>(nf + 6);
-            const float4 nx = sx ? lx : hx, fx = sx ? hx : lx;
-            const float4 ny = sy ? ly : hy, fy = sy ? hy : ly;
-            const float4 nz = sz ? lz : hz, fz = sz ? hz : lz;
-            const float lim = best.t * 1.0001f + 1e-6f;
-            const float tlo = -1e-5f;
-            float key[4];
-            uint32_t val[4] = {ch.x, ch.y, ch.z, ch.w};
-#define CUPBR_CHILD(i, cc)                                                                              \
-    {                                                                                                   \
-        float tn = fmaxf(fmaxf((nx.cc - o.x) * ix, (ny.cc - o.y) * iy), fmaxf((nz.cc - o.z) * iz, tlo)); \
-        float tf = fminf(fminf((fx.cc - o.x) * ix, (fy.cc - o.y) * iy), fminf((fz.cc - o.z) * iz, lim)); \
-        bool ok = val[i] != kChildEmpty && tn <= tf * 1.000002f;                                       \
-        key[i] = ok ? tn : inf;                                                                         \
-    }
-            CUPBR_CHILD(0, x)
-            CUPBR_CHILD(1, y)
-            CUPBR_CHILD(2, z)
-            CUPBR_CHILD(3, w)
-#undef CUPBR_CHILD
-            CUPBR_CSWAP(0, 1)
-            CUPBR_CSWAP(2, 3)
-            CUPBR_CSWAP(0, 2)
-            CUPBR_CSWAP(1, 3)
-            CUPBR_CSWAP(1, 2)
-            if (key[0] < inf) {
-                if (key[3] < inf) st.put(sp++, val[3]);
-                if (key[2] < inf) st.put(sp++, val[2]);
-                if (key[1] < inf) st.put(sp++, val[1]);
-                cur = val[0];
-                pop = false;
-            }
-        }
-        if (pop) {
-            if (sp == 0) {
-                q_hit[ray] = make_float2(best.t, __uint_as_float(best.prim));
-                ray = kNone;
-            } else {
-                cur = st.get(--sp);
-            }
-        }
-    }
-    if (kCount) wave_add_counts(tc, &c->ext_nodes, &c->ext_prims);
-}
-#undef CUPBR_CSWAP
 
 // geometry of a closest-hit record: world normal and object id (LocalGeometry, Geometry.h:28-36)
 __device__ __forceinline__ void hit_geometry(const DevScene& S, uint32_t prim, v3 P, v3& N, uint32_t& obj) {
@@ -516,9 +311,8 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_trace(DevScene S, int cur, cons
                                                        const float4* __restrict__ q_d, float2* __restrict__ q_hit,
                                                        const float4* __restrict__ s_o, const float4* __restrict__ s_d,
                                                        const float4* __restrict__ s_c, float4* __restrict__ rad,
-                                                       Counters* c, uint32_t* spill, uint32_t spill_stride,
-                                                       uint32_t spill_entries) {
-    __shared__ uint32_t lds[(kPtKeys ? 2 : 1) * kPtLds * kPtBlock];
+                                                       Counters* c, uint32_t* spill, uint32_t spill_stride) {
+    __shared__ uint32_t lds[kPtLds * kPtBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_ext = c->queue_count[cur].v;
     const uint32_t n_sh = c->shadow_count[cur ^ 1].v;
@@ -529,8 +323,6 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_trace(DevScene S, int cur, cons
         c->shadow_rays += n_sh;
     }
     LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
-    LaneStack<kPtLds> ks{lds + kPtLds * kPtBlock + threadIdx.x, kPtBlock,
-                         spill ? spill + (size_t)spill_entries * spill_stride + gtid : nullptr, spill_stride};
     TravCount te, ts;
     const uint32_t ext_span = (n_ext + 63u) & ~63u;
     const uint32_t total = ext_span + n_sh;
@@ -542,8 +334,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_trace(DevScene S, int cur, cons
             if (i < n_ext) {
                 const float4 o4 = q_o[i], d4 = q_d[i];
                 TraceResult r;
-                traverse<false, kPtLds, kCount, kPtKeys>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st,
-                                                         r, &te, &ks);
+                traverse<false, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r, &te);
                 q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
             }
         } else {
@@ -563,27 +354,6 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_trace(DevScene S, int cur, cons
     }
 }
 
-template <bool kCount>
-__global__ __launch_bounds__(kPtBlock) void k_pt_shadow(DevScene S, int par, const float4* __restrict__ s_o,
-                                                        const float4* __restrict__ s_d, const float4* __restrict__ s_c,
-                                                        float4* __restrict__ rad, Counters* c, uint32_t* spill,
-                                                        uint32_t spill_stride) {
-    __shared__ uint32_t lds[kPtLds * kPtBlock];
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = c->shadow_count[par].v;
-    if (gtid == 0) c->shadow_rays += n;
-    LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
-    TravCount tc;
-    const uint32_t stride = gridDim.x * kPtBlock;
-    for (uint32_t i = gtid; i < n; i += stride) {
-        const float4 o4 = s_o[i], d4 = s_d[i];
-        TraceResult r;
-        const bool occluded =
-            traverse<true, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r, &tc);
-        shadow_result(occluded, d4, s_c, i, rad);
-    }
-    if (kCount) wave_add_counts(tc, &c->shadow_nodes, &c->shadow_prims);
-}
 
 // running mean of PathTracer.cu:162-168, applied frame by frame in order
 __global__ __launch_bounds__(kPtBlock) void k_accumulate(float* __restrict__ img, const float4* __restrict__ rad,
@@ -617,7 +387,7 @@ static int ensure_wavefront(cupbr_renderer* r, uint32_t paths, uint32_t grid_thr
             if ((rc = wf.q_d[k].ensure(n * 16))) return rc;
         }
         if ((rc = wf.q_hit.ensure(n * 8))) return rc;
-        for (int k = 0; k < (kPtFused ? 2 : 1); ++k) {
+        for (int k = 0; k < 2; ++k) {
             if ((rc = wf.s_o[k].ensure(n * 16))) return rc;
             if ((rc = wf.s_d[k].ensure(n * 16))) return rc;
             if ((rc = wf.s_c[k].ensure(n * 16))) return rc;
@@ -627,7 +397,7 @@ static int ensure_wavefront(cupbr_renderer* r, uint32_t paths, uint32_t grid_thr
     if ((rc = wf.counters.ensure(sizeof(Counters)))) return rc;
     const uint32_t entries = r->scene->stack_entries;
     if (entries > (uint32_t)kPtLds) {
-        if ((rc = wf.spill.ensure((size_t)(kPtKeys ? 2 : 1) * (entries - kPtLds) * grid_threads * 4))) return rc;
+        if ((rc = wf.spill.ensure((size_t)(entries - kPtLds) * grid_threads * 4))) return rc;
     }
     return CUPBR_OK;
 }
@@ -654,12 +424,7 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     const DevScene& S = r->scene->dev;
     float* img = r->images[CUPBR_IMAGE_OUTPUT].as<float>();
     const uint32_t bounces = std::max<uint32_t>(1, r->max_trace_depth);  // do { } while (depth < max)
-    auto extend = r->counting ? k_pt_extend<true> : k_pt_extend<false>;
-    auto extend_rf = r->counting ? k_pt_extend_refill<true> : k_pt_extend_refill<false>;
-    auto shadow = r->counting ? k_pt_shadow<true> : k_pt_shadow<false>;
     auto trace = r->counting ? k_pt_trace<true> : k_pt_trace<false>;
-    const uint32_t spill_entries =
-        r->scene->stack_entries > (uint32_t)kPtLds ? r->scene->stack_entries - kPtLds : 0u;
     auto shade = nrc ? k_pt_shade<true> : k_pt_shade<false>;
     NrcShade ns{nullptr, nullptr, 0};
     uint32_t train_thresh = 0;
@@ -680,44 +445,22 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
                            h, frame0 + s0, npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
                            wf.weight.as<float4>(), wf.radiance.as<float4>(), c, train_thresh);
         if ((rc = stage_mark(r, 3))) return rc;
-        for (uint32_t b = 0; b < bounces; ++b) {
+        // per bounce: trace (this bounce's extension rays + the previous bounce's shadow rays), then
+        // shade (appends the next extension rays and this bounce's shadow rays, by bounce parity)
+        for (uint32_t b = 0; b <= bounces; ++b) {
             const int cur = (int)(b & 1);
-            const int sp = kPtFused ? cur : 0;  // shadow queue shade(b) appends to
-            if (kPtFused) {
-                hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
-                                   wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
-                                   wf.s_d[cur ^ 1].as<float4>(), wf.s_c[cur ^ 1].as<float4>(),
-                                   wf.radiance.as<float4>(), c, spill, grid_threads, spill_entries);
-            } else if (kPtRefill) {
-                hipLaunchKernelGGL(extend_rf, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur,
-                                   wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill,
-                                   grid_threads);
-            } else {
-                hipLaunchKernelGGL(extend, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
-                                   wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), c, spill, grid_threads,
-                                   spill_entries);
-            }
-            if ((rc = stage_mark(r, 0))) return rc;
-            hipLaunchKernelGGL(shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
-                               (int)r->russian_roulette, wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),
-                               wf.q_hit.as<float2>(), wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(),
-                               wf.s_o[sp].as<float4>(), wf.s_d[sp].as<float4>(), wf.s_c[sp].as<float4>(),
-                               wf.weight.as<float4>(), wf.radiance.as<float4>(), c, ns);
-            if ((rc = stage_mark(r, 1))) return rc;
-            if (!kPtFused) {
-                hipLaunchKernelGGL(shadow, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.s_o[0].as<float4>(),
-                                   wf.s_d[0].as<float4>(), wf.s_c[0].as<float4>(), wf.radiance.as<float4>(), c, spill,
-                                   grid_threads);
-                if ((rc = stage_mark(r, 2))) return rc;
-            }
-        }
-        if (kPtFused) {  // the last bounce's shadow rays (its extension queue is empty)
-            const int cur = (int)(bounces & 1);
             hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
                                wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
                                wf.s_d[cur ^ 1].as<float4>(), wf.s_c[cur ^ 1].as<float4>(), wf.radiance.as<float4>(),
-                               c, spill, grid_threads, spill_entries);
+                               c, spill, grid_threads);
             if ((rc = stage_mark(r, 0))) return rc;
+            if (b == bounces) break;  // the last trace only had the last bounce's shadow rays
+            hipLaunchKernelGGL(shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
+                               (int)r->russian_roulette, wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),
+                               wf.q_hit.as<float2>(), wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(),
+                               wf.s_o[cur].as<float4>(), wf.s_d[cur].as<float4>(), wf.s_c[cur].as<float4>(),
+                               wf.weight.as<float4>(), wf.radiance.as<float4>(), c, ns);
+            if ((rc = stage_mark(r, 1))) return rc;
         }
         if (nrc) {
             // throughput * cache for the paths that ended at a cache vertex, then the training rows

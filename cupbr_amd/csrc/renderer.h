@@ -23,7 +23,7 @@ struct alignas(128) CounterLine {
 struct Counters {
     CounterLine queue_count[2];  // extension queues (ping-pong by bounce parity)
     CounterLine shadow_count[2]; // shadow queues (bounce parity)
-    CounterLine work_extend;
+    CounterLine work_extend;     // work counter of the trace kernel
     CounterLine query_count;  // radiance-cache queries of the batch
     CounterLine row_count;    // radiance-cache training rows of the batch
     unsigned long long extension_rays;

@@ -129,13 +129,9 @@ struct TravCount {
 
 // kAny=false: closest hit (best.t = +inf, best.prim = kNoHit on a miss).
 // kAny=true : any valid hit with t <= dist; returns whether one exists.
-// kKeys (closest hit only): a second stack `ks` holds each pushed entry's entry distance, and a
-// popped entry is skipped when that distance already fails the child test against the current
-// best hit (the same conservative test as at push time, so the result is unchanged).
-template <bool kAny, int kLds, bool kCount = false, bool kKeys = false>
+template <bool kAny, int kLds, bool kCount = false>
 __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float dist, LaneStack<kLds>& st,
-                                         TraceResult& best, TravCount* cnt = nullptr,
-                                         LaneStack<kLds>* ks = nullptr) {
+                                         TraceResult& best, TravCount* cnt = nullptr) {
     const float inf = __builtin_inff();
     best.t = inf;
     best.prim = kNoHit;
@@ -226,23 +222,12 @@ __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float di
             CUPBR_CSWAP(1, 3)
             CUPBR_CSWAP(1, 2)
             if (key[0] < inf) {
-#define CUPBR_PUSH(i)                                                   \
-    if (key[i] < inf) {                                                 \
-        st.put(sp, val[i]);                                             \
-        if (kKeys && !kAny) ks->put(sp, __float_as_uint(key[i]));       \
-        ++sp;                                                           \
-    }
-                CUPBR_PUSH(3)
-                CUPBR_PUSH(2)
-                CUPBR_PUSH(1)
-#undef CUPBR_PUSH
+                if (key[3] < inf) st.put(sp++, val[3]);
+                if (key[2] < inf) st.put(sp++, val[2]);
+                if (key[1] < inf) st.put(sp++, val[1]);
                 cur = val[0];
                 continue;
             }
-        }
-        if (kKeys && !kAny) {
-            const float lim = (best.t * 1.0001f + 1e-6f) * 1.000002f;
-            while (sp > 0 && __uint_as_float(ks->get(sp - 1)) > lim) --sp;
         }
         if (sp == 0) break;
         cur = st.get(--sp);

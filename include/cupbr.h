@@ -157,8 +157,8 @@ typedef struct cupbr_render_stats {
     uint64_t shadow_rays;      /* NEE visibility rays */
     double milliseconds;       /* device time of the last cupbr_render call (HIP events) */
     /* with cupbr_renderer_set_profiling(r, 1): per-stage device time (HIP events around every
-     * launch on the renderer's stream) and launch counts; stage 0 extend, 1 shade, 2 shadow,
-     * 3 ray generation + accumulation */
+     * launch on the renderer's stream) and launch counts; stage 0 trace (extension and shadow
+     * rays), 1 shade, 2 unused, 3 ray generation, accumulation and radiance-cache work */
     double stage_ms[4];
     uint32_t stage_launches[4];
     /* with cupbr_renderer_set_traversal_counting(r, 1): BVH4 nodes visited and primitives
