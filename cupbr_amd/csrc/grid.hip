@@ -46,6 +46,7 @@ namespace cupbr {
 constexpr int kGridBlock = 256;
 constexpr int kGridLds = 32;
 constexpr float kSurfaceOffset = 1e-4f;
+constexpr int kAlbLds = 3 * 1024;  // surfaces whose albedo gradients a block accumulates in LDS
 constexpr size_t kMaxGridSamples = (size_t)1 << 24;  // samples per forward chunk (192 MB buffer)
 // Bound on tentative collisions per segment. The host requires majorant * box diagonal <= 1e5,
 // so the expected count per segment is below 1e5 and the bound is never reached in practice;
@@ -338,35 +339,52 @@ __device__ v3 grid_path(const DevScene& S, const GridDev& G, GridTracer& T, cons
     return L;
 }
 
+// work item i -> (pixel, sample): sample-major, so the 64 lanes of a wave trace 64 neighbouring
+// pixels (not 64 samples of one pixel, whose identical first segments would send the same
+// gradient atomics to the same voxels from every lane)
 template <bool kAdj>
 __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupbr_camera cam, uint32_t w, uint32_t h,
                                                      uint32_t spp, uint32_t frame0, uint32_t s_begin, uint32_t s_count,
                                                      uint32_t max_depth, float* __restrict__ samples,
                                                      const float* __restrict__ d_image, float* grad_rho,
-                                                     float* grad_alb, uint32_t* spill,
+                                                     float* grad_alb, uint32_t n_alb, uint32_t* spill,
                                                      unsigned long long* counts) {
     __shared__ uint32_t lds[kGridLds * kGridBlock];
+    // per-block albedo gradients (few surfaces, hit by every path): LDS atomics, one global
+    // atomic per entry at the end
+    __shared__ float alb_lds[kAlbLds];
+    const bool alb_local = kAdj && n_alb <= (uint32_t)kAlbLds;
+    if (alb_local) {
+        for (uint32_t k = threadIdx.x; k < n_alb; k += blockDim.x) alb_lds[k] = 0.0f;
+        __syncthreads();
+    }
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     LaneStack<kGridLds> st{lds + threadIdx.x, kGridBlock, spill ? spill + gtid : nullptr, (uint32_t)stride};
     GridTracer T{S, st};
-    const size_t n = (size_t)w * h * s_count;
+    const uint32_t npix = w * h;
+    const size_t n = (size_t)npix * s_count;
     for (size_t i = gtid; i < n; i += stride) {
-        const uint32_t p = (uint32_t)(i / s_count);
-        const uint32_t s = s_begin + (uint32_t)(i % s_count);
+        const uint32_t p = (uint32_t)(i % npix);
+        const uint32_t s = s_begin + (uint32_t)(i / npix);
         const uint32_t px = p % w, py = p / w;
         const v3 L = grid_path<false>(S, G, T, cam, px, py, w, h, frame0 + s, max_depth, nullptr);
         if (!kAdj) {
-            st3(samples + 3 * i, L);
+            st3(samples + 3 * i, L);  // sample-major: samples[(s - s_begin) * npix + p]
         } else {
             Adj A;
             A.delta = ld3(d_image + 3 * (size_t)p) / (float)spp;
             A.rem = L;
             A.grad_rho = grad_rho;
-            A.grad_alb = grad_alb;
+            A.grad_alb = alb_local ? alb_lds : grad_alb;
             if (A.delta.x != 0.0f || A.delta.y != 0.0f || A.delta.z != 0.0f)
                 grid_path<true>(S, G, T, cam, px, py, w, h, frame0 + s, max_depth, &A);
         }
+    }
+    if (alb_local) {
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < n_alb; k += blockDim.x)
+            if (alb_lds[k] != 0.0f) atomicAdd(grad_alb + k, alb_lds[k]);
     }
     if (counts) {
         unsigned long long c[4] = {T.n_ext, T.n_shadow, T.n_reads, T.n_atomics};
@@ -377,13 +395,13 @@ __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupb
     }
 }
 
-// image = mean over samples, summed in sample order across chunks
+// image = mean over samples (stored sample-major), summed in sample order across chunks
 __global__ void k_grid_accumulate(float* __restrict__ acc, const float* __restrict__ samples, uint32_t n_px,
                                   uint32_t s_count, int first, int last, uint32_t spp) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_px) return;
     v3 v = first ? splat(0.0f) : ld3(acc + 3 * (size_t)p);
-    for (uint32_t s = 0; s < s_count; ++s) v += ld3(samples + 3 * ((size_t)p * s_count + s));
+    for (uint32_t s = 0; s < s_count; ++s) v += ld3(samples + 3 * ((size_t)s * n_px + p));
     if (last) v = v / (float)spp;
     st3(acc + 3 * (size_t)p, v);
 }
@@ -483,7 +501,7 @@ int grid_render_device(const cupbr_scene* cs, const cupbr_camera* cam, const cup
         uint32_t* spill;
         if (grid_launch_shape(s, (size_t)n_px * cnt, &blocks, &spill)) return CUPBR_ERR_OUT_OF_MEMORY;
         hipLaunchKernelGGL(k_grid<false>, dim3(blocks), dim3(kGridBlock), 0, stream, s->dev, G, *cam, w, h, spp, frame0,
-                           s0, cnt, max_depth, s->grid_samples.as<float>(), nullptr, nullptr, nullptr, spill, counts);
+                           s0, cnt, max_depth, s->grid_samples.as<float>(), nullptr, nullptr, nullptr, 0u, spill, counts);
         CUPBR_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_grid_accumulate, dim3((n_px + 255) / 256), dim3(256), 0, stream, image,
                            s->grid_samples.as<float>(), n_px, cnt, s0 == 0 ? 1 : 0, s0 + cnt == spp ? 1 : 0, spp);
@@ -504,7 +522,8 @@ int grid_adjoint_device(const cupbr_scene* cs, const cupbr_camera* cam, const cu
     uint32_t* spill;
     if (grid_launch_shape(s, (size_t)w * h * spp, &blocks, &spill)) return CUPBR_ERR_OUT_OF_MEMORY;
     hipLaunchKernelGGL(k_grid<true>, dim3(blocks), dim3(kGridBlock), 0, stream, s->dev, G, *cam, w, h, spp, frame0, 0u,
-                       spp, max_depth, nullptr, d_image, grad_rho, grad_alb, spill, counts);
+                       spp, max_depth, nullptr, d_image, grad_rho, grad_alb,
+                       (uint32_t)(3 * (s->objects.size() + s->triangles.size() / 9)), spill, counts);
     CUPBR_HIP_CHECK(hipGetLastError());
     return CUPBR_OK;
 }
