@@ -27,9 +27,15 @@ struct Bvh4 {
     double sah_cost = 0.0;
 };
 
+#ifndef CUPBR_BVH_MAX_LEAF
+#define CUPBR_BVH_MAX_LEAF 8
+#endif
+#ifndef CUPBR_BVH_TRAVERSAL_COST
+#define CUPBR_BVH_TRAVERSAL_COST 1.0f
+#endif
 struct BvhBuildOptions {
-    uint32_t max_leaf = 8;      // <= kMaxLeafPrims
-    float traversal_cost = 1.0f;
+    uint32_t max_leaf = CUPBR_BVH_MAX_LEAF;  // <= kMaxLeafPrims
+    float traversal_cost = CUPBR_BVH_TRAVERSAL_COST;
     float intersect_cost = 1.0f;
     int threads = 0;            // 0: hardware concurrency
 };
