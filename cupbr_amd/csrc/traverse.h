@@ -160,50 +160,54 @@ __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float di
     const float tlo = -1e-5f;
     int sp = 0;
     uint32_t cur = 0;
-    for (;;) {
-        if (cur & kLeafBit) {
-            const uint32_t first = leaf_first(cur), n_leaf = leaf_count(cur);
-            if (kCount) cnt->prims += n_leaf;
-            for (uint32_t k = 0; k < n_leaf; ++k) {
-                const PrimRec pr = S.prims[first + k];
-                const uint32_t flags = __float_as_uint(pr.c.w);
-                float t;
-                bool h;
-                if ((flags & 3u) == kPrimTriangle)
-                    h = mt_test(mk(pr.a.x, pr.a.y, pr.a.z), mk(pr.a.w, pr.b.x, pr.b.y), mk(pr.b.z, pr.b.w, pr.c.x), o, d, t);
-                else
-                    h = sphere_test(mk(pr.a.x, pr.a.y, pr.a.z), pr.a.w, o, d, t);
-                if (!h) continue;
-                if (kAny) {
-                    if (!(t <= dist)) continue;
-                    if (flags & kPrimNeedsBoxCheck) {
-                        const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
-                        if (!ref_aabb_hit(ob.bmin, ob.bmax, o, d)) continue;
-                    }
-                    return true;
+
+    // the primitives of one leaf; true when an any-hit query is answered
+    auto leaf = [&](uint32_t word) -> bool {
+        const uint32_t first = leaf_first(word), n_leaf = leaf_count(word);
+        if (kCount) cnt->prims += n_leaf;
+        for (uint32_t k = 0; k < n_leaf; ++k) {
+            const PrimRec pr = S.prims[first + k];
+            const uint32_t flags = __float_as_uint(pr.c.w);
+            float t;
+            bool h;
+            if ((flags & 3u) == kPrimTriangle)
+                h = mt_test(mk(pr.a.x, pr.a.y, pr.a.z), mk(pr.a.w, pr.b.x, pr.b.y), mk(pr.b.z, pr.b.w, pr.c.x), o, d, t);
+            else
+                h = sphere_test(mk(pr.a.x, pr.a.y, pr.a.z), pr.a.w, o, d, t);
+            if (!h) continue;
+            if (kAny) {
+                if (!(t <= dist)) continue;
+                if (flags & kPrimNeedsBoxCheck) {
+                    const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
+                    if (!ref_aabb_hit(ob.bmin, ob.bmax, o, d)) continue;
                 }
-                const uint32_t rank = __float_as_uint(pr.c.y);
-                if (t < best.t || (t == best.t && rank < best.rank)) {
-                    if (flags & kPrimNeedsBoxCheck) {
-                        const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
-                        if (!ref_aabb_hit(ob.bmin, ob.bmax, o, d)) continue;
-                    }
-                    best.t = t;
-                    best.prim = first + k;
-                    best.rank = rank;
-                }
+                return true;
             }
-        } else {
-            if (kCount) cnt->nodes += 1;
-            const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
-            const float4 lx = nf[0], ly = nf[1], lz = nf[2], hx = nf[3], hy = nf[4], hz = nf[5];
-            const uint4 ch = *reinterpret_cast<const uint4*>(nf + 6);
-            const float4 nx = sx ? lx : hx, fx = sx ? hx : lx;
-            const float4 ny = sy ? ly : hy, fy = sy ? hy : ly;
-            const float4 nz = sz ? lz : hz, fz = sz ? hz : lz;
-            const float lim = kAny ? dist * 1.0001f + 1e-6f : best.t * 1.0001f + 1e-6f;
-            float key[4];
-            uint32_t val[4] = {ch.x, ch.y, ch.z, ch.w};
+            const uint32_t rank = __float_as_uint(pr.c.y);
+            if (t < best.t || (t == best.t && rank < best.rank)) {
+                if (flags & kPrimNeedsBoxCheck) {
+                    const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
+                    if (!ref_aabb_hit(ob.bmin, ob.bmax, o, d)) continue;
+                }
+                best.t = t;
+                best.prim = first + k;
+                best.rank = rank;
+            }
+        }
+        return false;
+    };
+    // one inner node: push the far children, return whether a child was entered (cur = nearest)
+    auto inner = [&]() -> bool {
+        if (kCount) cnt->nodes += 1;
+        const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
+        const float4 lx = nf[0], ly = nf[1], lz = nf[2], hx = nf[3], hy = nf[4], hz = nf[5];
+        const uint4 ch = *reinterpret_cast<const uint4*>(nf + 6);
+        const float4 nx = sx ? lx : hx, fx = sx ? hx : lx;
+        const float4 ny = sy ? ly : hy, fy = sy ? hy : ly;
+        const float4 nz = sz ? lz : hz, fz = sz ? hz : lz;
+        const float lim = kAny ? dist * 1.0001f + 1e-6f : best.t * 1.0001f + 1e-6f;
+        float key[4];
+        uint32_t val[4] = {ch.x, ch.y, ch.z, ch.w};
 #define CUPBR_CHILD(i, c)                                                                          \
     {                                                                                              \
         float tn = fmaxf(fmaxf((nx.c - o.x) * ix, (ny.c - o.y) * iy), fmaxf((nz.c - o.z) * iz, tlo)); \
@@ -211,23 +215,29 @@ __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float di
         bool ok = val[i] != kChildEmpty && tn <= tf * 1.000002f;                                  \
         key[i] = ok ? tn : inf;                                                                    \
     }
-            CUPBR_CHILD(0, x)
-            CUPBR_CHILD(1, y)
-            CUPBR_CHILD(2, z)
-            CUPBR_CHILD(3, w)
+        CUPBR_CHILD(0, x)
+        CUPBR_CHILD(1, y)
+        CUPBR_CHILD(2, z)
+        CUPBR_CHILD(3, w)
 #undef CUPBR_CHILD
-            CUPBR_CSWAP(0, 1)
-            CUPBR_CSWAP(2, 3)
-            CUPBR_CSWAP(0, 2)
-            CUPBR_CSWAP(1, 3)
-            CUPBR_CSWAP(1, 2)
-            if (key[0] < inf) {
-                if (key[3] < inf) st.put(sp++, val[3]);
-                if (key[2] < inf) st.put(sp++, val[2]);
-                if (key[1] < inf) st.put(sp++, val[1]);
-                cur = val[0];
-                continue;
-            }
+        CUPBR_CSWAP(0, 1)
+        CUPBR_CSWAP(2, 3)
+        CUPBR_CSWAP(0, 2)
+        CUPBR_CSWAP(1, 3)
+        CUPBR_CSWAP(1, 2)
+        if (!(key[0] < inf)) return false;
+        if (key[3] < inf) st.put(sp++, val[3]);
+        if (key[2] < inf) st.put(sp++, val[2]);
+        if (key[1] < inf) st.put(sp++, val[1]);
+        cur = val[0];
+        return true;
+    };
+
+    for (;;) {
+        if (cur & kLeafBit) {
+            if (leaf(cur)) return true;
+        } else if (inner()) {
+            continue;
         }
         if (sp == 0) break;
         cur = st.get(--sp);
