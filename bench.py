@@ -41,10 +41,12 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["pt", "grid", "nrc"], default="pt",
+    ap.add_argument("--workload", choices=["pt", "grid", "nrc", "gdpt", "volume"], default="pt",
                     help="pt: the headline PT+NEE frame; grid: one gradient evaluation (forward + path-replay "
                          "adjoint) of the 64^3 heterogeneous medium at 256x256, 64 spp; nrc: equal-time rel-MSE of "
-                         "1-spp frames with the neural radiance cache against plain PT on the 1080p/1M-triangle scene")
+                         "1-spp frames with the neural radiance cache against plain PT on the 1080p/1M-triangle scene; "
+                         "gdpt: gradient-domain frame (5 paths per sample + 50 reconstruction iterations) of the "
+                         "Cornell box at 512x512, 128 spp; volume: the homogeneous-media renderer at 512x512, 64 spp")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run three device-to-device copies of MIB MiB (a plain copy kernel of known "
                          "size, for scaling the HBM byte counters of a --pmc run)")
@@ -77,6 +79,8 @@ def main():
         return bench_grid(args, log)
     if args.workload == "nrc":
         return bench_nrc(args, log)
+    if args.workload in ("gdpt", "volume"):
+        return bench_integrator(args, log)
     W, H, SPP = args.width, args.height, args.spp
     t0 = time.time()
     desc, cam_args = scenes.million_triangles(W, H)
@@ -432,6 +436,106 @@ def bench_nrc(args, log):
                      "frac": round(tflops / 2500.0, 5), "traffic": None, "kernel": "k_nrc_infer<false>",
                      "avg_launch_ms": round(inf_ms, 4), "queries_per_launch": int(x.shape[0])},
         "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_integrator(args, log):
+    """The one-thread-per-pixel integrators: GDPT (C2: Cornell box 512x512, 128 spp, depth 5; every
+    sample traces the base path and 4 neighbour paths, then 50 reconstruction iterations) or the
+    volume renderer (homogeneous media: two scattering spheres in a glass-walled box and a scene
+    medium, 512x512, 64 spp, depth 8). One step = reset + one render call. Roofline: the
+    algorithmic BVH bytes of all traversals of a step (counted in an instrumented pass) over the
+    render call's device time (HIP events on the renderer's stream)."""
+    import numpy as np
+
+    import cupbr_amd as cb
+    import oracle
+    from cupbr_amd import scenes
+
+    W = H = 512
+    if args.workload == "gdpt":
+        desc, ca = scenes.cornell_box(W, H)
+        cam = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], W, H)
+        method, spp, depth, paths = cb.RenderingMethod.GRADIENTDOMAIN, 128, 5, 5
+    else:
+        desc = scenes.volume_spheres()
+        cam = cb.Camera(W, H)
+        method, spp, depth, paths = cb.RenderingMethod.VOLUME, 64, 8, 1
+    scene = cb.Scene.from_desc(desc)
+    scene.upload()
+    r = cb.PBRenderer(method)
+    r.set_output_size(W, H)
+    r.register_scene(scene)
+    r.set_method(method)
+    r.set_max_trace_depth(depth)
+
+    def step():
+        r.reset()
+        r.render(cam, spp)
+        r.synchronize()
+
+    r.set_traversal_counting(True)
+    step()
+    sc = r.stats()
+    r.set_traversal_counting(False)
+    bytes_step = (sc["extension_node_visits"] + sc["shadow_node_visits"]) * 128 + \
+                 (sc["extension_prim_tests"] + sc["shadow_prim_tests"]) * 48
+    for _ in range(args.warmup):
+        step()
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        dev_ms += r.stats()["milliseconds"]
+    t_total = time.perf_counter() - t0
+    samples = W * H * spp * args.steps
+    value = samples / t_total / 1e6
+    achieved = bytes_step * args.steps / (dev_ms * 1e-3) / 1e9
+    rays = sc["extension_rays"] + sc["shadow_rays"]
+    log(f"{args.workload}: {value:.1f} Msamples/s ({paths} paths per sample), {t_total / args.steps * 1e3:.1f} ms/step, "
+        f"device {dev_ms / args.steps:.1f} ms/step, {rays * args.steps / t_total / 1e6:.0f} Mrays/s, "
+        f"{bytes_step / max(rays, 1):.0f} algorithmic B/ray")
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        ref = oracle.OracleScene(desc, precisions=(32,))
+        cs = 32 if args.workload == "gdpt" else 64  # ~10 s of CPU work on 16 threads
+        t0 = time.perf_counter()
+        if args.workload == "gdpt":
+            bufs = ref.gdpt(cam.as_dict(), W, H, 0, cs, depth=depth, precision=32, threads=threads)
+            oracle.gdpt_reconstruct(bufs.astype(np.float32), 50)
+        else:
+            ref.render(cam.as_dict(), W, H, 0, cs, depth=depth, method=oracle.VOLUME, precision=32, threads=threads)
+        t_cpu = time.perf_counter() - t0
+        cpu_value = W * H * cs / t_cpu / 1e6
+        cpu = {"value": round(cpu_value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+               "sample": f"{cs} spp of the same {W}x{H} frame on the C++ oracle (FP32), {threads} threads, {t_cpu:.1f}s"}
+        log(f"CPU oracle: {cpu_value:.3f} Msamples/s on {threads} threads; GPU/CPU = {value / cpu_value:.0f}x")
+    out = {
+        "metric": f"Msamples/s ({args.workload})",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": (f"gradient-domain PT, Cornell box {W}x{H}, {spp} spp, depth {depth}, 5 paths per sample "
+                                f"+ 50 reconstruction iterations") if args.workload == "gdpt" else
+                               f"volume renderer (homogeneous media), {W}x{H}, {spp} spp, depth {depth}",
+                   "width": W, "height": H, "spp": spp, "max_depth": depth,
+                   "mrays_per_s": round(rays * args.steps / t_total / 1e6, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_gdpt (all launches of a render call)" if args.workload == "gdpt" else
+                               "k_volume (all launches of a render call)",
+                     "bytes_per_step": int(bytes_step), "device_ms_per_step": round(dev_ms / args.steps, 3)},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

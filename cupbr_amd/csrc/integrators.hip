@@ -13,21 +13,24 @@ namespace cupbr {
 constexpr int kIntBlock = 256;
 constexpr int kIntLds = 32;
 
-struct Tracer {
+// kCount: BVH4 node visits and primitive tests of closest-hit (te) and visibility (ts) queries
+template <bool kCount = false>
+struct TracerT {
     const DevScene& S;
     LaneStack<kIntLds>& st;
     uint32_t ext = 0, shadow = 0;  // ray counts of this lane
+    TravCount te, ts;
 
     __device__ __forceinline__ bool closest(v3 o, v3 d, TraceResult& r) {
         ++ext;
-        traverse<false>(S, o, d, 0.0f, st, r);
+        traverse<false, kIntLds, kCount>(S, o, d, 0.0f, st, r, &te);
         return r.prim != kNoHit;
     }
     // Tracing::traceVisibility, TracingDetail.h:118-183
     __device__ __forceinline__ bool visible(v3 o, v3 d, float dist) {
         ++shadow;
         TraceResult r;
-        return !traverse<true>(S, o, d, dist, st, r);
+        return !traverse<true, kIntLds, kCount>(S, o, d, dist, st, r, &ts);
     }
     // Tracing::traceRay(scene, ray, index), TracingDetail.h:68-116: one object, no box test;
     // a mesh keeps its first strictly-closer triangle (MeshDetail.h:28-35)
@@ -60,6 +63,7 @@ struct Tracer {
         return r.prim != kNoHit;
     }
 };
+using Tracer = TracerT<false>;
 
 // normal and object of a hit (LocalGeometry, Geometry.h:28-36)
 __device__ __forceinline__ void geometry_of(const DevScene& S, uint32_t prim, v3 P, v3& N, uint32_t& obj) {
@@ -79,6 +83,20 @@ __device__ __forceinline__ void geometry_of(const DevScene& S, uint32_t prim, v3
     }
 }
 
+template <bool kCount>
+__device__ __forceinline__ void add_visit_counts(Counters* c, const TracerT<kCount>& T) {
+    if (!kCount) return;
+    unsigned long long v[4] = {T.te.nodes, T.te.prims, T.ts.nodes, T.ts.prims};
+    for (int k = 0; k < 4; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+    if (__lane_id() == 0) {
+        atomicAdd(&c->ext_nodes, v[0]);
+        atomicAdd(&c->ext_prims, v[1]);
+        atomicAdd(&c->shadow_nodes, v[2]);
+        atomicAdd(&c->shadow_prims, v[3]);
+    }
+}
+
 __device__ __forceinline__ void add_ray_counts(Counters* c, uint32_t ext, uint32_t shadow) {
     unsigned long long e = ext, s = shadow;
     for (int off = 32; off > 0; off >>= 1) {
@@ -92,7 +110,8 @@ __device__ __forceinline__ void add_ray_counts(Counters* c, uint32_t ext, uint32
 }
 
 // ------------------------------------------------------------------ volume
-__device__ v3 volume_path(Tracer& T, const cupbr_camera& cam, uint32_t tid, uint32_t w, uint32_t h, uint32_t frame,
+template <class TR>
+__device__ v3 volume_path(TR& T, const cupbr_camera& cam, uint32_t tid, uint32_t w, uint32_t h, uint32_t frame,
                           uint32_t max_depth, bool rr) {
     const DevScene& S = T.S;
     const float inf = __builtin_inff();
@@ -233,13 +252,14 @@ __device__ __forceinline__ void running_mean(float* px, v3 L, uint32_t f) {  // 
     }
 }
 
+template <bool kCount>
 __global__ __launch_bounds__(kIntBlock) void k_volume(DevScene S, cupbr_camera cam, uint32_t w, uint32_t h,
                                                       uint32_t frame0, uint32_t nspp, uint32_t max_depth, int rr,
                                                       float* __restrict__ img, Counters* c, uint32_t* spill) {
     __shared__ uint32_t lds[kIntLds * kIntBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     LaneStack<kIntLds> st{lds + threadIdx.x, kIntBlock, spill ? spill + gtid : nullptr, gridDim.x * blockDim.x};
-    Tracer T{S, st};
+    TracerT<kCount> T{S, st};
     const uint32_t npix = w * h;
     if (gtid < npix) {
         for (uint32_t s = 0; s < nspp; ++s) {
@@ -248,12 +268,14 @@ __global__ __launch_bounds__(kIntBlock) void k_volume(DevScene S, cupbr_camera c
         }
     }
     add_ray_counts(c, T.ext, T.shadow);
+    add_visit_counts(c, T);
 }
 
 // ------------------------------------------------------------------ gradient-domain
 // collect_radiance, GradientDomain.cu:110-158. The base path records which vertices exist
 // (valid bits); a shift path stops at the first vertex the base path does not have (:127).
-__device__ v3 gdpt_collect(Tracer& T, v3 o, v3 d, uint32_t seed, uint32_t max_depth, uint32_t* valid_out,
+template <class TR>
+__device__ v3 gdpt_collect(TR& T, v3 o, v3 d, uint32_t seed, uint32_t max_depth, uint32_t* valid_out,
                            uint32_t base_valid, bool shift) {
     const DevScene& S = T.S;
     v3 radiance = splat(0.0f), weight = splat(1.0f), out_dir = splat(0.0f);
@@ -302,6 +324,7 @@ __device__ v3 gdpt_collect(Tracer& T, v3 o, v3 d, uint32_t seed, uint32_t max_de
 }
 
 // gdpt_kernel, GradientDomain.cu:160-246: images base, gxf, gxb, gyf, gyb
+template <bool kCount>
 __global__ __launch_bounds__(kIntBlock) void k_gdpt(DevScene S, cupbr_camera cam, uint32_t w, uint32_t h,
                                                     uint32_t frame0, uint32_t nspp, uint32_t max_depth,
                                                     float* __restrict__ base, float* __restrict__ gxf,
@@ -310,7 +333,7 @@ __global__ __launch_bounds__(kIntBlock) void k_gdpt(DevScene S, cupbr_camera cam
     __shared__ uint32_t lds[kIntLds * kIntBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     LaneStack<kIntLds> st{lds + threadIdx.x, kIntBlock, spill ? spill + gtid : nullptr, gridDim.x * blockDim.x};
-    Tracer T{S, st};
+    TracerT<kCount> T{S, st};
     const uint32_t npix = w * h;
     if (gtid < npix) {
         const uint32_t px = gtid % w, py = gtid / w;
@@ -345,6 +368,7 @@ __global__ __launch_bounds__(kIntBlock) void k_gdpt(DevScene S, cupbr_camera cam
         }
     }
     add_ray_counts(c, T.ext, T.shadow);
+    add_visit_counts(c, T);
 }
 
 // init_kernel, GradientDomain.cu:248-273
@@ -450,7 +474,8 @@ int render_volume(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, u
     if (rc) return rc;
     // one launch per 4 samples keeps each launch short
     for (uint32_t s = 0; s < spp; s += 4) {
-        hipLaunchKernelGGL(k_volume, dim3(blocks), dim3(kIntBlock), 0, r->stream, r->scene->dev, cam, r->width,
+        hipLaunchKernelGGL(r->counting ? k_volume<true> : k_volume<false>, dim3(blocks), dim3(kIntBlock), 0, r->stream,
+                           r->scene->dev, cam, r->width,
                            r->height, frame0 + s, std::min(4u, spp - s), std::max(1u, r->max_trace_depth),
                            (int)r->russian_roulette, r->images[CUPBR_IMAGE_OUTPUT].as<float>(), c, spill);
     }
@@ -468,7 +493,8 @@ int render_gdpt(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uin
     float* im[kNumImages];
     for (int k = 0; k < kNumImages; ++k) im[k] = r->images[k].as<float>();
     for (uint32_t s = 0; s < spp; s += 2) {
-        hipLaunchKernelGGL(k_gdpt, dim3(blocks), dim3(kIntBlock), 0, r->stream, r->scene->dev, cam, w, h, frame0 + s,
+        hipLaunchKernelGGL(r->counting ? k_gdpt<true> : k_gdpt<false>, dim3(blocks), dim3(kIntBlock), 0, r->stream,
+                           r->scene->dev, cam, w, h, frame0 + s,
                            std::min(2u, spp - s), std::max(1u, r->max_trace_depth), im[CUPBR_IMAGE_BASE],
                            im[CUPBR_IMAGE_GRADIENT_X_FORWARD], im[CUPBR_IMAGE_GRADIENT_X_BACKWARD],
                            im[CUPBR_IMAGE_GRADIENT_Y_FORWARD], im[CUPBR_IMAGE_GRADIENT_Y_BACKWARD], c, spill);

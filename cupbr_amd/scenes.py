@@ -365,3 +365,24 @@ def grid_medium_scene(res=64, width=256, height=256, seed=3, target_seed=4, targ
     cam = dict(eye=(0.5, 0.5, -1.9), target=(0.5, 0.5, 0.5), up=(0, 1, 0), vfov=45.0, width=width, height=height)
     return dict(desc=b.desc(), camera=cam, medium=medium, density=d.astype(np.float32),
                 target_density=target.astype(np.float32), albedo=np.array(albedo, np.float32))
+
+
+def volume_spheres():
+    """The volume renderer's test scene: a box of grey quads (green and red side walls), a
+    scattering sphere with a glass interface (sigma_s 2, sigma_a 0.5, g 0.5), a coloured
+    scattering sphere, a glass sphere and a 0.4 x 0.4 area light, in a thin scene medium
+    (sigma_s = 0.05, sigma_a = 0.02). Seen by the reference's default camera (origin, +z)."""
+    b = SceneBuilder()
+    grey = material(LAMBERT, albedo_d=(0.7, 0.7, 0.7))
+    b.quad((0, -1, 2), (0, 1, 0), (1, 0, 0), (0, 0, 1), grey)
+    b.quad((0, 1, 2), (0, -1, 0), (1, 0, 0), (0, 0, 1), grey)
+    b.quad((-1, 0, 2), (1, 0, 0), (0, 1, 0), (0, 0, 1), material(LAMBERT, albedo_d=(0.1, 0.7, 0.1)))
+    b.quad((1, 0, 2), (-1, 0, 0), (0, 1, 0), (0, 0, 1), material(LAMBERT, albedo_d=(0.7, 0.1, 0.1)))
+    b.quad((0, 0, 3), (0, 0, -1), (1, 0, 0), (0, 1, 0), grey)
+    b.sphere((-0.4, -0.5, 2.2), 0.4, material(VOLUME, sigma_s=(2, 2, 2), sigma_a=(0.5, 0.5, 0.5), g=0.5,
+                                               interface=1))
+    b.sphere((0.45, -0.6, 2.5), 0.35, material(VOLUME, sigma_s=(1, 3, 1), sigma_a=(0.2, 0.2, 0.2)))
+    b.sphere((0.3, 0.3, 2.6), 0.25, material(GLASS, albedo_s=(1, 1, 1)))
+    b.area_light((0, 0.95, 2), (10, 10, 10), (0.2, 0, 0), (0, 0, 0.2))
+    b.volume = np.array([0.05, 0.05, 0.05, 0.02, 0.02, 0.02, 0.0], np.float32)
+    return b.desc()

@@ -217,20 +217,7 @@ def test_furnace_on_gpu():
 
 
 def _volume_scene():
-    b = scenes.SceneBuilder()
-    grey = scenes.material(scenes.LAMBERT, albedo_d=(0.7, 0.7, 0.7))
-    b.quad((0, -1, 2), (0, 1, 0), (1, 0, 0), (0, 0, 1), grey)
-    b.quad((0, 1, 2), (0, -1, 0), (1, 0, 0), (0, 0, 1), grey)
-    b.quad((-1, 0, 2), (1, 0, 0), (0, 1, 0), (0, 0, 1), scenes.material(scenes.LAMBERT, albedo_d=(0.1, 0.7, 0.1)))
-    b.quad((1, 0, 2), (-1, 0, 0), (0, 1, 0), (0, 0, 1), scenes.material(scenes.LAMBERT, albedo_d=(0.7, 0.1, 0.1)))
-    b.quad((0, 0, 3), (0, 0, -1), (1, 0, 0), (0, 1, 0), grey)
-    b.sphere((-0.4, -0.5, 2.2), 0.4, scenes.material(scenes.VOLUME, sigma_s=(2, 2, 2), sigma_a=(0.5, 0.5, 0.5), g=0.5,
-                                                     interface=1))
-    b.sphere((0.45, -0.6, 2.5), 0.35, scenes.material(scenes.VOLUME, sigma_s=(1, 3, 1), sigma_a=(0.2, 0.2, 0.2)))
-    b.sphere((0.3, 0.3, 2.6), 0.25, scenes.material(scenes.GLASS, albedo_s=(1, 1, 1)))
-    b.area_light((0, 0.95, 2), (10, 10, 10), (0.2, 0, 0), (0, 0, 0.2))
-    b.volume = np.array([0.05, 0.05, 0.05, 0.02, 0.02, 0.02, 0.0], np.float32)
-    return b.desc()
+    return scenes.volume_spheres()
 
 
 def test_volume_renderer_same_seed_parity():
