@@ -500,7 +500,7 @@ def bench_integrator(args, log):
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         ref = oracle.OracleScene(desc, precisions=(32,))
-        cs = 32 if args.workload == "gdpt" else 64  # ~10 s of CPU work on 16 threads
+        cs = 128 if args.workload == "gdpt" else 640  # ~10 s of CPU work on 16 threads
         t0 = time.perf_counter()
         if args.workload == "gdpt":
             bufs = ref.gdpt(cam.as_dict(), W, H, 0, cs, depth=depth, precision=32, threads=threads)
