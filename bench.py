@@ -157,6 +157,14 @@ def main():
         f"({sc['shadow_node_visits'] / max(sh_rays, 1):.1f} nodes, {sc['shadow_prim_tests'] / max(sh_rays, 1):.1f} tris); "
         f"{'trace' if fused else 'extend'} kernel {ext_launch_ms:.3f} ms/launch, {achieved:.0f} GB/s")
 
+    # not part of `value`: reading the 1080p FP32 image back to the host over PCIe
+    t0 = time.perf_counter()
+    for _ in range(5):
+        r.get_output_image()
+    t_read = (time.perf_counter() - t0) / 5
+    log(f"host read-back of the {W}x{H} FP32 image: {t_read * 1e3:.2f} ms ({W * H * 12 / t_read / 1e9:.1f} GB/s); "
+        f"with one read-back per step the rate would be {samples / (t_total + args.steps * t_read) / 1e6:.1f} Msamples/s")
+
     traffic = None
     tf = os.path.join(ROOT, "profiles", "trace_traffic.json")
     if os.path.exists(tf):

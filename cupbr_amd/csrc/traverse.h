@@ -220,11 +220,17 @@ __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float di
         CUPBR_CHILD(2, z)
         CUPBR_CHILD(3, w)
 #undef CUPBR_CHILD
-        CUPBR_CSWAP(0, 1)
-        CUPBR_CSWAP(2, 3)
-        CUPBR_CSWAP(0, 2)
-        CUPBR_CSWAP(1, 3)
-        CUPBR_CSWAP(1, 2)
+        if (!kAny) {  // nearest child first; an any-hit query only needs the ones that were hit
+            CUPBR_CSWAP(0, 1)
+            CUPBR_CSWAP(2, 3)
+            CUPBR_CSWAP(0, 2)
+            CUPBR_CSWAP(1, 3)
+            CUPBR_CSWAP(1, 2)
+        } else {
+            CUPBR_CSWAP(0, 1)  // a hit child, if any, into slot 0
+            CUPBR_CSWAP(0, 2)
+            CUPBR_CSWAP(0, 3)
+        }
         if (!(key[0] < inf)) return false;
         if (key[3] < inf) st.put(sp++, val[3]);
         if (key[2] < inf) st.put(sp++, val[2]);
