@@ -628,6 +628,11 @@ class GridRenderer:
             alb = albedo.detach().contiguous().float()
             prop = None if proposal is None else proposal.detach().contiguous().float()
             di = d_image.detach().contiguous().float()
+            for name, buf, like in (("grad_density", grad_density, rho), ("grad_albedo", grad_albedo, alb)):
+                if buf is not None and not (_is_torch_gpu(buf) and buf.dtype == torch.float32 and buf.is_contiguous()
+                                            and buf.device == like.device and buf.numel() == like.numel()):
+                    raise ValueError(f"{name} must be a contiguous float32 tensor on {like.device} with "
+                                     f"{like.numel()} elements")
             gr = torch.zeros_like(rho) if grad_density is None else grad_density
             ga = torch.zeros_like(alb) if grad_albedo is None else grad_albedo
             stream = torch.cuda.current_stream(rho.device).cuda_stream

@@ -52,6 +52,16 @@ constexpr size_t kMaxGridSamples = (size_t)1 << 24;  // samples per forward chun
 // so the expected count per segment is below 1e5 and the bound is never reached in practice;
 // reaching it ends the path (or zeroes the shadow ray's transmittance), so every loop terminates.
 constexpr uint32_t kMaxTrackSteps = 1u << 22;
+// The adjoint's density-gradient atomics go to one private copy of the gradient per group of
+// blocks, block b -> copy b % kGradCopies. Workgroups are dispatched to the 8 XCDs round-robin,
+// so each copy is written by one XCD and its lines stay in that XCD's L2 instead of moving
+// between L2s on every atomic; a reduction adds the copies to the caller's gradient in copy
+// order. The mapping only affects speed: any assignment of blocks to copies sums the same terms.
+#ifndef CUPBR_GRID_GRAD_COPIES
+#define CUPBR_GRID_GRAD_COPIES 8
+#endif
+constexpr uint32_t kGradCopies = CUPBR_GRID_GRAD_COPIES;
+constexpr size_t kMaxGradCopyBytes = (size_t)1 << 30;  // larger grids scatter into the output directly
 
 struct GridDev {
     const float* rho;   // density parameters
@@ -347,8 +357,8 @@ __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupb
                                                      uint32_t spp, uint32_t frame0, uint32_t s_begin, uint32_t s_count,
                                                      uint32_t max_depth, float* __restrict__ samples,
                                                      const float* __restrict__ d_image, float* grad_rho,
-                                                     float* grad_alb, uint32_t n_alb, uint32_t* spill,
-                                                     unsigned long long* counts) {
+                                                     float* grad_alb, uint32_t n_alb, size_t n_vox,
+                                                     uint32_t* spill, unsigned long long* counts) {
     __shared__ uint32_t lds[kGridLds * kGridBlock];
     // per-block albedo gradients (few surfaces, hit by every path): LDS atomics, one global
     // atomic per entry at the end
@@ -360,6 +370,7 @@ __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupb
     }
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
+    if (kAdj && n_vox) grad_rho += (size_t)(blockIdx.x % kGradCopies) * n_vox;  // this block's copy
     LaneStack<kGridLds> st{lds + threadIdx.x, kGridBlock, spill ? spill + gtid : nullptr, (uint32_t)stride};
     GridTracer T{S, st};
     const uint32_t npix = w * h;
@@ -392,6 +403,21 @@ __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupb
             for (int off = 32; off > 0; off >>= 1) c[k] += __shfl_xor(c[k], off);
             if (__lane_id() == 0) atomicAdd(counts + k, c[k]);
         }
+    }
+}
+
+__global__ void k_grid_zero(float4* __restrict__ p, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// grad[v] += sum over copies c (in order) of copies[c * stride + v], v < n_vox <= stride
+__global__ void k_grid_reduce(float* __restrict__ grad, const float* __restrict__ copies, size_t n_vox,
+                              size_t stride) {
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n_vox; v += (size_t)gridDim.x * blockDim.x) {
+        float sum = 0.0f;
+        for (uint32_t c = 0; c < kGradCopies; ++c) sum += copies[(size_t)c * stride + v];
+        if (sum != 0.0f) grad[v] += sum;
     }
 }
 
@@ -501,7 +527,8 @@ int grid_render_device(const cupbr_scene* cs, const cupbr_camera* cam, const cup
         uint32_t* spill;
         if (grid_launch_shape(s, (size_t)n_px * cnt, &blocks, &spill)) return CUPBR_ERR_OUT_OF_MEMORY;
         hipLaunchKernelGGL(k_grid<false>, dim3(blocks), dim3(kGridBlock), 0, stream, s->dev, G, *cam, w, h, spp, frame0,
-                           s0, cnt, max_depth, s->grid_samples.as<float>(), nullptr, nullptr, nullptr, 0u, spill, counts);
+                           s0, cnt, max_depth, s->grid_samples.as<float>(), nullptr, nullptr, nullptr, 0u, (size_t)0,
+                           spill, counts);
         CUPBR_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_grid_accumulate, dim3((n_px + 255) / 256), dim3(256), 0, stream, image,
                            s->grid_samples.as<float>(), n_px, cnt, s0 == 0 ? 1 : 0, s0 + cnt == spp ? 1 : 0, spp);
@@ -521,10 +548,29 @@ int grid_adjoint_device(const cupbr_scene* cs, const cupbr_camera* cam, const cu
     int blocks;
     uint32_t* spill;
     if (grid_launch_shape(s, (size_t)w * h * spp, &blocks, &spill)) return CUPBR_ERR_OUT_OF_MEMORY;
+    // private gradient copies (cleared by a kernel, not a memset, so that a captured graph replays it)
+    const size_t n_vox = (size_t)m->nx * m->ny * m->nz;
+    const size_t copy_floats = (n_vox + 3) & ~(size_t)3;  // 16-byte aligned copies
+    const bool private_copies = kGradCopies > 1 && copy_floats * kGradCopies * sizeof(float) <= kMaxGradCopyBytes;
+    float* target = grad_rho;
+    if (private_copies) {
+        if (s->grid_grad.ensure(copy_floats * kGradCopies * sizeof(float))) return CUPBR_ERR_OUT_OF_MEMORY;
+        target = s->grid_grad.as<float>();
+        const size_t n4 = copy_floats * kGradCopies / 4;
+        hipLaunchKernelGGL(k_grid_zero, dim3((unsigned)std::min<size_t>((n4 + 255) / 256, 4096)), dim3(256), 0, stream,
+                           reinterpret_cast<float4*>(target), n4);
+        CUPBR_HIP_CHECK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_grid<true>, dim3(blocks), dim3(kGridBlock), 0, stream, s->dev, G, *cam, w, h, spp, frame0, 0u,
-                       spp, max_depth, nullptr, d_image, grad_rho, grad_alb,
-                       (uint32_t)(3 * (s->objects.size() + s->triangles.size() / 9)), spill, counts);
+                       spp, max_depth, nullptr, d_image, target, grad_alb,
+                       (uint32_t)(3 * (s->objects.size() + s->triangles.size() / 9)),
+                       private_copies ? copy_floats : (size_t)0, spill, counts);
     CUPBR_HIP_CHECK(hipGetLastError());
+    if (private_copies) {
+        hipLaunchKernelGGL(k_grid_reduce, dim3((unsigned)std::min<size_t>((n_vox + 255) / 256, 4096)), dim3(256), 0,
+                           stream, grad_rho, target, n_vox, copy_floats);
+        CUPBR_HIP_CHECK(hipGetLastError());
+    }
     return CUPBR_OK;
 }
 

@@ -38,6 +38,7 @@ struct cupbr_scene {
     cupbr::DevBuf grid_spill;          // traversal stack spill of the grid-medium kernels
     cupbr::DevBuf grid_samples;        // per-sample radiance of the grid-medium forward pass
     cupbr::DevBuf grid_counts;         // 4 x u64 work counters of the grid-medium kernels
+    cupbr::DevBuf grid_grad;           // per-XCD density-gradient copies of the adjoint
     bool grid_counting = false;
 };
 
