@@ -32,9 +32,10 @@ constexpr int kPtWaves = kPtBlock / 64;
 // VGPRs, not LDS, then bound its occupancy at 8 waves per SIMD; deeper entries spill to memory)
 constexpr int kPtLds = 16;
 
+// one wave-uniform ticket per call
 __device__ __forceinline__ uint32_t wave_fetch(uint32_t* counter) {
     uint32_t base = 0;
-    if (__lane_id() == 0) base = atomicAdd(counter, 64u);
+    if (__lane_id() == 0) base = atomicAdd(counter, 1u);
     return __shfl(base, 0);
 }
 
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
         c->queue_count[0].v = npath;
-        c->work_extend.v = 0;
+        for (int g = 0; g < 8; ++g) c->work[g].v = 0;
         c->shadow_count[0].v = 0;
         c->shadow_count[1].v = 0;
         c->query_count.v = 0;
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
                                                        Counters* c, NrcShade nrc) {
     __shared__ uint32_t s_append[kPtWaves + 1];
     const uint32_t n = c->queue_count[cur].v;
-    if (blockIdx.x == 0 && threadIdx.x == 0) c->work_extend.v = 0;  // extend of the next bounce starts at 0
+    if (blockIdx.x == 0 && threadIdx.x < 8) c->work[threadIdx.x].v = 0;  // the next trace starts at 0
     const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
     for (uint32_t base = blockIdx.x * kPtBlock; base < n; base += gridDim.x * kPtBlock) {
         const uint32_t i = base + threadIdx.x;
@@ -324,27 +325,38 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_trace(DevScene S, int cur, cons
     }
     LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
     TravCount te, ts;
-    const uint32_t ext_span = (n_ext + 63u) & ~63u;
-    const uint32_t total = ext_span + n_sh;
-    for (;;) {
-        const uint32_t base = wave_fetch(&c->work_extend.v);
-        if (base >= total) break;
-        const uint32_t i = base + __lane_id();
-        if (base < ext_span) {
-            if (i < n_ext) {
-                const float4 o4 = q_o[i], d4 = q_d[i];
-                TraceResult r;
-                traverse<false, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r, &te);
-                q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
-            }
-        } else {
-            const uint32_t j = i - ext_span;
-            if (j < n_sh) {
-                const float4 o4 = s_o[j], d4 = s_d[j];
-                TraceResult r;
-                const bool occluded =
-                    traverse<true, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r, &ts);
-                shadow_result(occluded, d4, s_c, j, rad);
+    // 64-ray chunks, split into 8 groups: group h owns the h-th eighth of the extension chunks
+    // and the h-th eighth of the shadow chunks. Workgroups are dispatched to the 8 XCDs
+    // round-robin, so the waves of block b start on group b % 8: each XCD traces neighbouring
+    // queue entries (neighbouring pixels and hit points) and its own L2 holds the nodes they
+    // share. A group that runs dry takes chunks from the next groups in turn; every chunk is
+    // handed out once, whatever XCD a block runs on.
+    const uint32_t e_chunks = (n_ext + 63u) >> 6, s_chunks = (n_sh + 63u) >> 6;
+    const uint32_t g0 = blockIdx.x & 7u;
+    for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t h = (g0 + k) & 7u;
+        const uint32_t e0 = h * e_chunks / 8u, e_len = (h + 1u) * e_chunks / 8u - e0;
+        const uint32_t s0 = h * s_chunks / 8u, s_len = (h + 1u) * s_chunks / 8u - s0;
+        for (;;) {
+            const uint32_t q = wave_fetch(&c->work[h].v);
+            if (q >= e_len + s_len) break;
+            if (q < e_len) {
+                const uint32_t i = ((e0 + q) << 6) + __lane_id();
+                if (i < n_ext) {
+                    const float4 o4 = q_o[i], d4 = q_d[i];
+                    TraceResult r;
+                    traverse<false, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r, &te);
+                    q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
+                }
+            } else {
+                const uint32_t j = ((s0 + q - e_len) << 6) + __lane_id();
+                if (j < n_sh) {
+                    const float4 o4 = s_o[j], d4 = s_d[j];
+                    TraceResult r;
+                    const bool occluded =
+                        traverse<true, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r, &ts);
+                    shadow_result(occluded, d4, s_c, j, rad);
+                }
             }
         }
     }

@@ -13,8 +13,8 @@ namespace cupbr {
 constexpr int kNumImages = 8;  // cupbr_image
 
 // Device counters of one render call, one 128-byte line each so that atomics on
-// different counters never serialise on the same L2 line. The extension work
-// counter hands out 64-ray chunks to persistent waves; each counter is reset by
+// different counters never serialise on the same L2 line. The trace kernel's work
+// counters hand out 64-ray chunks to persistent waves; each counter is reset by
 // the kernel that runs after its last user.
 struct alignas(128) CounterLine {
     uint32_t v;
@@ -23,7 +23,7 @@ struct alignas(128) CounterLine {
 struct Counters {
     CounterLine queue_count[2];  // extension queues (ping-pong by bounce parity)
     CounterLine shadow_count[2]; // shadow queues (bounce parity)
-    CounterLine work_extend;     // work counter of the trace kernel
+    CounterLine work[8];         // work counters of the trace kernel, one per XCD group
     CounterLine query_count;  // radiance-cache queries of the batch
     CounterLine row_count;    // radiance-cache training rows of the batch
     unsigned long long extension_rays;
