@@ -304,15 +304,17 @@ __device__ __forceinline__ void shadow_result(bool occluded, const float4 d4, co
     rad[p] = make_float4(L.x, L.y, L.z, R.w);
 }
 
-// one bounce's extension rays (queue cur) and the previous bounce's shadow rays (parity cur ^ 1)
-// under one work counter: chunk k < ceil(n_ext / 64) is extension rays [64k, 64k + 64), the rest
-// shadow rays; every wave's chunk is of one kind
+// One bounce's extension rays (queue cur) and the previous bounce's shadow rays (parity cur ^ 1),
+// handed out in 64-ray chunks by the 8 group counters below; every wave's chunk is of one kind.
+// amdgpu_waves_per_eu(8, 8): the compiler keeps the kernel within the SGPR budget of 8 waves per
+// SIMD (spilling a few SGPRs into VGPR lanes); without it 104 SGPRs held the kernel to 6 resident
+// waves per SIMD and a quarter of the persistent blocks only started when the queue was empty.
 template <bool kCount>
-__global__ __launch_bounds__(kPtBlock) void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o,
-                                                       const float4* __restrict__ q_d, float2* __restrict__ q_hit,
-                                                       const float4* __restrict__ s_o, const float4* __restrict__ s_d,
-                                                       const float4* __restrict__ s_c, float4* __restrict__ rad,
-                                                       Counters* c, uint32_t* spill, uint32_t spill_stride) {
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o, const float4* __restrict__ q_d,
+                float2* __restrict__ q_hit, const float4* __restrict__ s_o, const float4* __restrict__ s_d,
+                const float4* __restrict__ s_c, float4* __restrict__ rad, Counters* c, uint32_t* spill,
+                uint32_t spill_stride) {
     __shared__ uint32_t lds[kPtLds * kPtBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_ext = c->queue_count[cur].v;
