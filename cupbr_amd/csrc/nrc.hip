@@ -446,6 +446,11 @@ int nrc_infer(cupbr_nrc* c, const float* rec, uint32_t n, const uint32_t* n_dev,
     return CUPBR_OK;
 }
 
+__global__ void k_nrc_zero(uint4* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 int nrc_train(cupbr_nrc* c, const float* rows, uint32_t n_rows, uint32_t batch, uint32_t steps, float lr,
               float* loss_dev, hipStream_t stream) {
     if (n_rows == 0 || steps == 0) return CUPBR_OK;
@@ -457,9 +462,13 @@ int nrc_train(cupbr_nrc* c, const float* rows, uint32_t n_rows, uint32_t batch, 
         return rc;
     __bf16* X = c->acts.as<__bf16>();
     __bf16* G = X + (size_t)5 * 64 * batch;
-    // rows 3..63 of the output-layer gradient are never written: clear them once per size
+    // rows 3..63 of the output-layer gradient are never written: clear them once per size (by a
+    // kernel rather than a memset, so that a caller may capture the call in a graph)
     if (c->acts_cleared_for != batch) {
-        CUPBR_HIP_CHECK(hipMemsetAsync(c->acts.ptr, 0, (size_t)2 * 5 * 64 * batch * 2, stream));
+        const size_t n4 = (size_t)2 * 5 * 64 * batch * 2 / 16;
+        hipLaunchKernelGGL(k_nrc_zero, dim3((unsigned)std::min<size_t>((n4 + 255) / 256, 4096)), dim3(256), 0, stream,
+                           c->acts.as<uint4>(), n4);
+        CUPBR_HIP_CHECK(hipGetLastError());
         c->acts_cleared_for = batch;
     }
     const int blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>(batch / 64, (uint32_t)device_cu_count() * 2));
