@@ -420,9 +420,19 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     const uint32_t w = r->width, h = r->height, npix = w * h;
     if (npix == 0 || spp == 0) return CUPBR_OK;
     const bool nrc = r->nrc != nullptr;
-    // paths in flight: enough to fill the GPU at the deepest bounce, bounded to ~8M; with a
-    // radiance cache one frame per batch (the cache trains between frames)
-    const uint32_t target = 1u << 22, cap = 1u << 23;
+    // Paths in flight, in whole frames: about 32M (200 B of path state and queues per path, so
+    // ~6.4 GB). A trace launch ends with the last 64-ray chunks of its slowest waves, and with
+    // more paths per launch that tail is a smaller share of it (1080p, 64 spp: 3 frames per batch
+    // 475 Msamples/s, 9 frames 518, 17 frames 533, all 64 frames 543). Bounded to 64M paths and
+    // to an eighth of the device memory; with a radiance cache one frame per batch (the cache
+    // trains between frames).
+    const uint32_t target = 1u << 25;
+    uint32_t cap = 1u << 26;
+    {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
+            cap = (uint32_t)std::min<size_t>(cap, std::max<size_t>(total_b / 8 / 200, 1u << 20));
+    }
     uint32_t batch = std::max<uint32_t>(1, (target + npix - 1) / npix);
     batch = std::min(batch, std::max<uint32_t>(1, cap / npix));
     batch = std::min(batch, spp);
