@@ -293,21 +293,10 @@ __device__ v3 grid_path(const DevScene& S, const GridDev& G, GridTracer& T, cons
         const v3 P = o + ts * d;
         v3 N;
         uint32_t surf;
-        if (hr.prim & kUnbBit) {
-            const UnbRec& u = S.unb[hr.prim & ~kUnbBit];
-            N = mk(u.n.x, u.n.y, u.n.z);
-            surf = __float_as_uint(u.e1.w);
-        } else {
-            const PrimRec& pr = S.prims[hr.prim];
-            const uint32_t obj = __float_as_uint(pr.c.z);
-            if ((__float_as_uint(pr.c.w) & 3u) == kPrimSphere) {
-                N = normalize(P - mk(pr.a.x, pr.a.y, pr.a.z));
-            } else {
-                const float4 n4 = S.normals[hr.prim];
-                N = mk(n4.x, n4.y, n4.z);
-            }
-            const ObjInfo& oi = S.objs[obj];
-            surf = oi.type == CUPBR_MESH ? G.n_objs + oi.first_triangle + (hr.rank - oi.rank_base) : obj;
+        hit_geometry(S, hr.prim, P, N, surf);
+        if (!(hr.prim & kUnbBit)) {
+            const ObjInfo& oi = S.objs[surf];
+            if (oi.type == CUPBR_MESH) surf = G.n_objs + oi.first_triangle + (hr.rank - oi.rank_base);
         }
         if (dot(N, d) > 0.0f) N = -1.0f * N;
         const v3 a = ld3(G.alb + 3 * (size_t)surf);

@@ -70,24 +70,6 @@ struct TracerT {
 };
 using Tracer = TracerT<false>;
 
-// normal and object of a hit (LocalGeometry, Geometry.h:28-36)
-__device__ __forceinline__ void geometry_of(const DevScene& S, uint32_t prim, v3 P, v3& N, uint32_t& obj) {
-    if (prim & kUnbBit) {
-        const UnbRec& u = S.unb[prim & ~kUnbBit];
-        N = mk(u.n.x, u.n.y, u.n.z);
-        obj = __float_as_uint(u.e1.w);
-        return;
-    }
-    const PrimRec& pr = S.prims[prim];
-    obj = __float_as_uint(pr.c.z);
-    if ((__float_as_uint(pr.c.w) & 3u) == kPrimSphere) {
-        N = normalize(P - mk(pr.a.x, pr.a.y, pr.a.z));
-    } else {
-        const float4 n = S.normals[prim];
-        N = mk(n.x, n.y, n.z);
-    }
-}
-
 template <bool kCount>
 __device__ __forceinline__ void add_visit_counts(Counters* c, const TracerT<kCount>& T) {
     if (!kCount) return;
@@ -141,7 +123,7 @@ __device__ v3 volume_path(TR& T, const cupbr_camera& cam, uint32_t tid, uint32_t
         const v3 P = o + hr.t * d;
         v3 N;
         uint32_t obj;
-        geometry_of(S, hr.prim, P, N, obj);
+        hit_geometry(S, hr.prim, P, N, obj);
         const cupbr_material* mp = S.mats + obj;
         const Mat m = load_mat(mp);
         if (inside) vol = mp->volume;  // the payload's volume pointer follows the current hit (:277)
@@ -299,7 +281,7 @@ __device__ v3 gdpt_collect(TR& T, v3 o, v3 d, uint32_t seed, uint32_t max_depth,
         const v3 P = o + hr.t * d;
         v3 N;
         uint32_t obj;
-        geometry_of(S, hr.prim, P, N, obj);
+        hit_geometry(S, hr.prim, P, N, obj);
         const Mat m = load_mat(S.mats + obj);
         const v3 inc = -1.0f * d;
         radiance += weight * m.albedo_e;
@@ -441,7 +423,7 @@ __global__ __launch_bounds__(kIntBlock) void k_raytrace(DevScene S, cupbr_camera
             const v3 P = pos + hr.t * d;
             v3 N;
             uint32_t obj;
-            geometry_of(S, hr.prim, P, N, obj);
+            hit_geometry(S, hr.prim, P, N, obj);
             const Mat m = load_mat(S.mats + obj);
             const v3 inc = normalize(pos - P);
             for (uint32_t l = 0; l < S.n_lights; ++l) {

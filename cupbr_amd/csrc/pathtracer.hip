@@ -116,26 +116,6 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
     rad[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(flags));
 }
 
-
-// geometry of a closest-hit record: world normal and object id (LocalGeometry, Geometry.h:28-36)
-__device__ __forceinline__ void hit_geometry(const DevScene& S, uint32_t prim, v3 P, v3& N, uint32_t& obj) {
-    if (prim & kUnbBit) {
-        const UnbRec& u = S.unb[prim & ~kUnbBit];
-        N = mk(u.n.x, u.n.y, u.n.z);
-        obj = __float_as_uint(u.e1.w);
-        return;
-    }
-    const float4 c = S.prims[prim].c;
-    obj = __float_as_uint(c.z);
-    if ((__float_as_uint(c.w) & 3u) == kPrimSphere) {
-        const float4 a = S.prims[prim].a;
-        N = normalize(P - mk(a.x, a.y, a.z));  // SphereDetail.h:66
-    } else {
-        const float4 n = S.normals[prim];
-        N = mk(n.x, n.y, n.z);
-    }
-}
-
 // radiance-cache records written by shade: a query (inputs 16, throughput 3, path) ends a rendering
 // path at its first diffuse/glossy vertex after the camera hit; a training path leaves a pending
 // row (inputs 16, radiance so far 3, throughput 3, path, pad) at each such vertex

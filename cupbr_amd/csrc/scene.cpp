@@ -230,7 +230,7 @@ int scene_finalize(cupbr_scene* s) {
             pr.b = make_float4(0, 0, 0, 0);
             pr.c.x = 0.0f;
             flags = kPrimSphere | kPrimNeedsBoxCheck;
-            normals[k] = make_float4(0, 0, 0, 0);
+            normals[k] = make_float4(0, 0, 0, bits_to_float(sp.obj | kNormalSphereBit));
         } else {
             v3 a = tri_vertex(s, o, sp.tri, 0), b = tri_vertex(s, o, sp.tri, 1), c = tri_vertex(s, o, sp.tri, 2);
             v3 e1 = b - a, e2 = c - a;  // TriangleDetail.h:27-28
@@ -239,7 +239,7 @@ int scene_finalize(cupbr_scene* s) {
             pr.b = make_float4(e1.y, e1.z, e2.x, e2.y);
             pr.c.x = e2.z;
             flags = kPrimTriangle | (o.type == CUPBR_MESH ? kPrimNeedsBoxCheck : 0u);
-            normals[k] = make_float4(n.x, n.y, n.z, 0.0f);
+            normals[k] = make_float4(n.x, n.y, n.z, bits_to_float(sp.obj));
         }
         pr.c.y = bits_to_float(sp.rank);
         pr.c.z = bits_to_float(sp.obj);

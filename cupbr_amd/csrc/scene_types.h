@@ -42,6 +42,9 @@ static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
 
 enum PrimKind : uint32_t { kPrimTriangle = 0, kPrimSphere = 1 };
 constexpr uint32_t kPrimNeedsBoxCheck = 4u;  // object box is finite (sphere, mesh): test it lazily
+// normals[prim].w (bits): object id in bits 0..30, bit 31 set for a sphere (whose normal is
+// computed from the hit point, not stored)
+constexpr uint32_t kNormalSphereBit = 0x80000000u;
 
 struct alignas(16) PrimRec {
     float4 a;  // tri: v1.x v1.y v1.z e1.x | sphere: c.x c.y c.z r
@@ -73,7 +76,7 @@ struct ObjInfo {
 struct DevScene {
     const Node4* nodes;
     const PrimRec* prims;
-    const float4* normals;  // per prim in leaf order (triangles)
+    const float4* normals;  // per prim in leaf order: triangle normal; w = object id | kNormalSphereBit
     const UnbRec* unb;
     const ObjInfo* objs;
     const cupbr_material* mats;

@@ -211,4 +211,25 @@ __device__ __forceinline__ void launch_ray(uint32_t px, uint32_t py, uint32_t w,
     d = normalize(wp - pos);
 }
 
+// Normal and object of a closest hit (LocalGeometry, lib/include/Geometry/Geometry.h:28-36):
+// planes and quads from their record, triangles from the stored face normal, spheres from the
+// hit point (SphereDetail.h:66). One 16-byte record per triangle hit.
+__device__ __forceinline__ void hit_geometry(const DevScene& S, uint32_t prim, v3 P, v3& N, uint32_t& obj) {
+    if (prim & kUnbBit) {
+        const UnbRec& u = S.unb[prim & ~kUnbBit];
+        N = mk(u.n.x, u.n.y, u.n.z);
+        obj = __float_as_uint(u.e1.w);
+        return;
+    }
+    const float4 n = S.normals[prim];
+    const uint32_t bits = __float_as_uint(n.w);
+    obj = bits & ~kNormalSphereBit;
+    if (bits & kNormalSphereBit) {
+        const float4 a = S.prims[prim].a;
+        N = normalize(P - mk(a.x, a.y, a.z));
+    } else {
+        N = mk(n.x, n.y, n.z);
+    }
+}
+
 }  // namespace cupbr
