@@ -4,15 +4,15 @@
 //   raygen      camera rays into extension queue 0                    (PathTracer.cu:112-117)
 //   per bounce: trace    closest hit for the bounce's extension rays   (:126), and any-hit
 //                        visibility for the previous bounce's shadow rays, adding their
-//                        NEE terms (:64-73), under one work counter
+//                        NEE terms (:64-73), in one launch
 //               shade    emission, NEE sample -> shadow queue,         (:127-158)
 //                        Russian roulette, BSDF sample -> next extension queue
 //   final trace the last bounce's shadow rays
 //   accumulate  the reference running mean, frame by frame             (:162-168)
 // Queue lengths live on the device, so no host round trip happens between bounces.
-// trace is a persistent grid whose waves take 64-ray chunks from a work counter
-// (traversal cost varies per ray; one launch per bounce keeps the GPU full where the
-// shadow queue alone is short); shade walks its queue with a static grid stride and
+// trace is a persistent grid whose waves take 64-ray chunks from 8 work counters, one
+// per XCD group (traversal cost varies per ray; one launch per bounce keeps the GPU full
+// where the shadow queue alone is short); shade walks its queue with a static grid stride and
 // appends to the next queues with one atomic per 256-entry block (ballot + LDS prefix).
 // Extension and shadow queues alternate by bounce parity.
 //
