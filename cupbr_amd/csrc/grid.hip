@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupb
                                                      uint32_t spp, uint32_t frame0, uint32_t s_begin, uint32_t s_count,
                                                      uint32_t max_depth, float* __restrict__ samples,
                                                      const float* __restrict__ d_image, float* grad_rho,
-                                                     float* grad_alb, uint32_t n_alb, size_t n_vox,
+                                                     float* grad_alb, uint32_t n_alb, size_t copy_stride,
                                                      uint32_t* spill, unsigned long long* counts) {
     __shared__ uint32_t lds[kGridLds * kGridBlock];
     // per-block albedo gradients (few surfaces, hit by every path): LDS atomics, one global
@@ -359,7 +359,8 @@ __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupb
     }
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    if (kAdj && n_vox) grad_rho += (size_t)(blockIdx.x % kGradCopies) * n_vox;  // this block's copy
+    // private gradient copies (copy_stride floats apart; 0: scatter into grad_rho itself)
+    if (kAdj && copy_stride) grad_rho += (size_t)(blockIdx.x % kGradCopies) * copy_stride;
     LaneStack<kGridLds> st{lds + threadIdx.x, kGridBlock, spill ? spill + gtid : nullptr, (uint32_t)stride};
     GridTracer T{S, st};
     const uint32_t npix = w * h;
