@@ -41,6 +41,7 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nrc-spp", type=int, default=4, help="samples per render call of the radiance-cache workload")
     ap.add_argument("--workload", choices=["pt", "grid", "nrc", "gdpt", "volume"], default="pt",
                     help="pt: the headline PT+NEE frame; grid: one gradient evaluation (forward + path-replay "
                          "adjoint) of the 64^3 heterogeneous medium at 256x256, 64 spp; nrc: equal-time rel-MSE of "
@@ -334,11 +335,14 @@ def bench_nrc(args, log):
     """Equal-time comparison on the 1080p / 1M-triangle scene (depth 8).
 
     reference  plain PT, 4096 spp (64 calls of 64 spp; the running mean over 4096 frames)
-    cache      online training over warm-up frames (successive frames, 2^16 training paths and 16
-               Adam steps of 2^14 rows per frame), then `steps` timed 1-spp frames from frame 0
-               (inference + training rows + the frame's training included)
-    plain PT   k spp with k = round(cache frame time / plain 1-spp frame time), frames from 0
-    value      rel-MSE(plain PT, k spp) / rel-MSE(cache, 1 spp); higher is better
+    cache      online training over 300 warm-up frames in calls of s frames (--nrc-spp, default 4;
+               2^16 training paths per frame, 16 Adam steps of 2^14 rows after each call), then
+               `steps` timed calls of s spp from frame 0 (inference, training rows and the call's
+               training included)
+    plain PT   timed at 1, 2, 4 and 8 spp; its time is fitted as a + b * spp (least squares) and
+               its rel-MSE follows c / spp (unbiased, measured at the same counts); at the cache
+               call's time T it affords s* = (T - a) / b samples, rel-MSE c / s*
+    value      rel-MSE(plain PT at time T) / rel-MSE(cache, s spp); higher is better
     roofline   the inference kernel (matrix cores): 2 * 16,576 flops per query / launch time"""
     import numpy as np
     import torch
@@ -347,6 +351,7 @@ def bench_nrc(args, log):
     from cupbr_amd import scenes
 
     W, H, DEPTH = args.width, args.height, args.depth
+    S_NRC = max(1, args.nrc_spp)
     desc, ca = scenes.million_triangles(W, H)
     scene = cb.Scene.from_desc(desc)
     scene.upload()
@@ -381,33 +386,40 @@ def bench_nrc(args, log):
     del R
 
     P = renderer()
-    frame_ms(P, 1)
-    t_pt = float(np.median([frame_ms(P, 1) for _ in range(5)]))
+    frame_ms(P, 8)
+    pt_spp, pt_ms, pt_err = [1, 2, 4, 8], [], []
+    for k in pt_spp:
+        pt_ms.append(float(np.median([frame_ms(P, k) for _ in range(3)])))
+        pt_err.append(rel_mse(P.get_output_image(), ref))
+    b_fit, a_fit = np.polyfit(pt_spp, pt_ms, 1)
+    c_fit = float(np.mean([e * k for e, k in zip(pt_err, pt_spp)]))
+    log("plain PT: " + ", ".join(f"{k} spp {t:.2f} ms rel-MSE {e:.4f}" for k, t, e in zip(pt_spp, pt_ms, pt_err)) +
+        f"; time {a_fit:.2f} + {b_fit:.2f} ms/spp, rel-MSE {c_fit:.4f}/spp")
+
     cache = cb.RadianceCache((-10.0, -10.0, -10.0), (110.0, 110.0, 110.0), seed=1)
     N = renderer(cache)
     t0 = time.perf_counter()
     warm = 300
-    for _ in range(warm):
-        N.render(cam, 1)
+    for _ in range(warm // S_NRC):
+        N.render(cam, S_NRC)
     N.synchronize()
-    log(f"cache: {warm} training frames in {time.perf_counter() - t0:.1f}s")
+    log(f"cache: {warm} training frames in calls of {S_NRC} in {time.perf_counter() - t0:.1f}s")
     times, errs = [], []
     for _ in range(args.steps):
-        times.append(frame_ms(N, 1))
+        times.append(frame_ms(N, S_NRC))
         errs.append(rel_mse(N.get_output_image(), ref))
     st = N.stats()
     t_nrc, e_nrc = float(np.median(times)), float(np.mean(errs))
-    k = max(1, int(round(t_nrc / t_pt)))
+    s_eq = max((t_nrc - a_fit) / b_fit, 1e-6)
+    e_pt = c_fit / s_eq
+    ratio = e_pt / e_nrc
+    k = max(1, int(round(s_eq)))
     P.reset()
     P.render(cam, k)
-    e_pt = rel_mse(P.get_output_image(), ref)
-    P.reset()
-    P.render(cam, 1)
-    e_pt1 = rel_mse(P.get_output_image(), ref)
-    ratio = e_pt / e_nrc
-    log(f"plain PT: 1 spp {t_pt:.2f} ms rel-MSE {e_pt1:.4f}; {k} spp (equal time) rel-MSE {e_pt:.4f}")
-    log(f"cache: frame {t_nrc:.2f} ms rel-MSE {e_nrc:.4f} ({st['cache_queries']} queries, "
-        f"{st['cache_training_rows']} training rows); equal-time ratio {ratio:.2f}")
+    e_pt_k = rel_mse(P.get_output_image(), ref)
+    log(f"cache: {S_NRC} spp in {t_nrc:.2f} ms, rel-MSE {e_nrc:.4f} ({st['cache_queries']} queries, "
+        f"{st['cache_training_rows']} training rows); plain PT in that time: {s_eq:.2f} spp, rel-MSE {e_pt:.4f} "
+        f"(rendered at {k} spp: {e_pt_k:.4f}); equal-time ratio {ratio:.2f}")
 
     # inference kernel alone, timed with HIP events on torch's stream
     q = int(st["cache_queries"])
@@ -437,11 +449,13 @@ def bench_nrc(args, log):
         "vs_baseline": None,
         "dtype": "bf16",
         "data": "synthetic (seeded procedural scene, cupbr_amd.scenes.million_triangles)",
-        "config": {"workload": f"PT+NEE {W}x{H}, depth {DEPTH}, 1 spp per frame with a 64->4x64->3 radiance cache "
-                               f"trained online; plain PT at equal time; reference 4096 spp",
-                   "rel_mse_cache": round(e_nrc, 5), "rel_mse_pt_equal_time": round(e_pt, 5),
-                   "pt_spp_equal_time": k, "cache_frame_ms": round(t_nrc, 3), "pt_frame_ms": round(t_pt, 3),
-                   "rel_mse_pt_1spp": round(e_pt1, 5)},
+        "config": {"workload": f"PT+NEE {W}x{H}, depth {DEPTH}, {S_NRC} spp per call with a 64->4x64->3 radiance "
+                               f"cache trained online after each call; plain PT at equal time; reference 4096 spp",
+                   "cache_spp": S_NRC, "rel_mse_cache": round(e_nrc, 5), "cache_call_ms": round(t_nrc, 3),
+                   "rel_mse_pt_equal_time": round(e_pt, 5), "pt_spp_equal_time": round(s_eq, 3),
+                   "pt_ms_fit": [round(float(a_fit), 3), round(float(b_fit), 3)], "pt_rel_mse_per_spp": round(c_fit, 5),
+                   "pt_measured": {str(k_): [round(t_, 3), round(e_, 5)] for k_, t_, e_ in zip(pt_spp, pt_ms, pt_err)},
+                   "rel_mse_pt_rounded_spp": [k, round(e_pt_k, 5)]},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 5), "traffic": None, "kernel": "k_nrc_infer<false>",
                      "avg_launch_ms": round(inf_ms, 4), "queries_per_launch": int(x.shape[0])},

@@ -404,8 +404,8 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     // ~6.4 GB). A trace launch ends with the last 64-ray chunks of its slowest waves, and with
     // more paths per launch that tail is a smaller share of it (1080p, 64 spp: 3 frames per batch
     // 475 Msamples/s, 9 frames 518, 17 frames 533, all 64 frames 543). Bounded to 64M paths and
-    // to an eighth of the device memory; with a radiance cache one frame per batch (the cache
-    // trains between frames).
+    // to an eighth of the device memory. With a radiance cache the same holds: a call of k frames
+    // is one batch, and the cache trains after each batch.
     const uint32_t target = 1u << 25;
     uint32_t cap = 1u << 26;
     {
@@ -416,7 +416,6 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     uint32_t batch = std::max<uint32_t>(1, (target + npix - 1) / npix);
     batch = std::min(batch, std::max<uint32_t>(1, cap / npix));
     batch = std::min(batch, spp);
-    if (nrc) batch = 1;
     const int blocks = device_cu_count() * 8;
     const uint32_t grid_threads = (uint32_t)blocks * kPtBlock;
     int rc = ensure_wavefront(r, batch * npix, grid_threads);

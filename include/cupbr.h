@@ -367,8 +367,10 @@ int cupbr_nrc_train_device(cupbr_nrc* cache, const float* rows, uint32_t n_rows,
  * `training_paths` pixels (chosen by a hash of pixel and frame) as full PT paths whose vertices
  * become training rows (target = the path's radiance after the vertex divided by its throughput
  * there), and ends every other path at its first diffuse or glossy vertex after the camera hit
- * with throughput * cache prediction. After each frame the cache takes `train_steps` Adam steps of
- * `batch` rows (a multiple of 512) at learning rate lr. Images stay the running mean of frames. */
+ * with throughput * cache prediction. The frames of one cupbr_render call are traced together
+ * (as one batch, up to the renderer's path bound) with the cache as it was at the start of the
+ * call; after each batch the cache takes `train_steps` Adam steps of `batch` rows (a multiple of
+ * 512) at learning rate lr. Images stay the running mean of frames. */
 int cupbr_renderer_set_radiance_cache(cupbr_renderer* r, cupbr_nrc* cache, uint32_t training_paths,
                                       uint32_t train_steps, uint32_t batch, float learning_rate);
 /* self-check of the matrix-core fragment layout: d[16][16] = a[16][32] b[32][16] (host, row-major,
