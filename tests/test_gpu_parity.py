@@ -137,6 +137,19 @@ def test_pathtracer_same_seed_parity(rr):
     assert e <= 1e-6
 
 
+@pytest.mark.parametrize("w,h", [(1, 1), (3, 5), (8, 8), (13, 7)])
+def test_pathtracer_tiny_images(w, h):
+    """Queues of 1 to 91 rays: most of the trace kernel's 8 work groups are empty and the last
+    64-ray chunk is partial; the image must still match the same-seed oracle."""
+    desc, ca = scenes.cornell_box()
+    cam = _camera(ca, w, h)
+    r = _render(cb.Scene.from_desc(desc), cam, w, h, 8)
+    img = r.get_output_image()
+    ref = oracle.OracleScene(desc).render(cam.as_dict(), w, h, 0, 8, depth=5, precision=32)
+    assert img.shape == (h, w, 3) and np.isfinite(img).all()
+    assert rel_mse(img, ref) <= 1e-6
+
+
 def test_pathtracer_mixed_scene_with_environment():
     desc, ca = scenes.mixed_primitives(env=True)
     w = h = 48
