@@ -331,9 +331,13 @@ int cupbr_renderer_destroy(cupbr_renderer* r) {
     for (auto& im : r->images) im.release();
     if (r->ev_start) (void)hipEventDestroy(r->ev_start);
     if (r->ev_stop) (void)hipEventDestroy(r->ev_stop);
-    hipStream_t st = r->stream;
+    if (r->side_stream) (void)hipStreamSynchronize(r->side_stream);
+    if (r->ev_queries) (void)hipEventDestroy(r->ev_queries);
+    if (r->ev_side_done) (void)hipEventDestroy(r->ev_side_done);
+    hipStream_t st = r->stream, side = r->side_stream;
     delete r;
     if (st) (void)hipStreamDestroy(st);
+    if (side) (void)hipStreamDestroy(side);
     return CUPBR_OK;
 }
 

@@ -36,10 +36,11 @@ int nrc_set_weights(cupbr_nrc* c, const float* host);
 int nrc_init_weights(cupbr_nrc* c, uint32_t seed);
 int nrc_pack(cupbr_nrc* c, hipStream_t stream);
 // predictions for n records of `stride` floats (inputs first); with n_dev, min(n, *n_dev) records
-// (a count on the device); accumulate: out[4 path + c] += record[16 + c] * max(pred_c, 0), path =
-// bits of record[19]; else out[3 q + c] = pred_c
+// (a count on the device), and with begin_dev only the records from *begin_dev on; accumulate:
+// out[4 path + c] += record[16 + c] * max(pred_c, 0), path = bits of record[19]; else
+// out[3 q + c] = pred_c
 int nrc_infer(cupbr_nrc* c, const float* rec, uint32_t n, const uint32_t* n_dev, uint32_t stride, float* out,
-              bool accumulate, hipStream_t stream);
+              bool accumulate, hipStream_t stream, const uint32_t* begin_dev = nullptr);
 // D = A B (16x32 by 32x16) through one matrix-core instruction with the cache's fragment layout
 int nrc_mfma_product(const float* a, const float* b, float* d);
 // `steps` Adam steps on batches of `batch` rows drawn from rows[0, n_rows) (stride kNrcRowStride)

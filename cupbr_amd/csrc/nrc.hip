@@ -120,16 +120,18 @@ template <bool kAccumulate>
 __global__ __launch_bounds__(kNrcBlock) void k_nrc_infer(const bf16x8* __restrict__ packed, NrcBox box,
                                                          const float* __restrict__ rec, uint32_t n,
                                                          const uint32_t* __restrict__ n_dev, uint32_t stride,
-                                                         float* __restrict__ out) {
+                                                         float* __restrict__ out,
+                                                         const uint32_t* __restrict__ begin_dev) {
     __shared__ bf16x8 w[kFwdSize / 8];
     load_weights_lds(w, packed, kFwdSize / 8);
     if (n_dev) n = min(n, *n_dev);  // count written by an earlier kernel on the stream
+    const uint32_t begin = begin_dev ? min(*begin_dev, n) : 0u;
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t tiles = (n + 15) / 16;
+    const uint32_t tiles = (n - begin + 15) / 16;
     for (uint32_t t = wave; t < tiles; t += n_waves) {
-        const uint32_t q = t * 16 + (lane & 15);
+        const uint32_t q = begin + t * 16 + (lane & 15);
         float raw[16];
         if (q < n) {
             const float4* r4 = reinterpret_cast<const float4*>(rec + (size_t)q * stride);
@@ -432,16 +434,16 @@ int nrc_create(const float lo[3], const float hi[3], uint32_t seed, cupbr_nrc** 
 }
 
 int nrc_infer(cupbr_nrc* c, const float* rec, uint32_t n, const uint32_t* n_dev, uint32_t stride, float* out,
-              bool accumulate, hipStream_t stream) {
+              bool accumulate, hipStream_t stream, const uint32_t* begin_dev) {
     if (n == 0) return CUPBR_OK;
     const uint32_t tiles = (n + 15) / 16;
     const int blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>((tiles + 3) / 4, (uint32_t)device_cu_count() * 4));
     if (accumulate)
         hipLaunchKernelGGL(k_nrc_infer<true>, dim3(blocks), dim3(kNrcBlock), 0, stream, c->packed_fwd.as<bf16x8>(),
-                           c->box, rec, n, n_dev, stride, out);
+                           c->box, rec, n, n_dev, stride, out, begin_dev);
     else
         hipLaunchKernelGGL(k_nrc_infer<false>, dim3(blocks), dim3(kNrcBlock), 0, stream, c->packed_fwd.as<bf16x8>(),
-                           c->box, rec, n, n_dev, stride, out);
+                           c->box, rec, n, n_dev, stride, out, begin_dev);
     CUPBR_HIP_CHECK(hipGetLastError());
     return CUPBR_OK;
 }

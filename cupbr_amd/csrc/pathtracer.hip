@@ -252,6 +252,9 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
     }
 }
 
+// query count after the second bounce's shade: the queries inferred on the side stream
+__global__ void k_snapshot_queries(Counters* c) { c->query_early.v = c->query_count.v; }
+
 // training rows from the pending rows of finished training paths: target = (final radiance -
 // radiance before the vertex's reflection) / throughput at the vertex (0 where the throughput is 0)
 __global__ __launch_bounds__(kPtBlock) void k_nrc_rows(const float4* __restrict__ pending, const Counters* c,
@@ -437,12 +440,19 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
             (rc = r->nrc_rows.ensure((size_t)r->nrc_max_rows * 80)))
             return rc;
         ns = NrcShade{r->nrc_queries.as<float4>(), r->nrc_pending.as<float4>(), r->nrc_max_rows};
+        if (!r->side_stream) {
+            if (hipStreamCreateWithFlags(&r->side_stream, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&r->ev_queries, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&r->ev_side_done, hipEventDisableTiming) != hipSuccess)
+                return set_error(CUPBR_ERR_HIP, "could not create the radiance cache's stream");
+        }
         const double frac = std::min(1.0, (double)r->nrc_train_paths / (double)npix);
         train_thresh = frac >= 1.0 ? 0xFFFFFFFFu : (uint32_t)(frac * 4294967296.0);
     }
     if ((rc = stage_mark(r, -1))) return rc;
     for (uint32_t s0 = 0; s0 < spp; s0 += batch) {
         const uint32_t nb = std::min(batch, spp - s0);
+        bool side_pending = false;  // cache inference of this batch running on the side stream
         const uint32_t npath = nb * npix;
         hipLaunchKernelGGL(k_pt_raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
                            h, frame0 + s0, npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
@@ -464,12 +474,33 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
                                wf.s_o[cur].as<float4>(), wf.s_d[cur].as<float4>(), wf.s_c[cur].as<float4>(),
                                wf.weight.as<float4>(), wf.radiance.as<float4>(), c, ns);
             if ((rc = stage_mark(r, 1))) return rc;
+            if (nrc && b == 1 && bounces > 2) {
+                // The paths that queried the cache so far have no shadow ray left in flight (the
+                // shadow rays of bounce 0 were traced before this shade, and a query vertex
+                // traces none), and no later kernel touches their radiance until the final pass
+                // below: their inference runs on the side stream while the remaining bounces
+                // (mostly training paths) are traced.
+                hipLaunchKernelGGL(k_snapshot_queries, dim3(1), dim3(1), 0, r->stream, c);
+                CUPBR_HIP_CHECK(hipEventRecord(r->ev_queries, r->stream));
+                CUPBR_HIP_CHECK(hipStreamWaitEvent(r->side_stream, r->ev_queries, 0));
+                if ((rc = nrc_infer(r->nrc, r->nrc_queries.as<float>(), npath, &c->query_early.v, 20,
+                                    wf.radiance.as<float>(), true, r->side_stream)))
+                    return rc;
+                CUPBR_HIP_CHECK(hipEventRecord(r->ev_side_done, r->side_stream));
+                side_pending = true;
+            }
         }
         if (nrc) {
-            // throughput * cache for the paths that ended at a cache vertex, then the training rows
+            // throughput * cache for the paths that ended at a cache vertex (those not already
+            // inferred on the side stream), then the training rows
             if ((rc = nrc_infer(r->nrc, r->nrc_queries.as<float>(), npath, &c->query_count.v, 20,
-                                wf.radiance.as<float>(), true, r->stream)))
+                                wf.radiance.as<float>(), true, r->stream,
+                                side_pending ? &c->query_early.v : nullptr)))
                 return rc;
+            if (side_pending) {
+                CUPBR_HIP_CHECK(hipStreamWaitEvent(r->stream, r->ev_side_done, 0));
+                side_pending = false;
+            }
             hipLaunchKernelGGL(k_nrc_rows, dim3((r->nrc_max_rows + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0,
                                r->stream, r->nrc_pending.as<float4>(), c, r->nrc_max_rows, wf.radiance.as<float4>(),
                                r->nrc_rows.as<float4>());

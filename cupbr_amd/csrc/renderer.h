@@ -25,6 +25,7 @@ struct Counters {
     CounterLine shadow_count[2]; // shadow queues (bounce parity)
     CounterLine work[8];         // work counters of the trace kernel, one per XCD group
     CounterLine query_count;  // radiance-cache queries of the batch
+    CounterLine query_early;  // queries that existed after the second bounce's shade
     CounterLine row_count;    // radiance-cache training rows of the batch
     unsigned long long extension_rays;
     unsigned long long shadow_rays;
@@ -94,6 +95,9 @@ struct cupbr_renderer {
     float nrc_lr = 1e-3f;
     uint32_t nrc_max_rows = 1u << 18;     // training rows kept per frame
     cupbr::DevBuf nrc_queries, nrc_pending, nrc_rows;
+    // second stream for cache inference that overlaps the training paths' later bounces
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_queries = nullptr, ev_side_done = nullptr;
     uint64_t nrc_queries_total = 0, nrc_rows_total = 0;  // of the last cupbr_render call
 };
 
