@@ -139,7 +139,7 @@ def test_torch_autograd_wrapper():
     np.testing.assert_allclose(albedo.grad.cpu().numpy(), ga, rtol=1e-4, atol=1e-4 * np.abs(ga).max())
 
     # a few projected gradient steps on the density lower the loss (inverse rendering)
-    losses = [float(loss)]
+    losses = [float(loss.detach())]
     with torch.no_grad():
         step = 0.02 / float(density.grad.abs().max())
     for _ in range(4):
@@ -150,7 +150,7 @@ def test_torch_autograd_wrapper():
         albedo.grad = None
         loss = ((render(density, albedo) - target) ** 2).sum()
         loss.backward()
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     print("inverse rendering loss:", ", ".join(f"{v:.4f}" for v in losses), flush=True)
     assert losses[-1] < losses[0]
 
