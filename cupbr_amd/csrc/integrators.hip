@@ -97,6 +97,26 @@ __device__ __forceinline__ void add_ray_counts(Counters* c, uint32_t ext, uint32
 }
 
 // ------------------------------------------------------------------ volume
+// coefficients of the current medium: the scene's (obj < 0) or an object's interior
+struct VolCoef {
+    v3 sa, ss;
+    float g;
+};
+__device__ __forceinline__ VolCoef vol_coef(const DevScene& S, int32_t obj) {
+    VolCoef c;
+    if (obj < 0) {
+        c.sa = ld3(S.volume.sigma_a);
+        c.ss = ld3(S.volume.sigma_s);
+        c.g = S.volume.g;
+    } else {
+        const cupbr_volume& v = S.mats[obj].volume;
+        c.sa = ld3(v.sigma_a);
+        c.ss = ld3(v.sigma_s);
+        c.g = v.g;
+    }
+    return c;
+}
+
 template <class TR>
 __device__ v3 volume_path(TR& T, const cupbr_camera& cam, uint32_t tid, uint32_t w, uint32_t h, uint32_t frame,
                           uint32_t max_depth, bool rr) {
@@ -109,7 +129,7 @@ __device__ v3 volume_path(TR& T, const cupbr_camera& cam, uint32_t tid, uint32_t
     bool inside = false;
     uint32_t obj_index = 0;
     const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
-    cupbr_volume vol = S.volume;
+    int32_t vol_obj = -1;  // current medium: -1 the scene's, else the interior of that object
     uint32_t depth = 0;
     do {
         bool next_valid = false;
@@ -126,12 +146,13 @@ __device__ v3 volume_path(TR& T, const cupbr_camera& cam, uint32_t tid, uint32_t
         hit_geometry(S, hr.prim, P, N, obj);
         const cupbr_material* mp = S.mats + obj;
         const Mat m = load_mat(mp);
-        if (inside) vol = mp->volume;  // the payload's volume pointer follows the current hit (:277)
+        if (inside) vol_obj = (int32_t)obj;  // the payload's volume pointer follows the current hit (:277)
         // handleMediumInteraction :88-212
         bool medium_event = false;
         {
-            const float g = vol.g;
-            const v3 sa = ld3(vol.sigma_a), ss = ld3(vol.sigma_s), st = sa + ss;
+            const VolCoef vc = vol_coef(S, vol_obj);
+            const float g = vc.g;
+            const v3 sa = vc.sa, ss = vc.ss, st = sa + ss;
             uint32_t ch = (uint32_t)(rnd(seed) * 3.0f);
             ch = min(ch, 2u);
             const float stc = comp(st, (int)ch);
@@ -194,7 +215,7 @@ __device__ v3 volume_path(TR& T, const cupbr_camera& cam, uint32_t tid, uint32_t
                 inside = refl ? inside : !inside;
                 obj_index = obj;
                 next_valid = true;
-                vol = inside ? mp->volume : S.volume;
+                vol_obj = inside ? (int32_t)obj : -1;
             } else {
                 if (m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f) {  // directIlluminationVolumetric :24-70
                     weight = splat(0.0f);
@@ -202,10 +223,12 @@ __device__ v3 volume_path(TR& T, const cupbr_camera& cam, uint32_t tid, uint32_t
                     v3 ldir, le;
                     float dist;
                     if (nee_sample(S, m, seed, P, inc, N, ldir, le, dist)) {
-                        if (T.visible(P + 0.001f * ldir, normalize(ldir), dist))
+                        if (T.visible(P + 0.001f * ldir, normalize(ldir), dist)) {
+                            const VolCoef vc = vol_coef(S, vol_obj);
                             radiance += (nl * fmaxf(0.0f, dot(N, ldir))) *
-                                        vexp(-1.0f * (ld3(vol.sigma_s) + ld3(vol.sigma_a)) * fminf(dist, 100000000.0f)) *
+                                        vexp(-1.0f * (vc.ss + vc.sa) * fminf(dist, 100000000.0f)) *
                                         brdf(m, inc, ldir, N) * le * weight;
+                        }
                     }
                 }
                 const DirPdf s = sample_direction(m, seed, inc, N);  // indirectIlluminationVolumetric :72-86
