@@ -81,11 +81,33 @@ __device__ __forceinline__ void derive(const NrcBox& box, const float* raw, floa
 }
 
 // B operand of the first layer for lane group g: slot (kc, j) holds feature perm(kc, 8g + j)
+// Slots j and j + 1 (j even) hold features i and i + 1 with i even: below 52 these are the sine
+// and cosine of the same argument 2^k pi x, computed together by sincospif on the exact 2^k x.
 __device__ __forceinline__ void encode(const NrcBox& box, const float* raw, int g, bf16x8 b[2]) {
     float d[14];
     derive(box, raw, d);
     for (int kc = 0; kc < 2; ++kc)
-        for (int j = 0; j < 8; ++j) b[kc][j] = to_bf16(feature((2 * kc + (j >> 2)) * 16 + g * 4 + (j & 3), d));
+        for (int j = 0; j < 8; j += 2) {
+            const int i = (2 * kc + (j >> 2)) * 16 + g * 4 + (j & 3);
+            float f0, f1;
+            if (i < 52) {
+                int a, k;
+                if (i < 36) {
+                    a = i / 12;
+                    k = (i % 12) >> 1;
+                } else {
+                    const int r = (i - 36) & 7;
+                    a = (i < 44 ? 3 : 5) + (r >> 2);
+                    k = (r & 3) >> 1;
+                }
+                sincospif((float)(1 << k) * d[a], &f0, &f1);
+            } else {
+                f0 = feature(i, d);
+                f1 = feature(i + 1, d);
+            }
+            b[kc][j] = to_bf16(f0);
+            b[kc][j + 1] = to_bf16(f1);
+        }
 }
 
 // one hidden layer: out = relu(W x) as the next B operand; w = packed fragments [ot][kc][lane][8]

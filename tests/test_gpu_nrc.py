@@ -43,8 +43,8 @@ def encode(raw, lo, hi):
         else:
             r = (i - 36) & 7
             a, k, sc = (3 if i < 44 else 5) + (r >> 2), (r & 3) >> 1, r & 1
-        arg = float(2 ** k) * PI * d[:, a]
-        f[:, i] = torch.cos(arg) if sc else torch.sin(arg)
+        arg = (float(2 ** k) * d[:, a]).double() * math.pi  # sin/cos(2^k pi x), evaluated in FP64
+        f[:, i] = (torch.cos(arg) if sc else torch.sin(arg)).float()
     f[:, 52] = 1.0 - torch.exp(-d[:, 7])
     f[:, 53:59] = d[:, 8:14]
     f[:, 59] = 1.0
@@ -101,7 +101,7 @@ def test_inference_matches_torch_reference():
     e_bf = np.linalg.norm(out - ref_bf) / np.linalg.norm(ref_bf)
     e_32 = np.linalg.norm(out - ref_32) / np.linalg.norm(ref_32)
     print(f"inference: rel-L2 vs BF16-emulating reference {e_bf:.2e}, vs FP32 reference {e_32:.2e}", flush=True)
-    assert e_bf <= 1e-2 and e_32 <= 5e-2
+    assert e_bf <= 1e-3 and e_32 <= 5e-2
 
 
 def _hash32(x):
@@ -139,9 +139,9 @@ def test_training_step_matches_torch_gradient():
     big = np.abs(g) > 1e-2 * np.abs(g).max()
     agree = np.mean(np.sign(step[big]) == -np.sign(g[big]))
     lr_ok = np.mean(np.abs(np.abs(step[big]) - lr) <= 0.05 * lr)
-    print(f"train step: loss {float(loss[0]):.4f} vs reference {float(l_ref):.4f}; sign agreement {agree:.4f} "
+    print(f"train step: loss {float(loss[0]):.4f} vs reference {float(l_ref.detach()):.4f}; sign agreement {agree:.4f} "
           f"over {big.sum()} weights; |step| = lr for {lr_ok:.4f}", flush=True)
-    assert abs(float(loss[0]) - float(l_ref)) <= 1e-2 * float(l_ref)
+    assert abs(float(loss[0]) - float(l_ref.detach())) <= 1e-2 * float(l_ref.detach())
     assert agree >= 0.98 and lr_ok >= 0.98
 
 
