@@ -225,7 +225,9 @@ def main():
 
 def bench_grid(args, log):
     """One step = forward render of the 64^3 medium at 256x256, 64 spp, and the path-replay adjoint of
-    L = sum (I - T)^2 against a target rendered from the perturbed density (untimed)."""
+    L = sum (I - T)^2 against a target rendered from the perturbed density (untimed). The adjoint
+    takes each path's radiance from the forward render (cupbr_grid_adjoint_from_forward_device), as
+    the torch.autograd wrapper does, so every path is replayed once in it."""
     import numpy as np
     import torch
 
@@ -247,13 +249,13 @@ def bench_grid(args, log):
 
     def step():
         img = R.render(rho, alb)
-        return R.adjoint(rho, alb, 2 * (img - target))
+        return R.adjoint(rho, alb, 2 * (img - target), from_forward=True)
 
     R.set_counting(True)
     step()
     cnt = R.counts()
     R.set_counting(False)
-    # per adjoint launch: the replayed paths read the grid in the forward pass and the replay
+    # per adjoint launch: the grid reads of the replay (the step's reads minus the forward render's)
     R.set_counting(True)
     img = R.render(rho, alb)
     fwd = R.counts()
@@ -271,7 +273,7 @@ def bench_grid(args, log):
         img = R.render(rho, alb)
         d = 2 * (img - target)
         ev[k][0].record()
-        R.adjoint(rho, alb, d)
+        R.adjoint(rho, alb, d, from_forward=True)
         ev[k][1].record()
     torch.cuda.synchronize()
     t_total = time.perf_counter() - t_start

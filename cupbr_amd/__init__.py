@@ -222,6 +222,7 @@ def lib():
             "cupbr_grid_get_counts": (C.c_int, [P, P]),
             "cupbr_grid_render_device": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P]),
             "cupbr_grid_adjoint_device": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P, P, P]),
+            "cupbr_grid_adjoint_from_forward_device": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P, P, P]),
             "cupbr_grid_render": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P]),
             "cupbr_grid_adjoint": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P, P]),
         }
@@ -618,10 +619,16 @@ class GridRenderer:
         _check(lib().cupbr_grid_render(*self._args(), _ptr(rho), _ptr(prop), _ptr(alb), *self._tail(), _ptr(img)))
         return img
 
-    def adjoint(self, density, albedo, d_image, proposal=None, grad_density=None, grad_albedo=None):
+    def adjoint(self, density, albedo, d_image, proposal=None, grad_density=None, grad_albedo=None,
+                from_forward=False):
         """dLoss/d(density), dLoss/d(albedo) for the image cotangent d_image. With torch GPU tensors
-        the gradients are added to grad_density / grad_albedo when given (else new zero tensors)."""
+        the gradients are added to grad_density / grad_albedo when given (else new zero tensors).
+        from_forward (torch GPU tensors only): take the paths' radiance from the preceding render
+        call with the same tensors (cupbr_grid_adjoint_from_forward_device); CupbrError if there
+        is none."""
         self._check_sizes(density, albedo, proposal, d_image)
+        if from_forward and not _is_torch_gpu(density):
+            raise ValueError("from_forward needs torch GPU tensors (the forward's samples stay on the device)")
         if _is_torch_gpu(density):
             import torch
             rho = density.detach().contiguous().float()
@@ -636,9 +643,9 @@ class GridRenderer:
             gr = torch.zeros_like(rho) if grad_density is None else grad_density
             ga = torch.zeros_like(alb) if grad_albedo is None else grad_albedo
             stream = torch.cuda.current_stream(rho.device).cuda_stream
-            _check(lib().cupbr_grid_adjoint_device(*self._args(), rho.data_ptr(),
-                                                   None if prop is None else prop.data_ptr(), alb.data_ptr(),
-                                                   *self._tail(), di.data_ptr(), gr.data_ptr(), ga.data_ptr(), stream))
+            fn = lib().cupbr_grid_adjoint_from_forward_device if from_forward else lib().cupbr_grid_adjoint_device
+            _check(fn(*self._args(), rho.data_ptr(), None if prop is None else prop.data_ptr(), alb.data_ptr(),
+                      *self._tail(), di.data_ptr(), gr.data_ptr(), ga.data_ptr(), stream))
             return gr, ga
         rho = np.ascontiguousarray(density, np.float32)
         alb = np.ascontiguousarray(albedo, np.float32)

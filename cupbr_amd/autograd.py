@@ -3,7 +3,7 @@
     render = GridRender(scene, camera, medium, width, height, spp=16, max_depth=6)
     image = render(density, albedo)           # density [nz, ny, nx], albedo [n_surfaces, 3], CUDA
     loss = ((image - target) ** 2).sum()
-    loss.backward()                            # density.grad, albedo.grad from cupbr_grid_adjoint_device
+    loss.backward()                            # density.grad, albedo.grad from the path-replay adjoint
 
 The forward and backward run the HIP kernels of libcupbr on torch's current stream; nothing
 is computed by torch except the loss. The proposal grid that drives the real/null collision
@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import torch
 
-from . import Camera, GridMedium, GridRenderer, Scene
+from . import Camera, CupbrError, GridMedium, GridRenderer, Scene
 
 
 class _GridRenderFn(torch.autograd.Function):
@@ -29,7 +29,13 @@ class _GridRenderFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_image):
         density, albedo, prop = ctx.saved_tensors
-        gd, ga = ctx.renderer.adjoint(density, albedo, grad_image, proposal=prop)
+        # The saved tensors cannot have changed since the forward (autograd checks their versions),
+        # so the adjoint may reuse the forward pass's per-path radiance when that render was the
+        # last one on the scene; otherwise every path is replayed once more.
+        try:
+            gd, ga = ctx.renderer.adjoint(density, albedo, grad_image, proposal=prop, from_forward=True)
+        except CupbrError:
+            gd, ga = ctx.renderer.adjoint(density, albedo, grad_image, proposal=prop)
         return gd, ga, None, None
 
 

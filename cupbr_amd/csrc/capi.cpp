@@ -770,6 +770,17 @@ int cupbr_grid_adjoint_device(const cupbr_scene* s, const cupbr_camera* cam, con
                                grad_density, grad_albedo, (hipStream_t)stream);
 }
 
+int cupbr_grid_adjoint_from_forward_device(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m,
+                                           const float* density, const float* proposal, const float* albedo,
+                                           uint32_t w, uint32_t h, uint32_t frame0, uint32_t spp, uint32_t max_depth,
+                                           const float* d_image, float* grad_density, float* grad_albedo,
+                                           void* stream) {
+    if (int rc = grid_validate(s, cam, m, w, h, spp, max_depth)) return rc;
+    CHECK_ARG(density && albedo && d_image && grad_density && grad_albedo, "null buffer");
+    return grid_adjoint_device(s, cam, m, density, proposal, albedo, w, h, frame0, spp, max_depth, d_image,
+                               grad_density, grad_albedo, (hipStream_t)stream, true);
+}
+
 namespace {
 struct GridHostBufs {
     DevBuf rho, prop, alb;

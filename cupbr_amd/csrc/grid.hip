@@ -34,6 +34,7 @@
 //   device atomics. A factor that is exactly zero ends the path, so the derivative through
 //   that factor is dropped (a zero-measure event for densities below the majorant).
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 
 #include "device.h"
@@ -347,7 +348,8 @@ __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupb
                                                      uint32_t max_depth, float* __restrict__ samples,
                                                      const float* __restrict__ d_image, float* grad_rho,
                                                      float* grad_alb, uint32_t n_alb, size_t copy_stride,
-                                                     uint32_t* spill, unsigned long long* counts) {
+                                                     uint32_t* spill, unsigned long long* counts,
+                                                     const float* __restrict__ fwd_samples) {
     __shared__ uint32_t lds[kGridLds * kGridBlock];
     // per-block albedo gradients (few surfaces, hit by every path): LDS atomics, one global
     // atomic per entry at the end
@@ -369,7 +371,10 @@ __global__ __launch_bounds__(kGridBlock) void k_grid(DevScene S, GridDev G, cupb
         const uint32_t p = (uint32_t)(i % npix);
         const uint32_t s = s_begin + (uint32_t)(i / npix);
         const uint32_t px = p % w, py = p / w;
-        const v3 L = grid_path<false>(S, G, T, cam, px, py, w, h, frame0 + s, max_depth, nullptr);
+        // the adjoint needs the path's radiance: from the forward pass's samples when given
+        // (stored at the same index, sample-major), else by running the path once more
+        const v3 L = (kAdj && fwd_samples) ? ld3(fwd_samples + 3 * i)
+                                           : grid_path<false>(S, G, T, cam, px, py, w, h, frame0 + s, max_depth, nullptr);
         if (!kAdj) {
             st3(samples + 3 * i, L);  // sample-major: samples[(s - s_begin) * npix + p]
         } else {
@@ -518,20 +523,47 @@ int grid_render_device(const cupbr_scene* cs, const cupbr_camera* cam, const cup
         if (grid_launch_shape(s, (size_t)n_px * cnt, &blocks, &spill)) return CUPBR_ERR_OUT_OF_MEMORY;
         hipLaunchKernelGGL(k_grid<false>, dim3(blocks), dim3(kGridBlock), 0, stream, s->dev, G, *cam, w, h, spp, frame0,
                            s0, cnt, max_depth, s->grid_samples.as<float>(), nullptr, nullptr, nullptr, 0u, (size_t)0,
-                           spill, counts);
+                           spill, counts, nullptr);
         CUPBR_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_grid_accumulate, dim3((n_px + 255) / 256), dim3(256), 0, stream, image,
                            s->grid_samples.as<float>(), n_px, cnt, s0 == 0 ? 1 : 0, s0 + cnt == spp ? 1 : 0, spp);
         CUPBR_HIP_CHECK(hipGetLastError());
     }
+    // all samples of this call stay in grid_samples when they fit one chunk: an adjoint with the
+    // same arguments may take the paths' radiance from there
+    auto& f = s->grid_forward;
+    f.valid = chunk >= spp;
+    f.rho = rho;
+    f.prop = prop;
+    f.alb = alb;
+    f.cam = *cam;
+    f.medium = *m;
+    f.w = w;
+    f.h = h;
+    f.frame0 = frame0;
+    f.spp = spp;
+    f.max_depth = max_depth;
     return CUPBR_OK;
 }
 
 int grid_adjoint_device(const cupbr_scene* cs, const cupbr_camera* cam, const cupbr_grid_medium* m, const float* rho,
                         const float* prop, const float* alb, uint32_t w, uint32_t h, uint32_t frame0, uint32_t spp,
                         uint32_t max_depth, const float* d_image, float* grad_rho, float* grad_alb,
-                        hipStream_t stream) {
+                        hipStream_t stream, bool from_forward) {
     cupbr_scene* s = const_cast<cupbr_scene*>(cs);
+    const float* fwd_samples = nullptr;
+    if (from_forward) {
+        const auto& f = s->grid_forward;
+        const bool same = f.valid && f.rho == rho && f.prop == prop && f.alb == alb && f.w == w && f.h == h &&
+                          f.frame0 == frame0 && f.spp == spp && f.max_depth == max_depth &&
+                          std::memcmp(&f.cam, cam, sizeof(cupbr_camera)) == 0 &&
+                          std::memcmp(&f.medium, m, sizeof(cupbr_grid_medium)) == 0;
+        if (!same)
+            return set_error(CUPBR_ERR_INVALID_ARGUMENT,
+                             "no preceding grid render with these arguments whose samples are kept (the last render "
+                             "had other arguments, or more than 2^24 samples)");
+        fwd_samples = s->grid_samples.as<float>();
+    }
     if (int rc = scene_upload(s)) return rc;
     const GridDev G = make_grid(s, m, rho, prop, alb);
     unsigned long long* counts = s->grid_counting ? s->grid_counts.as<unsigned long long>() : nullptr;
@@ -554,7 +586,7 @@ int grid_adjoint_device(const cupbr_scene* cs, const cupbr_camera* cam, const cu
     hipLaunchKernelGGL(k_grid<true>, dim3(blocks), dim3(kGridBlock), 0, stream, s->dev, G, *cam, w, h, spp, frame0, 0u,
                        spp, max_depth, nullptr, d_image, target, grad_alb,
                        (uint32_t)(3 * (s->objects.size() + s->triangles.size() / 9)),
-                       private_copies ? copy_floats : (size_t)0, spill, counts);
+                       private_copies ? copy_floats : (size_t)0, spill, counts, fwd_samples);
     CUPBR_HIP_CHECK(hipGetLastError());
     if (private_copies) {
         hipLaunchKernelGGL(k_grid_reduce, dim3((unsigned)std::min<size_t>((n_vox + 255) / 256, 4096)), dim3(256), 0,

@@ -40,6 +40,14 @@ struct cupbr_scene {
     cupbr::DevBuf grid_counts;         // 4 x u64 work counters of the grid-medium kernels
     cupbr::DevBuf grid_grad;           // per-XCD density-gradient copies of the adjoint
     bool grid_counting = false;
+    // arguments of the last grid forward render whose per-sample radiance is all in grid_samples
+    struct GridForward {
+        bool valid = false;
+        const float *rho = nullptr, *prop = nullptr, *alb = nullptr;
+        cupbr_camera cam{};
+        cupbr_grid_medium medium{};
+        uint32_t w = 0, h = 0, frame0 = 0, spp = 0, max_depth = 0;
+    } grid_forward;
 };
 
 namespace cupbr {
@@ -72,9 +80,11 @@ int grid_validate(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_gri
 int grid_render_device(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m, const float* rho,
                        const float* prop, const float* alb, uint32_t w, uint32_t h, uint32_t frame0, uint32_t spp,
                        uint32_t max_depth, float* image, hipStream_t stream);
+// from_forward: take each path's radiance from the preceding grid_render_device call with the same
+// arguments instead of replaying the path for it (an error when there is no such call)
 int grid_adjoint_device(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m, const float* rho,
                         const float* prop, const float* alb, uint32_t w, uint32_t h, uint32_t frame0, uint32_t spp,
                         uint32_t max_depth, const float* d_image, float* grad_rho, float* grad_alb,
-                        hipStream_t stream);
+                        hipStream_t stream, bool from_forward = false);
 
 }  // namespace cupbr

@@ -328,6 +328,18 @@ int cupbr_grid_adjoint_device(const cupbr_scene* scene, const cupbr_camera* came
                               const float* density, const float* proposal, const float* surface_albedo,
                               uint32_t width, uint32_t height, uint32_t frame0, uint32_t spp, uint32_t max_depth,
                               const float* d_image, float* grad_density, float* grad_albedo, void* hip_stream);
+/* Adjoint that takes each path's radiance from the immediately preceding
+ * cupbr_grid_render_device call on this scene instead of running every path an extra time (the
+ * path-replay pass then runs once). The arguments must equal that call's (same buffers, camera,
+ * medium, size, frame0, spp, max_depth), its width * height * spp must not exceed 2^24, the
+ * parameter values must be unchanged since it, and the work must be ordered after it (same stream
+ * or synchronised). Otherwise CUPBR_ERR_INVALID_ARGUMENT and nothing is launched. */
+int cupbr_grid_adjoint_from_forward_device(const cupbr_scene* scene, const cupbr_camera* camera,
+                                           const cupbr_grid_medium* medium, const float* density,
+                                           const float* proposal, const float* surface_albedo, uint32_t width,
+                                           uint32_t height, uint32_t frame0, uint32_t spp, uint32_t max_depth,
+                                           const float* d_image, float* grad_density, float* grad_albedo,
+                                           void* hip_stream);
 /* instrumentation: with counting on, the grid kernels add their work to four counters of the
  * scene: closest-hit rays, shadow rays, grid stencil reads (8 voxels each) and gradient atomics.
  * Enabling resets them; cupbr_grid_get_counts synchronizes the device and reads them. */

@@ -118,6 +118,32 @@ def test_grid_adjoint_against_gpu_central_differences():
     assert worst <= 2e-2
 
 
+def test_adjoint_from_forward_matches_the_full_replay():
+    """The adjoint that reuses the forward render's per-path radiance equals the one that replays
+    every path for it (up to the order of float atomics), and refuses arguments that differ from
+    the last render's."""
+    torch = pytest.importorskip("torch")
+    G, cam, R = _case(32, 48, 48)
+    rho = torch.from_numpy(G["density"]).cuda()
+    alb = torch.from_numpy(G["albedo"]).cuda()
+    target = R.render(torch.from_numpy(G["target_density"]).cuda(), alb)
+    img = R.render(rho, alb)
+    d_img = 2 * (img - target)
+    g1, a1 = R.adjoint(rho, alb, d_img, from_forward=True)
+    g0, a0 = R.adjoint(rho, alb, d_img)
+    torch.testing.assert_close(g1, g0, rtol=1e-4, atol=1e-5 * float(g0.abs().max()))
+    torch.testing.assert_close(a1, a0, rtol=1e-4, atol=1e-5 * float(a0.abs().max()))
+    with pytest.raises(cb.CupbrError):  # the last render used other densities
+        R.render(torch.from_numpy(G["target_density"]).cuda(), alb)
+        R.adjoint(rho, alb, d_img, from_forward=True)
+    R2 = cb.GridRenderer(R.scene, cam, R.medium, 48, 48, spp=4, max_depth=6)
+    R.render(rho, alb)
+    with pytest.raises(cb.CupbrError):  # other sample count
+        R2.adjoint(rho, alb, d_img, from_forward=True)
+    with pytest.raises(ValueError):  # host arrays: the forward's samples stay on the device
+        R.adjoint(G["density"], G["albedo"], d_img.cpu().numpy(), from_forward=True)
+
+
 def test_torch_autograd_wrapper():
     torch = pytest.importorskip("torch")
     from cupbr_amd.autograd import GridRender
