@@ -103,17 +103,21 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
         c->query_count.v = 0;
         c->row_count.v = 0;
     }
-    if (p >= npath) return;
+    if (p >= npath) return;  // p: queue slot
     const uint32_t npix = w * h;
-    const uint32_t s = p / npix, pix = p - s * npix;
+    const uint32_t s = p / npix;
+    uint32_t px, py;
+    tile_pixel(p - s * npix, w, h, px, py);  // the queue holds 8x8 pixel tiles
+    const uint32_t pix = py * w + px;
+    const uint32_t pid = s * npix + pix;  // path id: frame-major, pixels in row order
     uint32_t seed = tea4(pix, frame0 + s);  // PathTracer.cu:112
     v3 o, d;
-    launch_ray(pix % w, pix / w, w, h, cam, seed, o, d);
+    launch_ray(px, py, w, h, cam, seed, o, d);
     q_o[p] = make_float4(o.x, o.y, o.z, 0.0f);
-    q_d[p] = make_float4(d.x, d.y, d.z, __uint_as_float(p));
-    weight[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(seed));
+    q_d[p] = make_float4(d.x, d.y, d.z, __uint_as_float(pid));
+    weight[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(seed));
     const uint32_t flags = is_training_path(pix, frame0 + s, train_thresh) ? 1u : 0u;
-    rad[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(flags));
+    rad[pid] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(flags));
 }
 
 // radiance-cache records written by shade: a query (inputs 16, throughput 3, path) ends a rendering
