@@ -214,7 +214,7 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_pt_trace (extension closest hit + shadow any hit, BVH4)" if fused else
+                     "kernel": "k_pt_trace_camera + k_pt_trace (bounce 0: camera rays as wave packets; then per bounce extension closest hit + shadow any hit, BVH4)" if fused else
                                "k_pt_extend (closest-hit BVH4 traversal)",
                      "bytes_per_launch": int(ext_bytes_per_launch), "bytes_per_ext_ray": round(bytes_per_ext_ray, 1),
                      "bytes_per_shadow_ray": round(bytes_per_sh_ray, 1), "avg_launch_ms": round(ext_launch_ms, 4)},

@@ -355,6 +355,33 @@ void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o, const float
     }
 }
 
+// Bounce 0 (PathTracer.cu:112-126): the camera rays, traced as packets. Queue slot chunks of 64
+// are 8x8 pixel tiles whose rays leave the camera position in nearby directions, so each wave
+// runs traverse_packet over one chunk. Blocks are dispatched to the 8 XCDs round-robin; block b
+// takes chunk range (b % 8) * (gridDim.x / 8) + b / 8, so each XCD traces a contiguous eighth of
+// the queue (neighbouring tiles share its L2). gridDim.x is a multiple of 8; n = queue 0's length.
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_pt_trace_camera(DevScene S, uint32_t n, const float4* __restrict__ q_o, const float4* __restrict__ q_d,
+                       float2* __restrict__ q_hit, Counters* c) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // as k_pt_trace for cur = 0 (no shadow rays yet)
+        c->queue_count[1].v = 0;
+        c->shadow_count[0].v = 0;
+        c->extension_rays += n;
+    }
+    const uint32_t blk = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const uint32_t wave_first = blk * kPtBlock + (threadIdx.x & ~63u);
+    if (wave_first >= n) return;  // whole waves only: traverse_packet needs all 64 lanes
+    const uint32_t i = wave_first + __lane_id();
+    const bool valid = i < n;
+    float4 o4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d4 = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+    if (valid) {
+        o4 = q_o[i];
+        d4 = q_d[i];
+    }
+    TraceResult r;
+    traverse_packet(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), valid, r);
+    if (valid) q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
+}
 
 // running mean of PathTracer.cu:162-168, applied frame by frame in order
 __global__ __launch_bounds__(kPtBlock) void k_accumulate(float* __restrict__ img, const float4* __restrict__ rad,
@@ -435,6 +462,9 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     float* img = r->images[CUPBR_IMAGE_OUTPUT].as<float>();
     const uint32_t bounces = std::max<uint32_t>(1, r->max_trace_depth);  // do { } while (depth < max)
     auto trace = r->counting ? k_pt_trace<true> : k_pt_trace<false>;
+    // camera rays as wave packets (the visit-counting pass keeps the per-ray traversal, whose
+    // counts define the algorithmic bytes of a trace launch)
+    const bool packet = !r->counting && r->scene->stack_entries <= kPacketStack && S.n_nodes > 0;
     auto shade = nrc ? k_pt_shade<true> : k_pt_shade<false>;
     NrcShade ns{nullptr, nullptr, 0};
     uint32_t train_thresh = 0;
@@ -466,10 +496,16 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
         // shade (appends the next extension rays and this bounce's shadow rays, by bounce parity)
         for (uint32_t b = 0; b <= bounces; ++b) {
             const int cur = (int)(b & 1);
-            hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
-                               wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
-                               wf.s_d[cur ^ 1].as<float4>(), wf.s_c[cur ^ 1].as<float4>(), wf.radiance.as<float4>(),
-                               c, spill, grid_threads);
+            if (b == 0 && packet) {
+                const uint32_t nblk = (npath + kPtBlock - 1) / kPtBlock;
+                hipLaunchKernelGGL(k_pt_trace_camera, dim3((nblk + 7u) / 8u * 8u), dim3(kPtBlock), 0, r->stream, S,
+                                   npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(), wf.q_hit.as<float2>(), c);
+            } else {
+                hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
+                                   wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
+                                   wf.s_d[cur ^ 1].as<float4>(), wf.s_c[cur ^ 1].as<float4>(),
+                                   wf.radiance.as<float4>(), c, spill, grid_threads);
+            }
             if ((rc = stage_mark(r, 0))) return rc;
             if (b == bounces) break;  // the last trace only had the last bounce's shadow rays
             hipLaunchKernelGGL(shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,

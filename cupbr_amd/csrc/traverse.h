@@ -98,6 +98,36 @@ struct TraceResult {
     uint32_t rank;
 };
 
+// triangle or sphere test of one primitive record
+__device__ __forceinline__ bool prim_test(const PrimRec& pr, v3 o, v3 d, float& t) {
+    if ((__float_as_uint(pr.c.w) & 3u) == kPrimTriangle)
+        return mt_test(mk(pr.a.x, pr.a.y, pr.a.z), mk(pr.a.w, pr.b.x, pr.b.y), mk(pr.b.z, pr.b.w, pr.c.x), o, d, t);
+    return sphere_test(mk(pr.a.x, pr.a.y, pr.a.z), pr.a.w, o, d, t);
+}
+
+// the reference's biased test of the primitive's object box (finite boxes only: spheres, meshes)
+__device__ __forceinline__ bool object_box_ok(const DevScene& S, const PrimRec& pr, v3 o, v3 d) {
+    if (!(__float_as_uint(pr.c.w) & kPrimNeedsBoxCheck)) return true;
+    const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
+    return ref_aabb_hit(ob.bmin, ob.bmax, o, d);
+}
+
+// closest-hit step of one primitive (leaf-order index idx): best <- (t, idx, rank) when the
+// primitive is hit at (t, rank) lexicographically below best and passes its object-box test
+// (tested last, only for a candidate that would win)
+__device__ __forceinline__ void closest_prim(const DevScene& S, const PrimRec& pr, uint32_t idx, v3 o, v3 d,
+                                             TraceResult& best) {
+    float t;
+    if (!prim_test(pr, o, d, t)) return;
+    const uint32_t rank = __float_as_uint(pr.c.y);
+    if (t < best.t || (t == best.t && rank < best.rank)) {
+        if (!object_box_ok(S, pr, o, d)) return;
+        best.t = t;
+        best.prim = idx;
+        best.rank = rank;
+    }
+}
+
 // Per-lane stack: entries [0, kLds) in LDS at lds[i * lds_stride], the rest in global
 // memory at spill[(i - kLds) * spill_stride].
 template <int kLds>
@@ -167,32 +197,12 @@ __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float di
         if (kCount) cnt->prims += n_leaf;
         for (uint32_t k = 0; k < n_leaf; ++k) {
             const PrimRec pr = S.prims[first + k];
-            const uint32_t flags = __float_as_uint(pr.c.w);
-            float t;
-            bool h;
-            if ((flags & 3u) == kPrimTriangle)
-                h = mt_test(mk(pr.a.x, pr.a.y, pr.a.z), mk(pr.a.w, pr.b.x, pr.b.y), mk(pr.b.z, pr.b.w, pr.c.x), o, d, t);
-            else
-                h = sphere_test(mk(pr.a.x, pr.a.y, pr.a.z), pr.a.w, o, d, t);
-            if (!h) continue;
             if (kAny) {
-                if (!(t <= dist)) continue;
-                if (flags & kPrimNeedsBoxCheck) {
-                    const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
-                    if (!ref_aabb_hit(ob.bmin, ob.bmax, o, d)) continue;
-                }
+                float t;
+                if (!prim_test(pr, o, d, t) || !(t <= dist) || !object_box_ok(S, pr, o, d)) continue;
                 return true;
             }
-            const uint32_t rank = __float_as_uint(pr.c.y);
-            if (t < best.t || (t == best.t && rank < best.rank)) {
-                if (flags & kPrimNeedsBoxCheck) {
-                    const ObjInfo& ob = S.objs[__float_as_uint(pr.c.z)];
-                    if (!ref_aabb_hit(ob.bmin, ob.bmax, o, d)) continue;
-                }
-                best.t = t;
-                best.prim = first + k;
-                best.rank = rank;
-            }
+            closest_prim(S, pr, first + k, o, d, best);
         }
         return false;
     };
@@ -251,6 +261,142 @@ __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float di
     return false;
 }
 #undef CUPBR_CSWAP
+
+// a load whose address is the same in every lane, issued as a scalar load (records of the
+// BVH and primitive arrays, which no kernel writes while it traverses them)
+__device__ __forceinline__ float4 uniform_load4(const void* p) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = *(const __attribute__((address_space(4))) f4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ PrimRec uniform_prim(const PrimRec* p) {
+    return PrimRec{uniform_load4(&p->a), uniform_load4(&p->b), uniform_load4(&p->c)};
+}
+
+__device__ __forceinline__ float lane_float(float v, uint32_t lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+
+// Entries of the wave-uniform stack of traverse_packet: entry k is held by lane k % 64 of
+// register k / 64.
+constexpr uint32_t kPacketStack = 128;
+
+// Closest hit for the 64 rays of a wave traversed as one packet (rays with a common origin and
+// nearby directions, such as the camera rays of an 8x8 pixel tile). Every node and leaf is
+// visited by the whole wave: its record is a wave-uniform scalar load, each lane tests its own
+// ray, and a child is entered when its box overlaps the current interval of any lane (the same
+// conservative test as traverse), nearest first by the entry distance of the first such lane.
+// Every lane therefore tests a superset of the primitives its own traversal would test, none
+// outside the object boxes' candidates, and the closest hit, a minimum of (t, reference rank)
+// over them, is the same. Lanes with valid = false take part in the traversal (the stack is
+// one value per wave, spread over the lanes) but have an empty interval and keep no hit.
+// Requires the BVH's worst-case stack (3 entries per BVH4 level + 1) <= kPacketStack and the
+// whole wave active. kUniformSigns: every valid lane's direction has the signs (ux, uy, uz), so
+// the near and far planes of a box are chosen once per wave.
+template <bool kUniformSigns>
+__device__ __forceinline__ void traverse_packet_impl(const DevScene& S, v3 o, v3 d, bool valid, bool ux, bool uy,
+                                                     bool uz, TraceResult& best) {
+    const float inf = __builtin_inff();
+    best.t = valid ? inf : -1.0f;
+    best.prim = kNoHit;
+    best.rank = 0xFFFFFFFFu;
+    for (uint32_t i = 0; i < S.n_unb; ++i) {  // planes and quads, as in traverse
+        const UnbRec u = S.unb[i];
+        float t;
+        if (!valid || !plane_quad_test(u, o, d, t)) continue;
+        const uint32_t rank = __float_as_uint(u.n.w);
+        if (t < best.t || (t == best.t && rank < best.rank)) {
+            best.t = t;
+            best.prim = kUnbBit | i;
+            best.rank = rank;
+        }
+    }
+    if (S.n_nodes == 0) return;
+
+    const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+    const bool sx = kUniformSigns ? ux : ix >= 0.0f, sy = kUniformSigns ? uy : iy >= 0.0f,
+               sz = kUniformSigns ? uz : iz >= 0.0f;
+    const float tlo = -1e-5f;
+    // float4 offsets of the near and far planes in a node (lo x, y, z at 0, 1, 2; hi at 3, 4, 5)
+    const uint32_t onx = __builtin_amdgcn_readfirstlane(sx ? 0 : 3), ofx = 3 - onx;
+    const uint32_t ony = __builtin_amdgcn_readfirstlane(sy ? 1 : 4), ofy = 5 - ony;
+    const uint32_t onz = __builtin_amdgcn_readfirstlane(sz ? 2 : 5), ofz = 7 - onz;
+    const uint32_t lane = __lane_id();
+    uint32_t stk0 = 0, stk1 = 0;  // the wave's stack
+    uint32_t sp = 0, cur = 0;     // wave-uniform
+    for (;;) {
+        if (cur & kLeafBit) {
+            const uint32_t first = leaf_first(cur), n_leaf = leaf_count(cur);
+            for (uint32_t k = 0; k < n_leaf; ++k) closest_prim(S, uniform_prim(S.prims + first + k), first + k, o, d, best);
+        } else {
+            const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
+            const float4 cw = uniform_load4(nf + 6);
+            const uint32_t child[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
+                                       __float_as_uint(cw.w)};
+            float4 NX, NY, NZ, FX, FY, FZ;  // per child: near and far plane of each axis
+            if (kUniformSigns) {  // the wave loads its near and far planes directly
+                NX = uniform_load4(nf + onx), NY = uniform_load4(nf + ony), NZ = uniform_load4(nf + onz);
+                FX = uniform_load4(nf + ofx), FY = uniform_load4(nf + ofy), FZ = uniform_load4(nf + ofz);
+            } else {
+                const float4 lx = uniform_load4(nf), ly = uniform_load4(nf + 1), lz = uniform_load4(nf + 2);
+                const float4 hx = uniform_load4(nf + 3), hy = uniform_load4(nf + 4), hz = uniform_load4(nf + 5);
+                NX = sx ? lx : hx, FX = sx ? hx : lx;
+                NY = sy ? ly : hy, FY = sy ? hy : ly;
+                NZ = sz ? lz : hz, FZ = sz ? hz : lz;
+            }
+            const float nx[4] = {NX.x, NX.y, NX.z, NX.w}, ny[4] = {NY.x, NY.y, NY.z, NY.w};
+            const float nz[4] = {NZ.x, NZ.y, NZ.z, NZ.w}, fx[4] = {FX.x, FX.y, FX.z, FX.w};
+            const float fy[4] = {FY.x, FY.y, FY.z, FY.w}, fz[4] = {FZ.x, FZ.y, FZ.z, FZ.w};
+            const float lim = best.t * 1.0001f + 1e-6f;  // < tlo for an invalid lane: nothing overlaps
+            float key[4];
+            uint32_t val[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float tn = fmaxf(fmaxf((nx[c] - o.x) * ix, (ny[c] - o.y) * iy), fmaxf((nz[c] - o.z) * iz, tlo));
+                const float tf = fminf(fminf((fx[c] - o.x) * ix, (fy[c] - o.y) * iy), fminf((fz[c] - o.z) * iz, lim));
+                const unsigned long long m = __ballot(child[c] != kChildEmpty && tn <= tf * 1.000002f);
+                key[c] = m ? lane_float(tn, (uint32_t)__builtin_ctzll(m)) : inf;
+                val[c] = child[c];
+            }
+#define CUPBR_PSWAP(a, b)                                                  \
+    if (key[b] < key[a]) {                                                  \
+        const float tk = key[a]; key[a] = key[b]; key[b] = tk;              \
+        const uint32_t tv = val[a]; val[a] = val[b]; val[b] = tv;           \
+    }
+            CUPBR_PSWAP(0, 1)
+            CUPBR_PSWAP(2, 3)
+            CUPBR_PSWAP(0, 2)
+            CUPBR_PSWAP(1, 3)
+            CUPBR_PSWAP(1, 2)
+#undef CUPBR_PSWAP
+            if (key[0] < inf) {
+#pragma unroll
+                for (int c = 3; c >= 1; --c) {
+                    if (!(key[c] < inf)) continue;
+                    if (sp < 64) stk0 = lane == sp ? val[c] : stk0;
+                    else stk1 = lane == sp - 64 ? val[c] : stk1;
+                    ++sp;
+                }
+                cur = val[0];
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = (uint32_t)__builtin_amdgcn_readlane((int)(sp < 64 ? stk0 : stk1), (int)(sp & 63u));
+    }
+}
+
+__device__ __forceinline__ void traverse_packet(const DevScene& S, v3 o, v3 d, bool valid, TraceResult& best) {
+    // an invalid lane's interval is empty whatever planes it is given, so only valid lanes vote
+    const unsigned long long v = __ballot(valid);
+    const unsigned long long px = __ballot(valid && 1.0f / d.x >= 0.0f), py = __ballot(valid && 1.0f / d.y >= 0.0f),
+                             pz = __ballot(valid && 1.0f / d.z >= 0.0f);
+    if ((px == 0 || px == v) && (py == 0 || py == v) && (pz == 0 || pz == v))
+        traverse_packet_impl<true>(S, o, d, valid, px != 0, py != 0, pz != 0, best);
+    else
+        traverse_packet_impl<false>(S, o, d, valid, false, false, false, best);
+}
 
 // object id and triangle-in-mesh index of a closest-hit result
 __device__ __forceinline__ void hit_ids(const DevScene& S, const TraceResult& r, int32_t& obj, int32_t& tri) {
