@@ -1,6 +1,7 @@
 #include "bvh_build.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <limits>
@@ -154,34 +155,99 @@ private:
     }
 };
 
+// SAH-optimal collapse of the binary tree into a BVH4 (the dynamic program of Ylitie, Karras and
+// Laine 2017, "Efficient incoherent ray traversal on GPUs through compressed wide BVHs", for 4
+// children). Cost of a tree, in units of the root's surface area: c_node x area of every BVH4 node
+// (a visit tests its children's boxes) + c_prim x area x primitives of every leaf. For binary node n,
+// G[n][j-1] is the least cost of representing its subtree as at most j subtrees under one parent:
+//   G[n][0] = min(leaf(n) if n holds <= max_leaf primitives, c_node A(n) + S(n, 4))
+//   G[n][j-1] = min(G[n][0], S(n, j)) for j = 2..4,  S(n, j) = min_k G[l][k-1] + G[r][j-k-1]
+// where S distributes j slots between the two binary children.
 struct Collapser {
     const std::vector<BNode>& B;
     Bvh4& out;
+    uint32_t max_leaf;
+    double c_node, c_prim;
     double root_area = 1.0;
+    std::vector<std::array<double, 4>> G;
+    std::vector<std::array<uint8_t, 4>> split;  // slots given to the left child (0: keep whole)
+    std::vector<uint8_t> as_leaf;
+    std::vector<uint32_t> first, count;
+
+    double a(uint32_t n) const { return (double)area(B[n].mn, B[n].mx) / root_area; }
+
+    // post-order over the binary tree (iterative: degenerate inputs can make it deep)
+    void solve(uint32_t root) {
+        const size_t nn = B.size();
+        G.assign(nn, {INFINITY, INFINITY, INFINITY, INFINITY});
+        split.assign(nn, {0, 0, 0, 0});
+        as_leaf.assign(nn, 0);
+        first.assign(nn, 0);
+        count.assign(nn, 0);
+        std::vector<std::pair<uint32_t, bool>> st{{root, false}};
+        while (!st.empty()) {
+            auto [n, done] = st.back();
+            st.pop_back();
+            const BNode& b = B[n];
+            if (!b.count && !done) {
+                st.push_back({n, true});
+                st.push_back({b.right, false});
+                st.push_back({b.left, false});
+                continue;
+            }
+            if (b.count) {
+                first[n] = b.first;
+                count[n] = b.count;
+            } else {
+                first[n] = first[b.left];
+                count[n] = count[b.left] + count[b.right];
+            }
+            const double leaf = count[n] <= max_leaf ? c_prim * a(n) * (double)count[n] : INFINITY;
+            if (b.count) {
+                G[n] = {leaf, leaf, leaf, leaf};
+                as_leaf[n] = 1;
+                continue;
+            }
+            double S[5];
+            uint8_t K[5] = {0, 0, 0, 0, 0};
+            for (int j = 2; j <= 4; ++j) {
+                S[j] = INFINITY;
+                for (int k = 1; k < j; ++k) {
+                    const double c = G[b.left][k - 1] + G[b.right][j - k - 1];
+                    if (c < S[j]) { S[j] = c; K[j] = (uint8_t)k; }
+                }
+            }
+            const double inner = c_node * a(n) + S[4];
+            as_leaf[n] = leaf <= inner;
+            G[n][0] = std::min(leaf, inner);
+            split[n][0] = 0;
+            for (int j = 2; j <= 4; ++j) {
+                if (S[j] < G[n][0]) { G[n][j - 1] = S[j]; split[n][j - 1] = K[j]; }
+                else { G[n][j - 1] = G[n][0]; split[n][j - 1] = 0; }
+            }
+        }
+    }
+
+    // the subtrees that fill j slots of a parent
+    void slots(uint32_t n, int j, std::vector<uint32_t>& outv) const {
+        const uint8_t k = j > 1 ? split[n][j - 1] : 0;
+        if (k == 0) { outv.push_back(n); return; }
+        slots(B[n].left, k, outv);
+        slots(B[n].right, j - k, outv);
+    }
 
     uint32_t emit(uint32_t b, uint32_t depth) {
         out.depth = std::max(out.depth, depth + 1);
         std::vector<uint32_t> kids;
         const BNode& bn = B[b];
         if (bn.count) kids.push_back(b);  // a leaf root gets a wrapper node
-        else { kids.push_back(bn.left); kids.push_back(bn.right); }
-        while (kids.size() < 4) {
-            int pick = -1;
-            float best = -1.0f;
-            for (size_t i = 0; i < kids.size(); ++i) {
-                const BNode& k = B[kids[i]];
-                if (k.count) continue;
-                float a = area(k.mn, k.mx);
-                if (a > best) { best = a; pick = (int)i; }
-            }
-            if (pick < 0) break;
-            uint32_t k = kids[pick];
-            kids[pick] = B[k].left;
-            kids.insert(kids.begin() + pick + 1, B[k].right);
+        else {
+            slots(bn.left, split_root_left(b), kids);
+            slots(bn.right, 4 - split_root_left(b), kids);
         }
         const uint32_t me = (uint32_t)out.nodes.size();
         out.nodes.emplace_back();
-        out.sah_cost += area(bn.mn, bn.mx) / root_area;
+        out.sah_cost += c_node * a(b);
         Node4 n;
         for (int i = 0; i < 4; ++i) {
             n.lo_x[i] = n.lo_y[i] = n.lo_z[i] = INFINITY;
@@ -193,16 +259,28 @@ struct Collapser {
             const BNode& k = B[kids[i]];
             n.lo_x[i] = k.mn[0]; n.lo_y[i] = k.mn[1]; n.lo_z[i] = k.mn[2];
             n.hi_x[i] = k.mx[0]; n.hi_y[i] = k.mx[1]; n.hi_z[i] = k.mx[2];
-            if (k.count) {
-                n.child[i] = leaf_word(k.first, k.count);
+            if (as_leaf[kids[i]]) {
+                n.child[i] = leaf_word(first[kids[i]], count[kids[i]]);
                 out.leaves++;
-                out.sah_cost += area(k.mn, k.mx) / root_area * k.count;
+                out.sah_cost += c_prim * a(kids[i]) * (double)count[kids[i]];
             } else {
                 n.child[i] = emit(kids[i], depth + 1);
             }
         }
         out.nodes[me] = n;
         return me;
+    }
+
+    // slots of a BVH4 node at binary node b given to its left binary child
+    int split_root_left(uint32_t b) const {
+        const BNode& bn = B[b];
+        int best_k = 1;
+        double best = INFINITY;
+        for (int k = 1; k < 4; ++k) {
+            const double c = G[bn.left][k - 1] + G[bn.right][4 - k - 1];
+            if (c < best) { best = c; best_k = k; }
+        }
+        return best_k;
     }
 };
 
@@ -213,10 +291,14 @@ Bvh4 build_bvh4(const std::vector<BuildPrim>& prims, const BvhBuildOptions& opt_
     if (prims.empty()) return out;
     BvhBuildOptions opt = opt_in;
     opt.max_leaf = std::min<uint32_t>(std::max<uint32_t>(opt.max_leaf, 1), kMaxLeafPrims);
-    BinaryBuilder bb(prims, opt);
+    // the binary tree goes down to single primitives; the collapse chooses the leaves
+    BvhBuildOptions bopt = opt;
+    bopt.max_leaf = 1;
+    BinaryBuilder bb(prims, bopt);
     uint32_t root = bb.run();
-    Collapser c{bb.nodes, out};
+    Collapser c{bb.nodes, out, opt.max_leaf, opt.traversal_cost, opt.intersect_cost};
     c.root_area = std::max(1e-30f, area(bb.nodes[root].mn, bb.nodes[root].mx));
+    c.solve(root);
     out.nodes.reserve(prims.size() / 2 + 16);
     c.emit(root, 0);
     out.order = std::move(bb.idx);

@@ -1,5 +1,6 @@
-// Host BVH builder: binned-SAH binary tree (task-parallel over subtrees),
-// collapsed into a BVH4 of 128-byte nodes in depth-first order.
+// Host BVH builder: binned-SAH binary tree down to single primitives (task-parallel over
+// subtrees), collapsed into a BVH4 of 128-byte nodes in depth-first order by a dynamic program
+// that minimises the SAH cost of the BVH4 and chooses its leaves (up to max_leaf primitives).
 //
 // The reference builds a median split over whole objects with one device
 // allocation per node (lib/src/DataStructure/BoundingVolumeHierarchy.cpp:46-136);
@@ -30,13 +31,16 @@ struct Bvh4 {
 #ifndef CUPBR_BVH_MAX_LEAF
 #define CUPBR_BVH_MAX_LEAF 8
 #endif
+// SAH cost of a BVH4 node visit (its 4 box tests) relative to one primitive test. 1.5 measured
+// best for PT+NEE at 1080p on 1M triangles (Msamples/s: 0.7 -> 627, 1.0 -> 633, 1.5 -> 637,
+// 2.0 -> 631, 3.0 -> 610; the earlier greedy collapse of a binary tree with SAH leaves: 605).
 #ifndef CUPBR_BVH_TRAVERSAL_COST
-#define CUPBR_BVH_TRAVERSAL_COST 1.0f
+#define CUPBR_BVH_TRAVERSAL_COST 1.5f
 #endif
 struct BvhBuildOptions {
     uint32_t max_leaf = CUPBR_BVH_MAX_LEAF;  // <= kMaxLeafPrims
-    float traversal_cost = CUPBR_BVH_TRAVERSAL_COST;
-    float intersect_cost = 1.0f;
+    float traversal_cost = CUPBR_BVH_TRAVERSAL_COST;  // BVH4 node visit
+    float intersect_cost = 1.0f;                      // one primitive test
     int threads = 0;            // 0: hardware concurrency
 };
 

@@ -273,10 +273,6 @@ __device__ __forceinline__ PrimRec uniform_prim(const PrimRec* p) {
     return PrimRec{uniform_load4(&p->a), uniform_load4(&p->b), uniform_load4(&p->c)};
 }
 
-__device__ __forceinline__ float lane_float(float v, uint32_t lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
-}
-
 // Entries of the wave-uniform stack of traverse_packet: entry k is held by lane k % 64 of
 // register k / 64.
 constexpr uint32_t kPacketStack = 128;
@@ -348,19 +344,26 @@ __device__ __forceinline__ void traverse_packet_impl(const DevScene& S, v3 o, v3
             const float nz[4] = {NZ.x, NZ.y, NZ.z, NZ.w}, fx[4] = {FX.x, FX.y, FX.z, FX.w};
             const float fy[4] = {FY.x, FY.y, FY.z, FY.w}, fz[4] = {FZ.x, FZ.y, FZ.z, FZ.w};
             const float lim = best.t * 1.0001f + 1e-6f;  // < tlo for an invalid lane: nothing overlaps
-            float key[4];
-            uint32_t val[4];
+            // ordering key of a child: the entry distance of the first lane that overlaps it, as
+            // float bits (negative distances as 0, so the keys order as unsigned integers on the
+            // scalar unit); 0xFFFFFFFF when no lane overlaps it
+            uint32_t key[4], val[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const float tn = fmaxf(fmaxf((nx[c] - o.x) * ix, (ny[c] - o.y) * iy), fmaxf((nz[c] - o.z) * iz, tlo));
                 const float tf = fminf(fminf((fx[c] - o.x) * ix, (fy[c] - o.y) * iy), fminf((fz[c] - o.z) * iz, lim));
-                const unsigned long long m = __ballot(child[c] != kChildEmpty && tn <= tf * 1.000002f);
-                key[c] = m ? lane_float(tn, (uint32_t)__builtin_ctzll(m)) : inf;
+                const unsigned long long m = child[c] != kChildEmpty ? __ballot(tn <= tf * 1.000002f) : 0ull;
+                uint32_t k = 0xFFFFFFFFu;
+                if (m) {
+                    k = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(tn), (int)__builtin_ctzll(m));
+                    k = (k >> 31) ? 0u : k;
+                }
+                key[c] = k;
                 val[c] = child[c];
             }
 #define CUPBR_PSWAP(a, b)                                                  \
     if (key[b] < key[a]) {                                                  \
-        const float tk = key[a]; key[a] = key[b]; key[b] = tk;              \
+        const uint32_t tk = key[a]; key[a] = key[b]; key[b] = tk;           \
         const uint32_t tv = val[a]; val[a] = val[b]; val[b] = tv;           \
     }
             CUPBR_PSWAP(0, 1)
@@ -369,10 +372,10 @@ __device__ __forceinline__ void traverse_packet_impl(const DevScene& S, v3 o, v3
             CUPBR_PSWAP(1, 3)
             CUPBR_PSWAP(1, 2)
 #undef CUPBR_PSWAP
-            if (key[0] < inf) {
+            if (key[0] != 0xFFFFFFFFu) {
 #pragma unroll
                 for (int c = 3; c >= 1; --c) {
-                    if (!(key[c] < inf)) continue;
+                    if (key[c] == 0xFFFFFFFFu) continue;
                     if (sp < 64) stk0 = lane == sp ? val[c] : stk0;
                     else stk1 = lane == sp - 64 ? val[c] : stk1;
                     ++sp;
