@@ -82,7 +82,7 @@ private:
         const float leaf_cost = opt.intersect_cost * (float)n;
         if (n <= 1) { make_leaf(node, lo, n); return; }
 
-        constexpr int NB = 32;
+        constexpr int NB = CUPBR_BVH_BINS;
         int best_axis = -1, best_bin = -1;
         float best_cost = INFINITY;
         const float parent_area = std::max(area(all.mn, all.mx), 1e-30f);

@@ -28,6 +28,10 @@ struct Bvh4 {
     double sah_cost = 0.0;
 };
 
+// SAH bins per axis and split of the binary build
+#ifndef CUPBR_BVH_BINS
+#define CUPBR_BVH_BINS 32
+#endif
 #ifndef CUPBR_BVH_MAX_LEAF
 #define CUPBR_BVH_MAX_LEAF 8
 #endif
