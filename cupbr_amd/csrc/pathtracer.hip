@@ -130,13 +130,12 @@ struct NrcShade {
 };
 
 template <bool kNrc>
-__global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int rr,
-                                                       const float4* __restrict__ q_o, const float4* __restrict__ q_d,
-                                                       const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
-                                                       float4* __restrict__ nq_d, float4* __restrict__ s_o,
-                                                       float4* __restrict__ s_d, float4* __restrict__ s_c,
-                                                       float4* __restrict__ weight, float4* __restrict__ rad,
-                                                       Counters* c, NrcShade nrc) {
+__device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bounce, uint32_t max_depth, int rr,
+                                         const float4* __restrict__ q_o, const float4* __restrict__ q_d,
+                                         const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
+                                         float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
+                                         float4* __restrict__ s_c, float4* __restrict__ weight,
+                                         float4* __restrict__ rad, Counters* c, const NrcShade& nrc) {
     __shared__ uint32_t s_append[kPtWaves + 1];
     const uint32_t n = c->queue_count[cur].v;
     if (blockIdx.x == 0 && threadIdx.x < 8) c->work[threadIdx.x].v = 0;  // the next trace starts at 0
@@ -254,6 +253,28 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(DevScene S, int cur, uint
                 for (int k = 0; k < 6; ++k) nrc.pending[(size_t)rs * 6 + k] = rec[k];
         }
     }
+}
+
+// Shade without the radiance cache. amdgpu_waves_per_eu(5, 8): 96 VGPRs instead of 102 and 5
+// waves per SIMD instead of 4, without spills (23.7 -> 23.1 ms of shading per 1080p 64-spp step;
+// 6 and 8 waves spill and are slower). The cache variant keeps its 123 VGPRs (it would spill).
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void k_pt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int rr, const float4* __restrict__ q_o,
+                const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
+                float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
+                float4* __restrict__ s_c, float4* __restrict__ weight, float4* __restrict__ rad, Counters* c,
+                NrcShade nrc) {
+    pt_shade<false>(S, cur, bounce, max_depth, rr, q_o, q_d, q_hit, nq_o, nq_d, s_o, s_d, s_c, weight, rad, c, nrc);
+}
+
+// Shade with the radiance cache: queries and training rows (NrcShade)
+__global__ __launch_bounds__(kPtBlock)
+void k_pt_shade_nrc(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int rr, const float4* __restrict__ q_o,
+                    const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
+                    float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
+                    float4* __restrict__ s_c, float4* __restrict__ weight, float4* __restrict__ rad, Counters* c,
+                    NrcShade nrc) {
+    pt_shade<true>(S, cur, bounce, max_depth, rr, q_o, q_d, q_hit, nq_o, nq_d, s_o, s_d, s_c, weight, rad, c, nrc);
 }
 
 // query count after the second bounce's shade: the queries inferred on the side stream
@@ -465,7 +486,7 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     // camera rays as wave packets (the visit-counting pass keeps the per-ray traversal, whose
     // counts define the algorithmic bytes of a trace launch)
     const bool packet = !r->counting && r->scene->stack_entries <= kPacketStack && S.n_nodes > 0;
-    auto shade = nrc ? k_pt_shade<true> : k_pt_shade<false>;
+    auto shade = nrc ? k_pt_shade_nrc : k_pt_shade;
     NrcShade ns{nullptr, nullptr, 0};
     uint32_t train_thresh = 0;
     if (nrc) {
