@@ -50,11 +50,33 @@ def _surface_rays(S, rays, seed):
     return np.concatenate([o, d], 1).astype(np.float32)
 
 
+def _degenerate_scene():
+    """Inputs that defeat the SAH build: 600 copies of one triangle in a mesh (every hit ties on t
+    and is decided by the reference rank), 400 triangles around one common centroid (no centroid
+    split exists) and 150 standalone triangles placed twice."""
+    rng = np.random.default_rng(5)
+    b = scenes.SceneBuilder()
+    m = scenes.material(scenes.LAMBERT, albedo_d=(0.5,) * 3)
+    t = np.array([[-1, -1, 0.3], [1, -0.5, 0.2], [0, 1, 0.4]], np.float32)
+    b.mesh(np.repeat(t.reshape(1, 9), 600, 0), m)
+    ang = rng.uniform(0, 2 * np.pi, (400, 3))
+    r = rng.uniform(0.05, 1.5, (400, 1))
+    v = np.stack([np.cos(ang), np.sin(ang), rng.uniform(-0.2, 0.2, (400, 3))], -1) * r[:, :, None]
+    v = v - v.mean(1, keepdims=True) + np.array([0.2, -0.1, -0.5])
+    b.mesh(v.reshape(400, 9), m)
+    for _ in range(150):
+        tri = rng.uniform(-1.5, 1.5, (3, 3))
+        b.triangle(tri[0], tri[1], tri[2], m)
+        b.triangle(tri[0], tri[1], tri[2], m)
+    return b.desc()
+
+
 SCENES = {
     "cornell": (lambda: scenes.cornell_box()[0], 0.0, 1.0),
     "mixed": (lambda: scenes.mixed_primitives()[0], -1.5, 4.0),
     "soup": (lambda: scenes.random_triangle_soup(3000, seed=1), -12, 12),
     "meshes": (lambda: scenes.random_mesh_scene(40, 300, seed=2), -12, 12),
+    "degenerate": (_degenerate_scene, -2.0, 2.0),
 }
 
 
