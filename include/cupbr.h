@@ -148,7 +148,8 @@ typedef struct cupbr_bvh_stats {
     uint32_t stack_entries;  /* traversal stack entries needed in the worst case */
     double build_ms;
     double upload_ms;
-    double sah_cost;
+    double sah_cost;         /* sum over BVH4 nodes of 1.5 x area + over leaves of area x primitives,
+                                areas relative to the root box */
 } cupbr_bvh_stats;
 
 typedef struct cupbr_render_stats {
