@@ -455,14 +455,15 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     const uint32_t w = r->width, h = r->height, npix = w * h;
     if (npix == 0 || spp == 0) return CUPBR_OK;
     const bool nrc = r->nrc != nullptr;
-    // Paths in flight, in whole frames: about 32M (200 B of path state and queues per path, so
-    // ~6.4 GB). A trace launch ends with the last 64-ray chunks of its slowest waves, and with
-    // more paths per launch that tail is a smaller share of it (1080p, 64 spp: 3 frames per batch
-    // 475 Msamples/s, 9 frames 518, 17 frames 533, all 64 frames 543). Bounded to 64M paths and
-    // to an eighth of the device memory. With a radiance cache the same holds: a call of k frames
-    // is one batch, and the cache trains after each batch.
-    const uint32_t target = 1u << 25;
-    uint32_t cap = 1u << 26;
+    // Paths in flight, in whole frames: up to 128M (200 B of path state and queues per path, so
+    // up to ~27 GB). A trace launch ends with the last 64-ray chunks of its slowest waves, and
+    // with more paths per launch that tail is a smaller share of it (1080p, 64 spp, Msamples/s:
+    // 17 frames per batch 637, 32 frames 651, all 64 frames 656). Bounded to an eighth of the
+    // device memory. A call of one frame (interactive use) holds one frame of paths. With a
+    // radiance cache the same holds: a call of k frames is one batch, and the cache trains after
+    // each batch.
+    const uint32_t target = 1u << 27;
+    uint32_t cap = 1u << 27;
     {
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)

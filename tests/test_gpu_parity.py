@@ -237,18 +237,18 @@ def test_running_mean_over_calls_equals_one_call():
 
 
 def test_batches_of_frames_equal_frame_by_frame_calls():
-    """One 16-spp call at 2048x2048 (67M paths) runs as two batches of 8 frames; its running mean
-    must equal sixteen one-frame calls bit for bit (frames are accumulated in order across
-    batches, PathTracer.cu:162-168)."""
+    """One 40-spp call at 2048x2048 (168M paths) runs as batches of 32 and 8 frames (at most 128M
+    paths in flight); its running mean must equal forty one-frame calls bit for bit (frames are
+    accumulated in order across batches, PathTracer.cu:162-168)."""
     desc, ca = scenes.cornell_box()
     w = h = 2048
     cam = _camera(ca, w, h)
     scene = cb.Scene.from_desc(desc)
-    a = _render(scene, cam, w, h, 16, calls=1)
+    a = _render(scene, cam, w, h, 40, calls=1)
     ia = a.get_output_image()
     del a
-    b = _render(scene, cam, w, h, 16, calls=16)
-    assert b.frame_index() == 16
+    b = _render(scene, cam, w, h, 40, calls=40)
+    assert b.frame_index() == 40
     ib = b.get_output_image()
     assert np.isfinite(ia).all() and float(ia.mean()) > 0
     assert np.array_equal(ia, ib)
