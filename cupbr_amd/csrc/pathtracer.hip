@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
         c->queue_count[0].v = npath;
-        for (int g = 0; g < 8; ++g) c->work[g].v = 0;
+        for (int g = 0; g < 8; ++g) c->work[g].v = c->swork[g].v = 0;
         c->shadow_count[0].v = 0;
         c->shadow_count[1].v = 0;
         c->query_count.v = 0;
@@ -138,7 +138,8 @@ __device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bo
                                          float4* __restrict__ rad, Counters* c, const NrcShade& nrc) {
     __shared__ uint32_t s_append[kPtWaves + 1];
     const uint32_t n = c->queue_count[cur].v;
-    if (blockIdx.x == 0 && threadIdx.x < 8) c->work[threadIdx.x].v = 0;  // the next trace starts at 0
+    if (blockIdx.x == 0 && threadIdx.x < 8)  // the next trace starts at 0
+        c->work[threadIdx.x].v = c->swork[threadIdx.x].v = 0;
     const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
     for (uint32_t base = blockIdx.x * kPtBlock; base < n; base += gridDim.x * kPtBlock) {
         const uint32_t i = base + threadIdx.x;
@@ -312,8 +313,63 @@ __device__ __forceinline__ void shadow_result(bool occluded, const float4 d4, co
     rad[p] = make_float4(L.x, L.y, L.z, R.w);
 }
 
-// One bounce's extension rays (queue cur) and the previous bounce's shadow rays (parity cur ^ 1),
-// handed out in 64-ray chunks by the 8 group counters below; every wave's chunk is of one kind.
+// One queue of the trace kernel, n rays, with per-lane refill: the queue is split into 8
+// contiguous groups, group h's rays handed out by counter[h]. A wave starts on group g0 (its
+// block's XCD), moves to the next group once its counter has passed the group's end, and gives
+// fresh rays to its idle lanes whenever at least 32 of them are idle (or all are), so that a
+// lane whose ray ended does not wait for the slowest ray of a 64-ray chunk. finish(i, hit, r)
+// consumes the result of ray i.
+template <bool kAny, class Load, class Finish>
+__device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint32_t g0, CounterLine* counter,
+                                            LaneStack<kPtLds>& st, Load load, Finish finish) {
+    const uint32_t lane = __lane_id();
+    uint32_t k = 0, h = g0;
+    uint32_t r0 = h * n / 8u, rl = (h + 1u) * n / 8u - r0;
+    bool active = false, pool = true, hit = false;
+    uint32_t my = 0;
+    RayState rs;
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        if (pool && (__popcll(idle) >= 32 || idle == ~0ull)) {
+            const uint32_t need = (uint32_t)__popcll(idle);
+            const uint32_t leader = (uint32_t)__builtin_ctzll(idle);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&counter[h].v, need);
+            base = (uint32_t)__shfl((int)base, (int)leader);
+            if (!active) {
+                const uint32_t idx = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (idx < rl) {  // ray r0 + idx < (h + 1) n / 8 <= n
+                    my = r0 + idx;
+                    v3 o, d;
+                    float dist;
+                    load(my, o, d, dist);
+                    active = ray_begin<kAny>(S, rs, o, d, dist, hit);
+                    if (!active) finish(my, hit, rs.best);
+                }
+            }
+            if (base + need >= rl) {  // group h has handed out all of its rays
+                if (++k == 8) pool = false;
+                else {
+                    h = (g0 + k) & 7u;
+                    r0 = h * n / 8u;
+                    rl = (h + 1u) * n / 8u - r0;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (!pool) break;
+            continue;
+        }
+        if (active && !ray_step<kAny>(S, rs, st, hit)) {
+            finish(my, hit, rs.best);
+            active = false;
+        }
+    }
+}
+
+// One bounce's extension rays (queue cur), then the previous bounce's shadow rays (parity cur ^ 1),
+// each queue handed out by 8 group counters (trace_queue; the visit-counting instantiation hands
+// out 64-ray chunks instead, one ray per lane until the chunk is done).
 // amdgpu_waves_per_eu(8, 8): the compiler keeps the kernel within the SGPR budget of 8 waves per
 // SIMD (spilling a few SGPRs into VGPR lanes); without it 104 SGPRs held the kernel to 6 resident
 // waves per SIMD and a quarter of the persistent blocks only started when the queue was empty.
@@ -335,36 +391,61 @@ void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o, const float
     }
     LaneStack<kPtLds> st{lds + threadIdx.x, kPtBlock, spill ? spill + gtid : nullptr, spill_stride};
     TravCount te, ts;
-    // 64-ray chunks, split into 8 groups: group h owns the h-th eighth of the extension chunks
-    // and the h-th eighth of the shadow chunks. Workgroups are dispatched to the 8 XCDs
-    // round-robin, so the waves of block b start on group b % 8: each XCD traces neighbouring
-    // queue entries (neighbouring pixels and hit points) and its own L2 holds the nodes they
-    // share. A group that runs dry takes chunks from the next groups in turn; every chunk is
-    // handed out once, whatever XCD a block runs on.
-    const uint32_t e_chunks = (n_ext + 63u) >> 6, s_chunks = (n_sh + 63u) >> 6;
+    // Each queue is split into 8 groups: group h owns its h-th eighth. Workgroups are dispatched
+    // to the 8 XCDs round-robin, so the waves of block b start on group b % 8: each XCD traces
+    // neighbouring queue entries (neighbouring pixels and hit points) and its own L2 holds the
+    // nodes they share. A group that runs dry takes rays from the next groups in turn; every ray
+    // is handed out once, whatever XCD a block runs on.
     const uint32_t g0 = blockIdx.x & 7u;
-    for (uint32_t k = 0; k < 8; ++k) {
-        const uint32_t h = (g0 + k) & 7u;
-        const uint32_t e0 = h * e_chunks / 8u, e_len = (h + 1u) * e_chunks / 8u - e0;
-        const uint32_t s0 = h * s_chunks / 8u, s_len = (h + 1u) * s_chunks / 8u - s0;
-        for (;;) {
-            const uint32_t q = wave_fetch(&c->work[h].v);
-            if (q >= e_len + s_len) break;
-            if (q < e_len) {
+    if (!kCount) {
+        trace_queue<false>(
+            S, n_ext, g0, c->work, st,
+            [&](uint32_t i, v3& o, v3& d, float& dist) {
+                const float4 o4 = q_o[i], d4 = q_d[i];
+                o = mk(o4.x, o4.y, o4.z);
+                d = mk(d4.x, d4.y, d4.z);
+                dist = 0.0f;
+            },
+            [&](uint32_t i, bool, const TraceResult& r) { q_hit[i] = make_float2(r.t, __uint_as_float(r.prim)); });
+        trace_queue<true>(
+            S, n_sh, g0, c->swork, st,
+            [&](uint32_t j, v3& o, v3& d, float& dist) {
+                const float4 o4 = s_o[j], d4 = s_d[j];
+                o = mk(o4.x, o4.y, o4.z);
+                d = mk(d4.x, d4.y, d4.z);
+                dist = o4.w;
+            },
+            [&](uint32_t j, bool occluded, const TraceResult&) { shadow_result(occluded, s_d[j], s_c, j, rad); });
+    } else {
+        // the visit-counting pass: 64-ray chunks, one ray per lane until the chunk is done
+        const uint32_t e_chunks = (n_ext + 63u) >> 6, s_chunks = (n_sh + 63u) >> 6;
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint32_t h = (g0 + k) & 7u;
+            const uint32_t e0 = h * e_chunks / 8u, e_len = (h + 1u) * e_chunks / 8u - e0;
+            for (;;) {
+                const uint32_t q = wave_fetch(&c->work[h].v);
+                if (q >= e_len) break;
                 const uint32_t i = ((e0 + q) << 6) + __lane_id();
                 if (i < n_ext) {
                     const float4 o4 = q_o[i], d4 = q_d[i];
                     TraceResult r;
-                    traverse<false, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r, &te);
+                    traverse<false, kPtLds, true>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r, &te);
                     q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
                 }
-            } else {
-                const uint32_t j = ((s0 + q - e_len) << 6) + __lane_id();
+            }
+        }
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint32_t h = (g0 + k) & 7u;
+            const uint32_t s0 = h * s_chunks / 8u, s_len = (h + 1u) * s_chunks / 8u - s0;
+            for (;;) {
+                const uint32_t q = wave_fetch(&c->swork[h].v);
+                if (q >= s_len) break;
+                const uint32_t j = ((s0 + q) << 6) + __lane_id();
                 if (j < n_sh) {
                     const float4 o4 = s_o[j], d4 = s_d[j];
                     TraceResult r;
                     const bool occluded =
-                        traverse<true, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r, &ts);
+                        traverse<true, kPtLds, true>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r, &ts);
                     shadow_result(occluded, d4, s_c, j, rad);
                 }
             }

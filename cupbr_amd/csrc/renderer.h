@@ -23,7 +23,8 @@ struct alignas(128) CounterLine {
 struct Counters {
     CounterLine queue_count[2];  // extension queues (ping-pong by bounce parity)
     CounterLine shadow_count[2]; // shadow queues (bounce parity)
-    CounterLine work[8];         // work counters of the trace kernel, one per XCD group
+    CounterLine work[8];         // trace kernel: extension rays handed out, per XCD group
+    CounterLine swork[8];        // trace kernel: shadow rays handed out, per XCD group
     CounterLine query_count;  // radiance-cache queries of the batch
     CounterLine query_early;  // queries that existed after the second bounce's shade
     CounterLine row_count;    // radiance-cache training rows of the batch

@@ -157,13 +157,77 @@ struct TravCount {
     uint32_t nodes = 0, prims = 0;
 };
 
+// The primitives of one leaf (leaf word `word`): closest-hit updates of best, or (kAny) true as
+// soon as one is hit within [0, dist].
+template <bool kAny>
+__device__ __forceinline__ bool visit_leaf(const DevScene& S, uint32_t word, v3 o, v3 d, float dist, TraceResult& best) {
+    const uint32_t first = leaf_first(word), n_leaf = leaf_count(word);
+    for (uint32_t k = 0; k < n_leaf; ++k) {
+        const PrimRec pr = S.prims[first + k];
+        if (kAny) {
+            float t;
+            if (!prim_test(pr, o, d, t) || !(t <= dist) || !object_box_ok(S, pr, o, d)) continue;
+            return true;
+        }
+        closest_prim(S, pr, first + k, o, d, best);
+    }
+    return false;
+}
+
+// One inner node `cur`: the children whose boxes overlap [tlo, lim] (conservatively) are pushed
+// farthest first, and cur becomes the nearest (closest hit) or a hit one (any hit); false when
+// no child overlaps.
+template <bool kAny, int kLds>
+__device__ __forceinline__ bool visit_inner(const DevScene& S, uint32_t& cur, v3 o, float ix, float iy, float iz,
+                                            float lim, LaneStack<kLds>& st, int& sp) {
+    const float inf = __builtin_inff();
+    const bool sx = ix >= 0.0f, sy = iy >= 0.0f, sz = iz >= 0.0f;
+    const float tlo = -1e-5f;
+    const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
+    const float4 lx = nf[0], ly = nf[1], lz = nf[2], hx = nf[3], hy = nf[4], hz = nf[5];
+    const uint4 ch = *reinterpret_cast<const uint4*>(nf + 6);
+    const float4 nx = sx ? lx : hx, fx = sx ? hx : lx;
+    const float4 ny = sy ? ly : hy, fy = sy ? hy : ly;
+    const float4 nz = sz ? lz : hz, fz = sz ? hz : lz;
+    float key[4];
+    uint32_t val[4] = {ch.x, ch.y, ch.z, ch.w};
+#define CUPBR_CHILD(i, c)                                                                          \
+    {                                                                                              \
+        float tn = fmaxf(fmaxf((nx.c - o.x) * ix, (ny.c - o.y) * iy), fmaxf((nz.c - o.z) * iz, tlo)); \
+        float tf = fminf(fminf((fx.c - o.x) * ix, (fy.c - o.y) * iy), fminf((fz.c - o.z) * iz, lim)); \
+        bool ok = val[i] != kChildEmpty && tn <= tf * 1.000002f;                                  \
+        key[i] = ok ? tn : inf;                                                                    \
+    }
+    CUPBR_CHILD(0, x)
+    CUPBR_CHILD(1, y)
+    CUPBR_CHILD(2, z)
+    CUPBR_CHILD(3, w)
+#undef CUPBR_CHILD
+    if (!kAny) {  // nearest child first; an any-hit query only needs the ones that were hit
+        CUPBR_CSWAP(0, 1)
+        CUPBR_CSWAP(2, 3)
+        CUPBR_CSWAP(0, 2)
+        CUPBR_CSWAP(1, 3)
+        CUPBR_CSWAP(1, 2)
+    } else {
+        CUPBR_CSWAP(0, 1)  // a hit child, if any, into slot 0
+        CUPBR_CSWAP(0, 2)
+        CUPBR_CSWAP(0, 3)
+    }
+    if (!(key[0] < inf)) return false;
+    if (key[3] < inf) st.put(sp++, val[3]);
+    if (key[2] < inf) st.put(sp++, val[2]);
+    if (key[1] < inf) st.put(sp++, val[1]);
+    cur = val[0];
+    return true;
+}
+
 // kAny=false: closest hit (best.t = +inf, best.prim = kNoHit on a miss).
 // kAny=true : any valid hit with t <= dist; returns whether one exists.
 template <bool kAny, int kLds, bool kCount = false>
 __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float dist, LaneStack<kLds>& st,
                                          TraceResult& best, TravCount* cnt = nullptr) {
-    const float inf = __builtin_inff();
-    best.t = inf;
+    best.t = __builtin_inff();
     best.prim = kNoHit;
     best.rank = 0xFFFFFFFFu;
 
@@ -186,79 +250,85 @@ __device__ __forceinline__ bool traverse(const DevScene& S, v3 o, v3 d, float di
     if (S.n_nodes == 0) return false;
 
     const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
-    const bool sx = ix >= 0.0f, sy = iy >= 0.0f, sz = iz >= 0.0f;
-    const float tlo = -1e-5f;
     int sp = 0;
     uint32_t cur = 0;
-
-    // the primitives of one leaf; true when an any-hit query is answered
-    auto leaf = [&](uint32_t word) -> bool {
-        const uint32_t first = leaf_first(word), n_leaf = leaf_count(word);
-        if (kCount) cnt->prims += n_leaf;
-        for (uint32_t k = 0; k < n_leaf; ++k) {
-            const PrimRec pr = S.prims[first + k];
-            if (kAny) {
-                float t;
-                if (!prim_test(pr, o, d, t) || !(t <= dist) || !object_box_ok(S, pr, o, d)) continue;
-                return true;
-            }
-            closest_prim(S, pr, first + k, o, d, best);
-        }
-        return false;
-    };
-    // one inner node: push the far children, return whether a child was entered (cur = nearest)
-    auto inner = [&]() -> bool {
-        if (kCount) cnt->nodes += 1;
-        const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
-        const float4 lx = nf[0], ly = nf[1], lz = nf[2], hx = nf[3], hy = nf[4], hz = nf[5];
-        const uint4 ch = *reinterpret_cast<const uint4*>(nf + 6);
-        const float4 nx = sx ? lx : hx, fx = sx ? hx : lx;
-        const float4 ny = sy ? ly : hy, fy = sy ? hy : ly;
-        const float4 nz = sz ? lz : hz, fz = sz ? hz : lz;
-        const float lim = kAny ? dist * 1.0001f + 1e-6f : best.t * 1.0001f + 1e-6f;
-        float key[4];
-        uint32_t val[4] = {ch.x, ch.y, ch.z, ch.w};
-#define CUPBR_CHILD(i, c)                                                                          \
-    {                                                                                              \
-        float tn = fmaxf(fmaxf((nx.c - o.x) * ix, (ny.c - o.y) * iy), fmaxf((nz.c - o.z) * iz, tlo)); \
-        float tf = fminf(fminf((fx.c - o.x) * ix, (fy.c - o.y) * iy), fminf((fz.c - o.z) * iz, lim)); \
-        bool ok = val[i] != kChildEmpty && tn <= tf * 1.000002f;                                  \
-        key[i] = ok ? tn : inf;                                                                    \
-    }
-        CUPBR_CHILD(0, x)
-        CUPBR_CHILD(1, y)
-        CUPBR_CHILD(2, z)
-        CUPBR_CHILD(3, w)
-#undef CUPBR_CHILD
-        if (!kAny) {  // nearest child first; an any-hit query only needs the ones that were hit
-            CUPBR_CSWAP(0, 1)
-            CUPBR_CSWAP(2, 3)
-            CUPBR_CSWAP(0, 2)
-            CUPBR_CSWAP(1, 3)
-            CUPBR_CSWAP(1, 2)
-        } else {
-            CUPBR_CSWAP(0, 1)  // a hit child, if any, into slot 0
-            CUPBR_CSWAP(0, 2)
-            CUPBR_CSWAP(0, 3)
-        }
-        if (!(key[0] < inf)) return false;
-        if (key[3] < inf) st.put(sp++, val[3]);
-        if (key[2] < inf) st.put(sp++, val[2]);
-        if (key[1] < inf) st.put(sp++, val[1]);
-        cur = val[0];
-        return true;
-    };
-
     for (;;) {
         if (cur & kLeafBit) {
-            if (leaf(cur)) return true;
-        } else if (inner()) {
-            continue;
+            if (kCount) cnt->prims += leaf_count(cur);
+            if (visit_leaf<kAny>(S, cur, o, d, dist, best)) return true;
+        } else {
+            if (kCount) cnt->nodes += 1;
+            const float lim = kAny ? dist * 1.0001f + 1e-6f : best.t * 1.0001f + 1e-6f;
+            if (visit_inner<kAny>(S, cur, o, ix, iy, iz, lim, st, sp)) continue;
         }
         if (sp == 0) break;
         cur = st.get(--sp);
     }
     return false;
+}
+
+// The same traversal as a resumable state, one node or leaf per step, for kernels that hand a
+// new ray to a lane as soon as its previous one ends. best.t holds the query distance of an
+// any-hit ray.
+struct RayState {
+    v3 o, d;
+    float ix, iy, iz;
+    TraceResult best;
+    uint32_t cur;
+    int sp;
+};
+
+// Starts a ray: planes and quads first (as in traverse). Returns whether BVH steps follow; when
+// not, the result is final (closest hit in best; any hit in `hit`).
+template <bool kAny>
+__device__ __forceinline__ bool ray_begin(const DevScene& S, RayState& r, v3 o, v3 d, float dist, bool& hit) {
+    r.o = o;
+    r.d = d;
+    r.best.t = kAny ? dist : __builtin_inff();
+    r.best.prim = kNoHit;
+    r.best.rank = 0xFFFFFFFFu;
+    hit = false;
+    for (uint32_t i = 0; i < S.n_unb; ++i) {
+        const UnbRec u = S.unb[i];
+        float t;
+        if (!plane_quad_test(u, o, d, t)) continue;
+        if (kAny) {
+            if (t <= dist) {
+                hit = true;
+                return false;
+            }
+            continue;
+        }
+        const uint32_t rank = __float_as_uint(u.n.w);
+        if (t < r.best.t || (t == r.best.t && rank < r.best.rank)) {
+            r.best.t = t;
+            r.best.prim = kUnbBit | i;
+            r.best.rank = rank;
+        }
+    }
+    r.ix = 1.0f / d.x;
+    r.iy = 1.0f / d.y;
+    r.iz = 1.0f / d.z;
+    r.cur = 0;
+    r.sp = 0;
+    return S.n_nodes != 0;
+}
+
+// One node or leaf of a started ray; false when its traversal has ended.
+template <bool kAny, int kLds>
+__device__ __forceinline__ bool ray_step(const DevScene& S, RayState& r, LaneStack<kLds>& st, bool& hit) {
+    if (r.cur & kLeafBit) {
+        if (visit_leaf<kAny>(S, r.cur, r.o, r.d, r.best.t, r.best)) {
+            hit = true;
+            return false;
+        }
+    } else {
+        const float lim = r.best.t * 1.0001f + 1e-6f;  // the query distance of an any-hit ray
+        if (visit_inner<kAny>(S, r.cur, r.o, r.ix, r.iy, r.iz, lim, st, r.sp)) return true;
+    }
+    if (r.sp == 0) return false;
+    r.cur = st.get(--r.sp);
+    return true;
 }
 #undef CUPBR_CSWAP
 
