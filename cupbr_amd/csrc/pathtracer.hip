@@ -368,8 +368,8 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
 }
 
 // One bounce's extension rays (queue cur), then the previous bounce's shadow rays (parity cur ^ 1),
-// each queue handed out by 8 group counters (trace_queue; the visit-counting instantiation hands
-// out 64-ray chunks instead, one ray per lane until the chunk is done).
+// each queue handed out by 8 group counters (trace_queue; short launches and the visit-counting
+// instantiation hand out 64-ray chunks instead, one ray per lane until the chunk is done).
 // amdgpu_waves_per_eu(8, 8): the compiler keeps the kernel within the SGPR budget of 8 waves per
 // SIMD (spilling a few SGPRs into VGPR lanes); without it 104 SGPRs held the kernel to 6 resident
 // waves per SIMD and a quarter of the persistent blocks only started when the queue was empty.
@@ -397,7 +397,12 @@ void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o, const float
     // nodes they share. A group that runs dry takes rays from the next groups in turn; every ray
     // is handed out once, whatever XCD a block runs on.
     const uint32_t g0 = blockIdx.x & 7u;
-    if (!kCount) {
+    // Refill pays off once every lane of the persistent grid has several rays to trace; a short
+    // launch (deep bounces, one-frame calls) ends sooner with the one tail of 64-ray chunks than
+    // with the two tails of the refill (extension rays, then shadow rays). The decision is the
+    // same for every wave (n_ext and n_sh are read at the start).
+    const bool refill = !kCount && n_ext + n_sh >= 4u * gridDim.x * kPtBlock;
+    if (refill) {
         trace_queue<false>(
             S, n_ext, g0, c->work, st,
             [&](uint32_t i, v3& o, v3& d, float& dist) {
@@ -417,36 +422,34 @@ void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o, const float
             },
             [&](uint32_t j, bool occluded, const TraceResult&) { shadow_result(occluded, s_d[j], s_c, j, rad); });
     } else {
-        // the visit-counting pass: 64-ray chunks, one ray per lane until the chunk is done
+        // 64-ray chunks, one ray per lane until the chunk is done (and the visit-counting pass):
+        // group h's counter hands out its extension chunks, then its shadow chunks
         const uint32_t e_chunks = (n_ext + 63u) >> 6, s_chunks = (n_sh + 63u) >> 6;
         for (uint32_t k = 0; k < 8; ++k) {
             const uint32_t h = (g0 + k) & 7u;
             const uint32_t e0 = h * e_chunks / 8u, e_len = (h + 1u) * e_chunks / 8u - e0;
-            for (;;) {
-                const uint32_t q = wave_fetch(&c->work[h].v);
-                if (q >= e_len) break;
-                const uint32_t i = ((e0 + q) << 6) + __lane_id();
-                if (i < n_ext) {
-                    const float4 o4 = q_o[i], d4 = q_d[i];
-                    TraceResult r;
-                    traverse<false, kPtLds, true>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r, &te);
-                    q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
-                }
-            }
-        }
-        for (uint32_t k = 0; k < 8; ++k) {
-            const uint32_t h = (g0 + k) & 7u;
             const uint32_t s0 = h * s_chunks / 8u, s_len = (h + 1u) * s_chunks / 8u - s0;
             for (;;) {
-                const uint32_t q = wave_fetch(&c->swork[h].v);
-                if (q >= s_len) break;
-                const uint32_t j = ((s0 + q) << 6) + __lane_id();
-                if (j < n_sh) {
-                    const float4 o4 = s_o[j], d4 = s_d[j];
-                    TraceResult r;
-                    const bool occluded =
-                        traverse<true, kPtLds, true>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, st, r, &ts);
-                    shadow_result(occluded, d4, s_c, j, rad);
+                const uint32_t q = wave_fetch(&c->work[h].v);
+                if (q >= e_len + s_len) break;
+                if (q < e_len) {
+                    const uint32_t i = ((e0 + q) << 6) + __lane_id();
+                    if (i < n_ext) {
+                        const float4 o4 = q_o[i], d4 = q_d[i];
+                        TraceResult r;
+                        traverse<false, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.0f, st, r,
+                                                        &te);
+                        q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
+                    }
+                } else {
+                    const uint32_t j = ((s0 + q - e_len) << 6) + __lane_id();
+                    if (j < n_sh) {
+                        const float4 o4 = s_o[j], d4 = s_d[j];
+                        TraceResult r;
+                        const bool occluded = traverse<true, kPtLds, kCount>(S, mk(o4.x, o4.y, o4.z),
+                                                                              mk(d4.x, d4.y, d4.z), o4.w, st, r, &ts);
+                        shadow_result(occluded, d4, s_c, j, rad);
+                    }
                 }
             }
         }
