@@ -254,18 +254,17 @@ def test_batches_of_frames_equal_frame_by_frame_calls():
     assert np.array_equal(ia, ib)
 
 
-@pytest.mark.parametrize("name", ["mixed_env", "million"])
-def test_camera_packets_equal_per_ray_traversal(name):
+@pytest.mark.parametrize("name,w,h,spp", [("mixed_env", 67, 45, 4), ("million", 203, 117, 4),
+                                          ("mixed_env", 1536, 1024, 4), ("million", 640, 480, 16)])
+def test_fast_paths_equal_per_ray_traversal(name, w, h, spp):
     """Bounce 0 traces each 64-ray chunk of camera rays as one packet (wave-uniform node and
-    primitive loads, a child entered when any lane's interval overlaps it); the visit-counting mode
-    traces the same rays one per lane. The closest hit is a minimum over candidates, so the two
-    images must agree bit for bit, also where the last chunk is partial (odd image sizes)."""
-    if name == "million":
-        desc, ca = scenes.million_triangles()
-        w, h = 203, 117
-    else:
-        desc, ca = scenes.mixed_primitives(env=True)
-        w, h = 67, 45
+    primitive loads, a child entered when any lane's interval overlaps it), and launches of at
+    least 4 rays per lane of the persistent trace grid (about 2 M rays; the last two cases) hand
+    fresh rays to idle lanes as soon as half of a wave is idle. The visit-counting mode traces every
+    ray one per lane in 64-ray chunks. The closest hit is a minimum over candidates and visibility
+    an "or" over them, independent of which lane traces a ray when, so the images must agree bit
+    for bit, also where the last chunk is partial (odd image sizes)."""
+    desc, ca = scenes.million_triangles() if name == "million" else scenes.mixed_primitives(env=True)
     scene = cb.Scene.from_desc(desc)
     cam = _camera(ca, w, h)
     imgs = []
@@ -275,8 +274,11 @@ def test_camera_packets_equal_per_ray_traversal(name):
         r.register_scene(scene)
         r.set_max_trace_depth(5)
         r.set_traversal_counting(counting)
-        r.render(cam, 4)
+        r.render(cam, spp)
         imgs.append(r.get_output_image())
+        if counting:
+            rays = r.stats()["extension_rays"]
+    print(f"{name} {w}x{h} {spp} spp: {rays} extension rays", flush=True)
     assert np.isfinite(imgs[0]).all() and float(imgs[0].mean()) > 0
     assert np.array_equal(imgs[0], imgs[1])
 
