@@ -556,3 +556,280 @@ def test_gdpt_median_is_mean_of_middle_two():
                      base[y, x + 1, c] - gx[y, x, c], base[y + 1, x, c] - gy[y, x, c]]
                 s = np.sort(np.array(v, np.float64))
                 assert recon[y, x, c] == pytest.approx((s[1] + s[2]) / 2, rel=1e-5, abs=1e-6)
+
+
+# --------------------------------------------------------------------------- BSDF lobes without a pin so far
+
+
+def _hemi(rng, n, k):
+    v = _unit(rng.normal(size=(k, 3)))
+    return np.where((v @ n)[:, None] < 0, -v, v)
+
+
+def test_phong_is_reciprocal_and_uses_the_half_vector():
+    """brdf_phong + brdf_lambert (MaterialDetail.h:209-214,46-49): albedo_s (h.n)^s + albedo_d/pi
+    with h the normalised half vector. Reciprocity f(i,o) = f(o,i); and at normal incidence the
+    half vector of an outgoing direction at angle t lies at t/2, so the lobe is cos(t/2)^s, not the
+    mirror-lobe form cos(t)^s."""
+    s = 51.2
+    m = scenes.material(scenes.PHONG, albedo_d=(0.3, 0.2, 0.1), albedo_s=(0.6, 0.5, 0.4), shininess=s)
+    n = _unit([0.2, 0.9, -0.3])
+    rng = np.random.default_rng(30)
+    I, O = _hemi(rng, n, 200), _hemi(rng, n, 200)
+    for i, o in zip(I, O):
+        np.testing.assert_allclose(oracle.brdf(m, i, o, n), oracle.brdf(m, o, i, n), rtol=1e-12, atol=0)
+    ad = np.array([0.3, 0.2, 0.1], np.float32).astype(np.float64)
+    as_ = np.array([0.6, 0.5, 0.4], np.float32).astype(np.float64)
+    t1 = _unit(np.cross(n, [1.0, 0, 0]))
+    for t in (0.0, 0.1, 0.3, 0.7, 1.2):
+        o = math.cos(t) * n + math.sin(t) * t1
+        # brdf_lambert divides by static_cast<float>(M_PI) (MaterialDetail.h:206)
+        want = as_ * math.cos(t / 2) ** float(np.float32(s)) + ad / float(np.float32(math.pi))
+        np.testing.assert_allclose(oracle.brdf(m, n, o, n), want, rtol=1e-9)
+
+
+def _smith_g2_heitz(nl, nv, a):
+    """Height-correlated Smith masking-shadowing, Heitz 2014 (JCGT 3(2), eq. 99):
+    G2 = 1 / (1 + Lambda(l) + Lambda(v)), Lambda(w) = (-1 + sqrt(1 + a^2 tan^2 w)) / 2."""
+    lam = lambda c: (-1 + math.sqrt(1 + a * a * (1 - c * c) / (c * c))) / 2
+    return 1 / (1 + lam(nl) + lam(nv))
+
+
+@pytest.mark.parametrize("a", [0.05, 0.2, 0.5])
+def test_ggx_visibility_is_heitz_height_correlated_smith(a):
+    """brdf_ggx = D * V * F (MaterialDetail.h:259-274): with albedo_s = 1 the Schlick term is 1, so
+    brdf / D_GGX is V_SmithJointGGX (:20-31), which must equal G2 / (4 n.l n.v) with G2 from
+    Heitz's Lambda form (up to the reference's 1e-5 guards in both denominators)."""
+    m = scenes.material(scenes.GGX, albedo_s=(1, 1, 1), roughness=a)
+    n = np.array([0.0, 0.0, 1.0])
+    rng = np.random.default_rng(31)
+    for _ in range(60):
+        i, o = _hemi(rng, n, 2)
+        if i @ n < 0.1 or o @ n < 0.1:
+            continue
+        h = _unit(i + o)
+        V = oracle.brdf(m, i, o, n)[0] / oracle.d_ggx(h @ n, float(np.float32(a)))
+        af = float(np.float32(a))
+        nl, nv = o @ n, i @ n
+        g2 = _smith_g2_heitz(nl, nv, af)
+        want = g2 / (4 * nl * nv)
+        # the 1e-5 guards perturb the two denominators, 2 nl nv / G2 and 4 nl nv, by 1e-5 each
+        guard = 1e-5 * (g2 / (2 * nl * nv) + 1 / (4 * nl * nv))
+        assert abs(V / want - 1) <= 1.01 * guard + 1e-9, (V, want, guard)
+        assert abs(V / want - 1) >= 0.5 * guard  # and the guards are there
+
+
+def _ggx_albedo_quadrature(a, cos_i):
+    """Directional albedo int f cos dw of GGX with Heitz's height-correlated Smith G2 and F = 1,
+    by 2D trapezoidal quadrature over the outgoing hemisphere (independent of the oracle)."""
+    i = np.array([math.sqrt(1 - cos_i ** 2), 0.0, cos_i])
+    th = np.linspace(1e-6, math.pi / 2 - 1e-6, 1500)
+    ph = np.linspace(0, 2 * math.pi, 1500)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    o = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1)
+    h = i + o
+    h /= np.linalg.norm(h, axis=-1, keepdims=True)
+    nh, nl = h[..., 2], o[..., 2]
+    D = a * a / (math.pi * ((nh * nh) * (a * a - 1) + 1) ** 2)
+    lam = lambda c: (-1 + np.sqrt(1 + a * a * (1 - c * c) / (c * c))) / 2
+    G2 = 1 / (1 + lam(nl) + lam(cos_i))
+    f = D * G2 / (4 * nl * cos_i)
+    return float(np.trapezoid(np.trapezoid(f * nl * np.sin(T), ph, axis=1), th))
+
+
+def test_ggx_is_reciprocal_and_its_furnace_matches_quadrature():
+    """f(i,o) = f(o,i) for a coloured Schlick F0; and the white furnace with albedo_s = 1 (F = 1):
+    E[f cos / pdf] over the sampler's directions (pdf pinned by the histogram test above) equals
+    the directional albedo from a quadrature of D G2 / (4 n.l n.v) with Heitz's G2 (within 4 sigma
+    of the Monte Carlo error); below 1 because single scattering loses what masking hides."""
+    m = scenes.material(scenes.GGX, albedo_s=(0.9, 0.6, 0.2), roughness=0.3)
+    n = _unit([0.1, -0.2, 1.0])
+    rng = np.random.default_rng(32)
+    for i, o in zip(_hemi(rng, n, 100), _hemi(rng, n, 100)):
+        np.testing.assert_allclose(oracle.brdf(m, i, o, n), oracle.brdf(m, o, i, n), rtol=1e-10)
+    n = np.array([0.0, 0.0, 1.0])
+    i = _unit([0.4, 0.0, 1.0])
+    for a in (0.2, 0.5):
+        w = scenes.material(scenes.GGX, albedo_s=(1, 1, 1), roughness=a)
+        s = oracle.sample_direction(w, 33, i, n, 6000)
+        est = np.zeros(len(s))
+        for k in np.flatnonzero((s[:, :3] @ n) > 0):
+            est[k] = oracle.brdf(w, i, s[k, :3], n)[0] * (s[k, :3] @ n) / s[k, 3]
+        want = _ggx_albedo_quadrature(float(np.float32(a)), i[2])
+        err = est.std() / math.sqrt(len(est))
+        assert abs(est.mean() - want) < 4 * err, (a, est.mean(), want, err)
+        assert want < 1.0
+
+
+def test_mirror_reflection_law_and_delta_window():
+    """sample_mirror returns the reflection 2(i.n)n - i with pdf 1 (MaterialDetail.h:135-141);
+    brdf_mirror (:216-223) is albedo_s delta(1 - o.r) / o.n, where delta is 1 when
+    |1 - o.r| < 1e-5, i.e. within acos(1 - 1e-5) = 0.00447 rad of r. So f cos / pdf = albedo_s on
+    the sampled direction, and the lobe is zero just outside that window."""
+    m = scenes.material(scenes.MIRROR, albedo_s=(0.8, 0.7, 0.6))
+    a_s = np.array([0.8, 0.7, 0.6], np.float32).astype(np.float64)
+    rng = np.random.default_rng(34)
+    for n in _unit(rng.normal(size=(20, 3))):
+        i = _hemi(rng, n, 1)[0]
+        r = 2 * (i @ n) * n - i
+        s = oracle.sample_direction(m, 1, i, n, 1)[0]
+        np.testing.assert_allclose(s[:3], r, atol=1e-12)
+        assert s[3] == 1.0
+        np.testing.assert_allclose(oracle.brdf(m, i, r, n) * (r @ n) / s[3], a_s, rtol=1e-9)
+        t = _unit(np.cross(r, n if abs(r @ n) < 0.99 else [1.0, 0, 0]))
+        for ang, inside in ((0.004, True), (0.0050, False)):
+            o = math.cos(ang) * r + math.sin(ang) * t
+            assert (oracle.brdf(m, i, o, n)[0] > 0) == inside, (ang, inside)
+
+
+def test_environment_lookup_texels_of_the_axes():
+    """direction2UV (TracingDetail.h:185-196): theta = acos(y)/pi, phi = (atan2(z, x) + pi)/(2 pi),
+    texel ((w-1) phi, (h-1) theta) truncated. A ray that escapes at bounce 0 returns its texel, so a
+    map whose texel (x, y) holds x + 1000 y reveals the lookup: +y -> row 0, -y -> row h-1 (both
+    at atan2(0, 0) = 0, column (w-1)/2), -z/+x/+z/-x -> columns (w-1)/4, (w-1)/2, 3(w-1)/4, w-1."""
+    W, H = 64, 32
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float32)
+    env = np.stack([xx + 1000 * yy] * 3, -1).astype(np.float32)
+    b = scenes.SceneBuilder()
+    b.triangle((500, 500, 500), (501, 500, 500), (500, 501, 500), scenes.material(scenes.LAMBERT))
+    b.env = env
+    S = oracle.OracleScene(b.desc())
+    cases = {(0, 1, 0): (int(0.5 * (W - 1)), 0), (0, -1, 0): (int(0.5 * (W - 1)), H - 1), (0, 0, -1): (int(0.25 * (W - 1)), int(0.5 * (H - 1))),
+             (1, 0, 0): (int(0.5 * (W - 1)), int(0.5 * (H - 1))), (0, 0, 1): (int(0.75 * (W - 1)), int(0.5 * (H - 1))),
+             (-1, 0, 0): (W - 1, int(0.5 * (H - 1)))}
+    for d, (tx, ty) in cases.items():
+        cam = dict(position=[0, 0, 0], x_axis=[0, 0, 0], y_axis=[0, 0, 0], z_axis=list(d))
+        img = S.render(cam, 1, 1, 0, 1, depth=1, precision=64)
+        assert img[0, 0, 0] == tx + 1000 * ty, (d, img[0, 0, 0], tx, ty)
+    # an oblique direction against the closed form evaluated here
+    d = _unit([0.3, -0.5, 0.8])
+    th = math.acos(float(np.float32(d[1]))) / math.pi
+    ph = (math.atan2(d[2], d[0]) + math.pi) / (2 * math.pi)
+    cam = dict(position=[0, 0, 0], x_axis=[0, 0, 0], y_axis=[0, 0, 0], z_axis=list(d))
+    img = S.render(cam, 1, 1, 0, 1, depth=1, precision=64)
+    assert img[0, 0, 0] == int((W - 1) * ph) + 1000 * int((H - 1) * th)
+
+
+# --------------------------------------------------------------------------- gradient domain, volumes, roulette
+
+
+def test_gdpt_furnace_has_zero_gradients():
+    """gdpt_kernel (GradientDomain.cu:160-246) in a closed emissive Lambertian box: the base path
+    and its four shifted neighbours each collect E sum_k rho'^k, so every gradient buffer is 0 and
+    the base is the furnace's partial sum (border pixels are skipped and stay 0). A point light of
+    zero intensity keeps the NEE branch defined."""
+    rho, E, depth = 0.6, 0.5, 5
+    desc = _emissive_box(rho, E)
+    desc["lights"] = np.array([[0, 0, 0, 0, 0, 0, 0] + [0] * 9], np.float32)
+    S = oracle.OracleScene(desc)
+    cam = dict(position=[0, 0, 0], x_axis=[1, 0, 0], y_axis=[0, 1, 0], z_axis=[0, 0, 1])
+    w = h = 16
+    bufs = S.gdpt(cam, w, h, 0, 2, depth=depth, precision=64)
+    rp = float(np.float32(rho)) * float(np.float32(3.14159)) / float(np.float32(math.pi))
+    full = float(np.float32(E)) * sum(rp ** k for k in range(depth))
+    inner = (slice(1, h - 1), slice(1, w - 1))
+    assert np.mean(np.abs(bufs[0][inner] - full) < 1e-9) > 0.97
+    for k in range(1, 5):  # zero up to the FP64 rounding of each path's throughput products
+        assert np.mean(np.abs(bufs[k][inner]) < 1e-12) > 0.97
+    for b in bufs:
+        assert (b[0] == 0).all() and (b[-1] == 0).all() and (b[:, 0] == 0).all() and (b[:, -1] == 0).all()
+
+
+def test_gdpt_gradients_at_an_emitter_edge():
+    """Half of the view is an emissive quad whose edge lies on a pixel boundary (depth 1, so each
+    path collects E or 0). gradient_x_forward = (base - left)/2 is E/2 in the first column on the
+    quad and 0 elsewhere; gradient_x_backward = (right - base)/2 is 0 everywhere, also in the last
+    column off the quad, because a shifted path stops where its base path found no vertex (the
+    reversibility rule, GradientDomain.cu:126). Vertical gradients are 0."""
+    E = 2.0
+    b = scenes.SceneBuilder()
+    b.quad((50.0, 0.0, 1.0), (0, 0, -1), (50.0, 0, 0), (0, 50.0, 0),
+           scenes.material(scenes.LAMBERT, albedo_e=(E,) * 3, albedo_d=(0.5,) * 3))
+    b.point_light((0, 0, -5), (0, 0, 0))
+    S = oracle.OracleScene(b.desc())
+    w, h = 16, 8
+    cam = dict(position=[0, 0, 0], x_axis=[1.0, 0, 0], y_axis=[0, 0.5, 0], z_axis=[0, 0, 1])
+    base, gxf, gxb, gyf, gyb = S.gdpt(cam, w, h, 0, 4, depth=1, precision=64)
+    c = w // 2  # first column whose rays (rx >= 0) hit the quad
+    inner = slice(1, h - 1)
+    np.testing.assert_array_equal(base[inner, c:w - 1], E)
+    np.testing.assert_array_equal(base[inner, 1:c], 0.0)
+    np.testing.assert_array_equal(gxf[inner, c], E / 2)
+    gxf2 = gxf.copy()
+    gxf2[inner, c] = 0
+    assert (gxf2 == 0).all() and (gxb == 0).all() and (gyf == 0).all() and (gyb == 0).all()
+
+
+def test_volume_single_scattering_in_the_scene_medium():
+    """Scene medium (VolumeRenderer.cu:88-211), depth 1: a free flight t ~ sigma_t exp(-sigma_t t)
+    ends in a scattering event with probability sigma_s/sigma_t (weight 1 for a grey medium), whose
+    NEE adds I exp(-sigma_t r) / r^2 for a point light at distance r; the scene-medium NEE carries
+    no phase function (':Phase/pdf = 1', :160-171). A black wall ends the other flights. So
+    E[L] = int_0^D sigma_s exp(-sigma_t t) I exp(-sigma_t r(t)) / r(t)^2 dt, by quadrature."""
+    from scipy.integrate import quad
+    ss, sa, D, I = 0.3, 0.2, 4.0, 2.0
+    Lp = np.array([0.6, 0.8, 2.0])
+    b = scenes.SceneBuilder()
+    b.plane((0, 0, D), (0, 0, -1), scenes.material(scenes.LAMBERT, albedo_d=(0, 0, 0)))
+    b.point_light(tuple(Lp), (I, I, I))
+    b.volume = np.array([ss, ss, ss, sa, sa, sa, 0.3], np.float32)
+    S = oracle.OracleScene(b.desc())
+    cam = dict(position=[0, 0, 0], x_axis=[1e-6, 0, 0], y_axis=[0, 1e-6, 0], z_axis=[0, 0, 1])
+    n = 40000
+    _, s, s2 = S.render(cam, 2, 2, 0, n, depth=1, method=oracle.VOLUME, precision=64, stats=True)
+    mean = s[..., 0].sum() / (4 * n)
+    var = (s2[..., 0].sum() / (4 * n) - mean ** 2) / (4 * n)
+    f32 = lambda x: float(np.float32(x))
+    st = f32(ss) + f32(sa)
+
+    def integrand(t):
+        r = math.sqrt(Lp[0] ** 2 + Lp[1] ** 2 + (Lp[2] - t) ** 2)
+        return f32(ss) * math.exp(-st * t) * I * math.exp(-st * r) / r ** 2
+
+    want = quad(integrand, 0, D, points=[Lp[2]], limit=200)[0]
+    assert abs(mean - want) < 4 * math.sqrt(var), (mean, want, math.sqrt(var))
+
+
+@pytest.mark.parametrize("interface", [0, 1])
+def test_volume_sphere_interior_and_glass_interface(interface):
+    """A homogeneous absorbing sphere (VOLUME material) seen along its diameter against a constant
+    environment of 1: inside, the ray is traced against the sphere alone (Tracing::traceRay(scene,
+    ray, object_index), VolumeRenderer.cu:242-249) and survives the chord 2R with probability
+    T = exp(-sigma_t 2R). Without an interface the pixel is T. With a glass interface (IOR 1.5,
+    normal incidence: Schlick reflectance R0 = 0.04, MaterialDetail.h:143-169) the path is reflected
+    at each surface with probability R0 and bounces inside until it leaves, so
+    E = R0 + (1 - R0)^2 T / (1 - R0 T) (depth 30 leaves (R0 T)^14 ~ 1e-20 out)."""
+    R, sa = 1.0, 0.35
+    b = scenes.SceneBuilder()
+    b.sphere((0, 0, 3.0), R, scenes.material(scenes.VOLUME, sigma_s=(0, 0, 0), sigma_a=(sa,) * 3, eta=1.5,
+                                              interface=interface))
+    b.env = np.ones((8, 16, 3), np.float32)
+    b.volume = np.array([0, 0, 0, 0, 0, 0, 0], np.float32)
+    S = oracle.OracleScene(b.desc())
+    cam = dict(position=[0, 0, 0], x_axis=[1e-7, 0, 0], y_axis=[0, 1e-7, 0], z_axis=[0, 0, 1])
+    n = 20000
+    _, s, s2 = S.render(cam, 2, 2, 0, n, depth=30, method=oracle.VOLUME, precision=64, stats=True)
+    mean = s[..., 0].sum() / (4 * n)
+    var = max(s2[..., 0].sum() / (4 * n) - mean ** 2, 1e-12) / (4 * n)
+    T = math.exp(-float(np.float32(sa)) * 2 * R)
+    R0 = ((1.5 - 1) / (1.5 + 1)) ** 2
+    want = T if interface == 0 else R0 + (1 - R0) ** 2 * T / (1 - R0 * T)
+    assert abs(mean - want) < 4 * math.sqrt(var) + 1e-6, (mean, want, math.sqrt(var))
+
+
+def test_russian_roulette_keeps_the_furnace_mean():
+    """Russian roulette (PathTracer.cu:141-152) kills a path with probability 1 - alpha,
+    alpha = clamp(max weight, 0, 1), and divides the survivors by alpha: unbiased, so the
+    closed-box furnace mean stays E sum_k rho'^k (within 4 sigma of the per-sample spread) while
+    single samples now differ from it."""
+    rho, E, depth = 0.6, 0.5, 5
+    S = oracle.OracleScene(_emissive_box(rho, E))
+    cam = dict(position=[0, 0, 0], x_axis=[1, 0, 0], y_axis=[0, 1, 0], z_axis=[0, 0, 1])
+    n = 400
+    _, s, s2 = S.render(cam, 16, 16, 0, n, depth=depth, rr=True, precision=64, stats=True)
+    rp = float(np.float32(rho)) * float(np.float32(3.14159)) / float(np.float32(math.pi))
+    full = float(np.float32(E)) * sum(rp ** k for k in range(depth))
+    m = s[..., 0].sum() / (256 * n)
+    v = (s2[..., 0].sum() / (256 * n) - m ** 2) / (256 * n)
+    assert v > 0  # roulette makes the samples random
+    assert abs(m - full) < 4 * math.sqrt(v), (m, full, math.sqrt(v))
