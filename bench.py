@@ -7,14 +7,22 @@ by bloom (threshold 1, knee 1) and the Reinhard tone map, all on the GPU with
 the scene resident in HBM. Prints one JSON line:
 
   value        Msamples/s = 1920*1080*64*steps / timed seconds (samples = camera paths)
-  roofline     the trace kernel (closest hit for extension rays + any hit for shadow rays,
-               one launch per bounce), the dominant stage: algorithmic bytes per launch =
-               BVH4 nodes visited x 128 B + triangles tested x 48 B + 40 B (extension) or
-               48 B (shadow) of queue traffic per ray (visit counts from an instrumented
-               pass), divided by the kernel's average launch time measured with HIP events
-               on the renderer's stream during the timed steps
+  value        Msamples/s = 1920*1080*64 / median step time (ms_per_step is the mean)
+  roofline     the trace kernels (closest hit for extension rays + any hit for shadow rays,
+               one launch per bounce), the dominant stage, against the VALU issue ceiling
+               they saturate: achieved = VALU instructions of a step's trace launches x 64
+               lanes x lane utilisation (rocprofv3 counters SQ_INSTS_VALU,
+               SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU), from a child run of the same
+               step) / the trace launches' time (HIP events on the renderer's stream during
+               the timed steps); peak = 157.3 TFLOP/s FP32 / 4 = 39.3 T lane-instructions/s;
+               traffic = HBM bytes per trace launch (FETCH_SIZE + WRITE_SIZE, two more child
+               runs, scaled by what the counters read for a copy of known size); the
+               algorithmic bytes (nodes x 128 B + triangles x 48 B + queue records, mostly
+               served by L2 and the MALL) are reported beside them
   cpu_baseline the CPU oracle (the reference estimator restated in C++, SAH BVH, FP32)
-               timed on this host on a bounded sample of the same workload
+               timed on this host on a bounded sample of the same workload, on this
+               process's CPU share (affinity mask, bounded by a cgroup quota), with the CPU
+               model and SMT state
 """
 from __future__ import annotations
 
@@ -48,6 +56,10 @@ def main():
                          "1-spp frames with the neural radiance cache against plain PT on the 1080p/1M-triangle scene; "
                          "gdpt: gradient-domain frame (5 paths per sample + 50 reconstruction iterations) of the "
                          "Cornell box at 512x512, 128 spp; volume: the homogeneous-media renderer at 512x512, 64 spp")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the three rocprofv3 child runs that measure the trace kernels' VALU and HBM counters")
+    ap.add_argument("--counter-dir", default=None, help="directory for the rocprofv3 child runs' output")
+    ap.add_argument("--counters-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run three device-to-device copies of MIB MiB (a plain copy kernel of known "
                          "size, for scaling the HBM byte counters of a --pmc run)")
@@ -65,6 +77,8 @@ def main():
     from cupbr_amd import scenes
 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    if args.counters_child:
+        return counters_child(args)
     if args.copy_calibration > 0:
         import torch
 
@@ -82,14 +96,223 @@ def main():
         return bench_nrc(args, log)
     if args.workload in ("gdpt", "volume"):
         return bench_integrator(args, log)
+    return bench_pt(args, log)
+
+
+# --------------------------------------------------------------------------- host CPU description
+def host_cpu():
+    """The host's CPU model, SMT state and the CPU share of this process: the CPUs in its affinity
+    mask, bounded by its cgroup CPU quota where one is set (the share a worker pool should use)."""
+    info = {"logical_cpus": os.cpu_count() or 1}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["affinity_cpus"] = info["logical_cpus"]
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpu_quota"] = quota
+    model, siblings, cores = None, None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "siblings" and siblings is None:
+                    siblings = int(v)
+                elif k == "cpu cores" and cores is None:
+                    cores = int(v)
+    except OSError:
+        pass
+    info["model"] = model
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        if siblings and cores:
+            smt = siblings > cores
+    info["smt"] = smt
+    info["threads_per_core"] = (siblings // cores) if siblings and cores else None
+    info["threads_used"] = min(info["affinity_cpus"], quota) if quota else info["affinity_cpus"]
+    return info
+
+
+def cpu_baseline_record(value, cpu, sample):
+    return {"value": round(value, 4), "unit": "Msamples/s", "cores": cpu["threads_used"], "kind": "port",
+            "sample": sample, "cpu_model": cpu["model"], "smt": cpu["smt"],
+            "threads_per_core": cpu["threads_per_core"], "logical_cpus": cpu["logical_cpus"],
+            "affinity_cpus": cpu["affinity_cpus"], "cgroup_cpu_quota": cpu["cgroup_cpu_quota"]}
+
+
+# --------------------------------------------------------------------------- counters (rocprofv3 children)
+VALU_COUNTERS = ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVES", "GRBM_GUI_ACTIVE"]
+VALU_FMA_ITERS = 20000
+COPY_MIB = 256
+# VALU issue ceiling from the published FP32 vector peak: 157.3 TFLOP/s counts packed FMAs (2 lanes
+# x 2 flops per lane-instruction); this code is built without packed FP32, so one VALU instruction
+# on one lane is one op: 157.3 / 4 = 39.3 T lane-instructions/s (256 CUs x 4 SIMDs x 64 lanes per
+# 4 cycles x 2.4 GHz)
+VALU_PEAK_TOPS = 157.3 / 4
+
+
+def _read_counter_csv(d):
+    """Rows of a rocprofv3 --output-format csv counter collection under directory d, as
+    [(dispatch, kernel, counter, value)], in dispatch order."""
+    import csv
+    import glob
+
+    files = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
+    rows = []
+    for fn in files:
+        with open(fn, newline="") as f:
+            for r in csv.DictReader(f):
+                disp = r.get("Dispatch_Id") or r.get("Dispatch_ID") or r.get("Correlation_Id") or "0"
+                rows.append((int(float(disp)), r.get("Kernel_Name", ""), r.get("Counter_Name", ""),
+                             float(r.get("Counter_Value", "nan"))))
+    rows.sort(key=lambda x: x[0])
+    return rows
+
+
+def _per_dispatch(rows):
+    out = {}
+    for disp, kern, cname, val in rows:
+        e = out.setdefault(disp, {"kernel": kern})
+        e[cname] = e.get(cname, 0.0) + val
+    return [out[k] for k in sorted(out)]
+
+
+def _run_counter_child(args, counters, log, copy=False):
+    """rocprofv3 --pmc <counters> over `bench.py --counters-child` (one step of the same workload, the
+    VALU microkernel and, with copy, a copy kernel of known size); returns per-dispatch counters."""
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        log("counters: rocprofv3 not found")
+        return None
+    out_dir = tempfile.mkdtemp(prefix="cupbr_pmc_", dir=args.counter_dir or None)
+    cmd = [prof, "--pmc", *counters, "--output-format", "csv", "-d", out_dir, "-o", "c", "--", sys.executable,
+           os.path.join(ROOT, "bench.py"), "--counters-child", "--spp", str(args.spp), "--width", str(args.width),
+           "--height", str(args.height), "--depth", str(args.depth)] + (["--copy-calibration", str(COPY_MIB)] if copy else [])
+    env = dict(os.environ, TMPDIR="/tmp")
+    t0 = time.time()
+    p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, env=env, start_new_session=True)
+    try:
+        text = p.communicate(timeout=240)[0].decode(errors="replace")
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.wait()
+        log(f"counters {counters}: rocprofv3 timed out")
+        return None
+    rows = _read_counter_csv(out_dir)
+    log(f"counters {' '.join(counters)}: rocprofv3 exit {p.returncode}, {len(rows)} rows in {time.time() - t0:.0f}s")
+    if not rows:
+        log(text[-2000:])
+        return None
+    return _per_dispatch(rows)
+
+
+def measure_counters(args, log):
+    """VALU and HBM counters of the trace kernels from three rocprofv3 runs of one step each."""
+    res = {}
+    valu = _run_counter_child(args, VALU_COUNTERS, log)
+    fetch = _run_counter_child(args, ["FETCH_SIZE"], log, copy=True)
+    write = _run_counter_child(args, ["WRITE_SIZE"], log, copy=True)
+    is_trace = lambda k: "k_pt_trace_camera" in k or "k_pt_trace<false>" in k  # noqa: E731
+    if valu:
+        tr = [d for d in valu if is_trace(d["kernel"])]
+        mk = [d for d in valu if "k_valu_peak" in d["kernel"]]
+        if tr:
+            res["trace_dispatches"] = len(tr)
+            res["valu_instructions"] = sum(d.get("SQ_INSTS_VALU", 0.0) for d in tr)
+            act = sum(d.get("SQ_ACTIVE_INST_VALU", 0.0) for d in tr)
+            res["lane_utilisation"] = sum(d.get("SQ_THREAD_CYCLES_VALU", 0.0) for d in tr) / max(64 * act, 1.0)
+            res["per_dispatch"] = [{"kernel": "k_pt_trace_camera" if "camera" in d["kernel"] else "k_pt_trace<false>",
+                                    "valu_instructions": d.get("SQ_INSTS_VALU", 0.0),
+                                    "lane_utilisation": d.get("SQ_THREAD_CYCLES_VALU", 0.0) /
+                                    max(64 * d.get("SQ_ACTIVE_INST_VALU", 0.0), 1.0),
+                                    "grbm_gui_active": d.get("GRBM_GUI_ACTIVE", 0.0)} for d in tr]
+        if mk:
+            d = mk[-1]
+            res["micro_valu_instructions"] = d.get("SQ_INSTS_VALU", 0.0)
+            res["micro_lane_utilisation"] = d.get("SQ_THREAD_CYCLES_VALU", 0.0) / max(64 * d.get("SQ_ACTIVE_INST_VALU", 0.0), 1.0)
+
+    def bytes_of(disp, name):
+        if not disp:
+            return None, None
+        first = next((i for i, d in enumerate(disp) if "cupbr" in d["kernel"]), len(disp))
+        cal = sorted(d.get(name, 0.0) for d in disp[:first] if d.get(name, 0.0) > 1000)
+        scale = (COPY_MIB * 1024.0) / cal[len(cal) // 2] if cal else None
+        tr = [d.get(name, 0.0) for d in disp if is_trace(d["kernel"])]
+        if not tr or scale is None:
+            return None, scale
+        return sum(tr) * 1024.0 * scale / len(tr), scale
+
+    res["fetch_bytes_per_launch"], res["fetch_scale"] = bytes_of(fetch, "FETCH_SIZE")
+    res["write_bytes_per_launch"], res["write_scale"] = bytes_of(write, "WRITE_SIZE")
+    return res
+
+
+def counters_child(args):
+    """The work profiled by measure_counters: the VALU microkernel, an optional copy of known size,
+    then one step of the headline workload (no warm-up, no counting pass)."""
+    import cupbr_amd as cb
+    from cupbr_amd import scenes
+
+    cb.measure_valu_peak(VALU_FMA_ITERS)
+    if args.copy_calibration > 0:
+        import torch
+
+        n = args.copy_calibration << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+    W, H = args.width, args.height
+    desc, ca = scenes.million_triangles(W, H)
+    scene = cb.Scene.from_desc(desc)
+    cam = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], W, H)
+    r = cb.PBRenderer(cb.RenderingMethod.PATHTRACER)
+    r.set_output_size(W, H)
+    r.register_scene(scene)
+    r.set_max_trace_depth(args.depth)
+    post = cb.PostProcessor()
+    post.register_renderer(r)
+    tm = cb.ToneMapper(cb.ToneMappingType.REINHARD)
+    tm.register_renderer(r)
+    r.render(cam, args.spp)
+    r.synchronize()
+    tm.tone_map_bloom(post, cb.threshold_curve(1.0, 1.0))
+    tm.get_render_buffer()
+
+
+# --------------------------------------------------------------------------- headline workload
+def bench_pt(args, log):
+    import numpy as np
+    import torch
+
+    import cupbr_amd as cb
+    from cupbr_amd import scenes
+
     W, H, SPP = args.width, args.height, args.spp
     t0 = time.time()
     desc, cam_args = scenes.million_triangles(W, H)
     t_gen = time.time() - t0
-    t0 = time.time()
     scene = cb.Scene.from_desc(desc)
     scene.upload()
-    t_build = time.time() - t0
     bvh = scene.bvh_stats()
     log(f"scene: {len(desc['tris'])} triangles, {len(desc['types'])} objects; generated in {t_gen:.1f}s, "
         f"BVH4 build {bvh['build_ms']:.0f} ms + upload {bvh['upload_ms']:.0f} ms ({bvh['nodes']} nodes, depth {bvh['depth']})")
@@ -124,39 +347,45 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    r.synchronize()
+    torch.cuda.synchronize()
     r.set_profiling(True)
     stage_ms = np.zeros(4)
     stage_n = np.zeros(4)
     rays = np.zeros(2)
+    step_s = []
     t_start = time.perf_counter()
     for _ in range(args.steps):
+        t0 = time.perf_counter()
         step()
-        st = r.stats()  # synchronizes the renderer stream
+        torch.cuda.synchronize()
+        step_s.append(time.perf_counter() - t0)
+        st = r.stats()
         stage_ms += st["stage_ms"]
         stage_n += st["stage_launches"]
         rays += (st["extension_rays"], st["shadow_rays"])
-    tm_buf = tm.get_render_buffer()  # synchronizes the post chain
+    torch.cuda.synchronize()
     t_total = time.perf_counter() - t_start
     r.set_profiling(False)
+    tm_buf = tm.get_render_buffer()
 
-    samples = W * H * SPP * args.steps
-    value = samples / t_total / 1e6
-    ms_per_step = t_total / args.steps * 1e3
-    ext_launch_ms = stage_ms[0] / max(stage_n[0], 1)
-    fused = stage_n[2] == 0  # one trace kernel per bounce takes extension and shadow rays
-    ext_bytes_per_launch = (bytes_per_ext_ray * rays[0] + (bytes_per_sh_ray * rays[1] if fused else 0.0)) / max(stage_n[0], 1)
-    achieved = ext_bytes_per_launch / (ext_launch_ms * 1e-3) / 1e9
+    samples_per_step = W * H * SPP
+    t_med = float(np.median(step_s))
+    value = samples_per_step / t_med / 1e6  # median step (BASELINE: median of the timed runs)
+    value_mean = samples_per_step * args.steps / t_total / 1e6
+    trace_launch_ms = stage_ms[0] / max(stage_n[0], 1)
+    trace_ms_per_step = stage_ms[0] / args.steps
+    alg_bytes_per_launch = (bytes_per_ext_ray * rays[0] + bytes_per_sh_ray * rays[1]) / max(stage_n[0], 1)
+    alg_gbs = alg_bytes_per_launch / (trace_launch_ms * 1e-3) / 1e9
     img = r.get_output_image()
-    log(f"GPU: {value:.1f} Msamples/s, {ms_per_step:.1f} ms/step, {(rays.sum()) / t_total / 1e6:.0f} Mrays/s "
-        f"(ext {rays[0] / args.steps / 1e6:.1f}M + shadow {rays[1] / args.steps / 1e6:.1f}M per step); stage ms/step "
-        f"trace {stage_ms[0] / args.steps:.1f} shade {stage_ms[1] / args.steps:.1f} "
-        f"other {stage_ms[3] / args.steps:.1f}; image mean {img.mean():.4f} finite {bool(np.isfinite(img).all())}; "
-        f"rgba mean {tm_buf[..., :3].mean():.1f}")
+    log(f"GPU: {value:.1f} Msamples/s (median step {t_med * 1e3:.1f} ms; mean over the timed steps {value_mean:.1f}), "
+        f"{(rays.sum()) / t_total / 1e6:.0f} Mrays/s (ext {rays[0] / args.steps / 1e6:.1f}M + shadow "
+        f"{rays[1] / args.steps / 1e6:.1f}M per step); stage ms/step trace {trace_ms_per_step:.1f} shade "
+        f"{stage_ms[1] / args.steps:.1f} other {stage_ms[3] / args.steps:.1f}; image mean {img.mean():.4f} finite "
+        f"{bool(np.isfinite(img).all())}; rgba mean {tm_buf[..., :3].mean():.1f}")
     log(f"extend: {bytes_per_ext_ray:.0f} algorithmic B/ray ({sc['extension_node_visits'] / max(ext_rays, 1):.1f} nodes, "
         f"{sc['extension_prim_tests'] / max(ext_rays, 1):.1f} tris per ray); shadow {bytes_per_sh_ray:.0f} B/ray "
         f"({sc['shadow_node_visits'] / max(sh_rays, 1):.1f} nodes, {sc['shadow_prim_tests'] / max(sh_rays, 1):.1f} tris); "
-        f"{'trace' if fused else 'extend'} kernel {ext_launch_ms:.3f} ms/launch, {achieved:.0f} GB/s")
+        f"trace kernel {trace_launch_ms:.3f} ms/launch, {alg_gbs:.0f} GB/s of algorithmic bytes (L2/MALL-served)")
 
     # not part of `value`: reading the 1080p FP32 image back to the host over PCIe
     t0 = time.perf_counter()
@@ -164,19 +393,60 @@ def main():
         r.get_output_image()
     t_read = (time.perf_counter() - t0) / 5
     log(f"host read-back of the {W}x{H} FP32 image: {t_read * 1e3:.2f} ms ({W * H * 12 / t_read / 1e9:.1f} GB/s); "
-        f"with one read-back per step the rate would be {samples / (t_total + args.steps * t_read) / 1e6:.1f} Msamples/s")
+        f"with one read-back per step the rate would be {samples_per_step / (t_med + t_read) / 1e6:.1f} Msamples/s")
 
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "trace_traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    # VALU ceiling: the microkernel timed here (unprofiled); its instruction count is checked by the
+    # counters below
+    micro = cb.measure_valu_peak(VALU_FMA_ITERS)
+    log(f"VALU microkernel: {micro['wave_instructions'] / 1e9:.2f} G wave64 FMAs in {micro['ms']:.3f} ms = "
+        f"{micro['wave_instructions_per_s'] * 64 / 1e12:.2f} T lane-instructions/s")
+    del r, post, tm
+
+    roof = {"bound": "valu", "achieved": None, "peak": round(VALU_PEAK_TOPS, 3), "unit": "TOP/s", "frac": None,
+            "traffic": None,
+            "kernel": "k_pt_trace_camera + k_pt_trace<false> (all trace launches of a step: bounce 0 camera rays as "
+                      "wave packets, then per bounce extension closest hit + previous bounce's shadow any hit, BVH4)",
+            "op": "one VALU instruction on one active lane",
+            "avg_launch_ms": round(trace_launch_ms, 4), "trace_ms_per_step": round(trace_ms_per_step, 3),
+            "micro_valu_tops": round(micro["wave_instructions_per_s"] * 64 / 1e12, 3),
+            "algorithmic_bytes_per_launch": int(alg_bytes_per_launch), "algorithmic_gbs": round(alg_gbs, 1),
+            "bytes_per_ext_ray": round(bytes_per_ext_ray, 1), "bytes_per_shadow_ray": round(bytes_per_sh_ray, 1)}
+    if not args.no_counters:
+        c = measure_counters(args, log)
+        if c.get("valu_instructions"):
+            t_tr = trace_ms_per_step * 1e-3  # the child's dispatches are one step's trace launches
+            issue = c["valu_instructions"] / t_tr  # wave instructions per second
+            lane = c["lane_utilisation"]
+            achieved = issue * 64 * lane / 1e12
+            roof.update({"achieved": round(achieved, 3), "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                         "lane_utilisation": round(lane, 4),
+                         "valu_issue_of_peak": round(issue * 64 / 1e12 / VALU_PEAK_TOPS, 4),
+                         "valu_instructions_per_step": int(c["valu_instructions"]),
+                         "trace_dispatches": c["trace_dispatches"]})
+            if c.get("micro_valu_instructions"):
+                roof["micro_counted_instructions"] = int(c["micro_valu_instructions"])
+                roof["micro_lane_utilisation"] = round(c["micro_lane_utilisation"], 4)
+                rate = c["micro_valu_instructions"] / (micro["ms"] * 1e-3)
+                roof["micro_valu_tops"] = round(rate * 64 / 1e12, 3)
+                roof["valu_issue_of_micro"] = round(issue / rate, 4)
+            log(f"trace VALU: {c['valu_instructions'] / 1e9:.1f} G wave instructions per step, lane utilisation "
+                f"{lane:.3f}, issue {issue * 64 / 1e12:.2f} T lane-slots/s = {issue * 64 / 1e12 / VALU_PEAK_TOPS:.3f} "
+                f"of peak; useful {achieved:.2f} TOP/s = {achieved / VALU_PEAK_TOPS:.3f} of peak")
+        if c.get("fetch_bytes_per_launch") is not None and c.get("write_bytes_per_launch") is not None:
+            traffic = c["fetch_bytes_per_launch"] + c["write_bytes_per_launch"]
+            hbm_gbs = traffic / (trace_launch_ms * 1e-3) / 1e9
+            roof.update({"traffic": int(traffic), "hbm_gbs": round(hbm_gbs, 1),
+                         "hbm_of_peak": round(hbm_gbs / HBM_PEAK_GBS, 4),
+                         "fetch_scale": round(c["fetch_scale"], 4), "write_scale": round(c["write_scale"], 4)})
+            log(f"trace HBM traffic: {traffic / 1e9:.2f} GB per launch (FETCH_SIZE x {c['fetch_scale']:.3f} + "
+                f"WRITE_SIZE x {c['write_scale']:.3f}) = {hbm_gbs:.0f} GB/s = {hbm_gbs / HBM_PEAK_GBS:.3f} of HBM peak")
 
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
 
-        threads = min(16, os.cpu_count() or 1)
+        hc = host_cpu()
+        threads = hc["threads_used"]
         t0 = time.time()
         ref = oracle.OracleScene(desc, precisions=(32,))
         t_ref_build = time.time() - t0
@@ -184,11 +454,13 @@ def main():
         ref.render(cam.as_dict(), W, H, 0, args.cpu_spp, depth=args.depth, precision=32, mode=2, threads=threads)
         t_cpu = time.perf_counter() - t0
         cpu_value = W * H * args.cpu_spp / t_cpu / 1e6
-        cpu = {"value": round(cpu_value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_spp} spp of the same 1920x1080 frame ({W * H * args.cpu_spp} paths, depth "
-                         f"{args.depth}) on the C++ oracle with its own SAH BVH, FP32, {threads} threads, "
-                         f"{t_cpu:.1f}s (scene build {t_ref_build:.1f}s not timed)"}
-        log(f"CPU oracle: {cpu_value:.3f} Msamples/s on {threads} threads ({t_cpu:.1f}s); GPU/CPU = {value / cpu_value:.0f}x")
+        cpu = cpu_baseline_record(
+            cpu_value, hc, f"{args.cpu_spp} spp of the same {W}x{H} frame ({W * H * args.cpu_spp} paths, depth "
+                           f"{args.depth}) on the C++ oracle with its own SAH BVH, FP32, {threads} threads (this "
+                           f"process's CPU share), {t_cpu:.1f}s (scene build {t_ref_build:.1f}s not timed)")
+        log(f"CPU oracle: {cpu_value:.3f} Msamples/s on {threads} threads of {hc['model']} (logical CPUs "
+            f"{hc['logical_cpus']}, affinity {hc['affinity_cpus']}, quota {hc['cgroup_cpu_quota']}, SMT {hc['smt']}); "
+            f"GPU/CPU = {value / cpu_value:.0f}x")
 
     out = {
         "metric": "Msamples/s at 1080p/1M-tri",
@@ -197,7 +469,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(t_total / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -209,15 +481,12 @@ def main():
                         f"Reinhard tone map",
             "width": W, "height": H, "spp": SPP, "max_depth": args.depth, "triangles": int(len(desc["tris"])),
             "russian_roulette": False, "rng": "reference TEA<4> seed + LCG",
+            "value_from": "median step time", "median_step_ms": round(t_med * 1e3, 3),
+            "value_mean_msamples": round(value_mean, 3),
             "mrays_per_s": round(float(rays.sum()) / t_total / 1e6, 1),
             "bvh_build_ms": round(bvh["build_ms"], 1),
         },
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_pt_trace_camera + k_pt_trace (bounce 0: camera rays as wave packets; then per bounce extension closest hit + shadow any hit, BVH4)" if fused else
-                               "k_pt_extend (closest-hit BVH4 traversal)",
-                     "bytes_per_launch": int(ext_bytes_per_launch), "bytes_per_ext_ray": round(bytes_per_ext_ray, 1),
-                     "bytes_per_shadow_ray": round(bytes_per_sh_ray, 1), "avg_launch_ms": round(ext_launch_ms, 4)},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -290,7 +559,8 @@ def bench_grid(args, log):
     if not args.no_cpu_baseline:
         import oracle
 
-        threads = min(16, os.cpu_count() or 1)
+        hc = host_cpu()
+        threads = hc["threads_used"]
         S = oracle.OracleScene(G["desc"], precisions=(32,))
         sw = sh = 256  # the full frame
         cam_s = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], sw, sh)
@@ -303,9 +573,9 @@ def bench_grid(args, log):
                    depth=DEPTH, precision=32, threads=threads, d_image=2 * im)  # cotangent of sum I^2
         t_cpu = time.perf_counter() - t0
         cpu_value = sw * sh * SPP * reps / t_cpu / 1e6
-        cpu = {"value": round(cpu_value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
-               "sample": f"{reps} forward + adjoint evaluations of the same medium at {sw}x{sh}, {SPP} spp, "
-                         f"frames 0..{reps * SPP - 1}, on the C++ oracle (FP32), {threads} threads, {t_cpu:.1f}s"}
+        cpu = cpu_baseline_record(cpu_value, hc, f"{reps} forward + adjoint evaluations of the same medium at "
+                                                 f"{sw}x{sh}, {SPP} spp, frames 0..{reps * SPP - 1}, on the C++ oracle "
+                                                 f"(FP32), {threads} threads, {t_cpu:.1f}s")
         log(f"CPU oracle: {cpu_value:.3f} Msamples/s on {threads} threads; GPU/CPU = {value / cpu_value:.0f}x")
 
     out = {
@@ -524,7 +794,8 @@ def bench_integrator(args, log):
         f"{bytes_step / max(rays, 1):.0f} algorithmic B/ray")
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        hc = host_cpu()
+        threads = hc["threads_used"]
         ref = oracle.OracleScene(desc, precisions=(32,))
         cs = 128 if args.workload == "gdpt" else 640  # ~10 s of CPU work on 16 threads
         t0 = time.perf_counter()
@@ -535,8 +806,8 @@ def bench_integrator(args, log):
             ref.render(cam.as_dict(), W, H, 0, cs, depth=depth, method=oracle.VOLUME, precision=32, threads=threads)
         t_cpu = time.perf_counter() - t0
         cpu_value = W * H * cs / t_cpu / 1e6
-        cpu = {"value": round(cpu_value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
-               "sample": f"{cs} spp of the same {W}x{H} frame on the C++ oracle (FP32), {threads} threads, {t_cpu:.1f}s"}
+        cpu = cpu_baseline_record(cpu_value, hc, f"{cs} spp of the same {W}x{H} frame on the C++ oracle (FP32), "
+                                                 f"{threads} threads, {t_cpu:.1f}s")
         log(f"CPU oracle: {cpu_value:.3f} Msamples/s on {threads} threads; GPU/CPU = {value / cpu_value:.0f}x")
     out = {
         "metric": f"Msamples/s ({args.workload})",

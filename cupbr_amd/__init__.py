@@ -37,6 +37,9 @@ class CupbrError(RuntimeError):
         self.code = code
 
 
+GDPT_TRIMMED_MEAN, GDPT_SCREENED_POISSON = 0, 1
+
+
 class RenderingMethod(enum.IntEnum):  # PBRenderer.h:14-22
     RAYTRACER = 0
     WHITTED = 1
@@ -94,6 +97,18 @@ class _Camera(C.Structure):
                 ("z_axis", C.c_float * 3), ("moved", C.c_int32)]
 
 
+class _CameraController(C.Structure):
+    _fields_ = [("aspect_ratio", C.c_float), ("pitch", C.c_float), ("yaw", C.c_float), ("first_mouse", C.c_int32),
+                ("last_x", C.c_float), ("last_y", C.c_float)]
+
+
+class _InputState(C.Structure):
+    _fields_ = [("keys", C.c_uint32), ("cursor_x", C.c_double), ("cursor_y", C.c_double)]
+
+
+KEY_W, KEY_A, KEY_S, KEY_D, KEY_LEFT_SHIFT, KEY_LEFT_CONTROL = 1, 2, 4, 8, 16, 32
+
+
 class _BvhStats(C.Structure):
     _fields_ = [("nodes", C.c_uint32), ("leaves", C.c_uint32), ("primitives", C.c_uint32), ("unbounded", C.c_uint32),
                 ("depth", C.c_uint32), ("stack_entries", C.c_uint32), ("build_ms", C.c_double),
@@ -105,7 +120,8 @@ class _RenderStats(C.Structure):
                 ("milliseconds", C.c_double), ("stage_ms", C.c_double * 4), ("stage_launches", C.c_uint32 * 4),
                 ("extension_node_visits", C.c_uint64), ("extension_prim_tests", C.c_uint64),
                 ("shadow_node_visits", C.c_uint64), ("shadow_prim_tests", C.c_uint64),
-                ("cache_queries", C.c_uint64), ("cache_training_rows", C.c_uint64)]
+                ("cache_queries", C.c_uint64), ("cache_training_rows", C.c_uint64),
+                ("refill_launches", C.c_uint64)]
 
 
 class _GridMedium(C.Structure):
@@ -150,8 +166,11 @@ def lib():
         sig = {
             "cupbr_last_error": (C.c_char_p, []),
             "cupbr_device_name": (C.c_int, [C.c_char_p, SZ]),
+            "cupbr_measure_valu_peak": (C.c_int, [U32, P, P]),
             "cupbr_camera_init": (C.c_int, [P, U32, U32]),
             "cupbr_camera_look_at": (C.c_int, [P, P, P, P, C.c_float, U32, U32]),
+            "cupbr_camera_controller_init": (C.c_int, [P, U32, U32]),
+            "cupbr_camera_process_input": (C.c_int, [P, P, P, C.c_float]),
             "cupbr_scene_load_xml": (C.c_int, [C.c_char_p, P]),
             "cupbr_scene_create": (C.c_int, [P, U32, P, U32, P, U32, P, U32, U32, P, P]),
             "cupbr_scene_destroy": (C.c_int, [P]),
@@ -186,6 +205,7 @@ def lib():
             "cupbr_renderer_get_stats": (C.c_int, [P, P]),
             "cupbr_renderer_synchronize": (C.c_int, [P]),
             "cupbr_renderer_set_profiling": (C.c_int, [P, I32]),
+            "cupbr_renderer_set_gdpt_reconstruction": (C.c_int, [P, I32, C.c_float, U32]),
             "cupbr_renderer_set_traversal_counting": (C.c_int, [P, I32]),
             "cupbr_post_create": (C.c_int, [P]),
             "cupbr_post_destroy": (C.c_int, [P]),
@@ -249,6 +269,16 @@ def device_name() -> str:
     return buf.value.decode()
 
 
+def measure_valu_peak(iterations: int = 20000) -> dict:
+    """cupbr_measure_valu_peak: time and wave64 FMA instruction count of a VALU-only kernel with
+    every lane active (the device's VALU issue ceiling)."""
+    ms = C.c_double()
+    n = C.c_uint64()
+    _check(lib().cupbr_measure_valu_peak(iterations, C.byref(ms), C.byref(n)))
+    return {"ms": ms.value, "wave_instructions": n.value,
+            "wave_instructions_per_s": n.value / (ms.value * 1e-3) if ms.value > 0 else float("nan")}
+
+
 def threshold_curve(threshold: float = 1.0, knee: float = 1.0) -> np.ndarray:
     """Interactor::Impl::compute_threshold (lib/src/Interaction/Interactor.cpp:436-440)."""
     out = (C.c_float * 4)()
@@ -292,6 +322,26 @@ class Camera:
 
     def as_dict(self) -> dict:
         return {k: self._get(k) for k in ("position", "x_axis", "y_axis", "z_axis")}
+
+    def process_input(self, controller: "CameraController", keys: int = 0, cursor=(0.0, 0.0),
+                      delta_time: float = 0.0):
+        """Camera::processInput (Camera.cpp:13-90) for one frame: `keys` is a mask of KEY_* held
+        down, `cursor` the window's cursor position."""
+        st = _InputState(int(keys), float(cursor[0]), float(cursor[1]))
+        _check(lib().cupbr_camera_process_input(C.byref(self._c), C.byref(controller._c), C.byref(st),
+                                                float(delta_time)))
+
+
+class CameraController:
+    """The input state Camera keeps privately (Camera.h:78-85): aspect ratio, pitch, yaw, and the
+    last cursor position."""
+
+    def __init__(self, width: int, height: int):
+        self._c = _CameraController()
+        _check(lib().cupbr_camera_controller_init(C.byref(self._c), width, height))
+
+    yaw = property(lambda s: s._c.yaw)
+    pitch = property(lambda s: s._c.pitch)
 
 
 # ------------------------------------------------------------------ Scene
@@ -795,6 +845,11 @@ class PBRenderer:
         self._cache = cache
         _check(lib().cupbr_renderer_set_radiance_cache(self._h, None if cache is None else cache.handle,
                                                        training_paths, train_steps, batch, learning_rate))
+
+    def set_gdpt_reconstruction(self, mode: int = 0, alpha: float = 0.2, iterations: int = 50):
+        """GRADIENTDOMAIN reconstruction: GDPT_TRIMMED_MEAN (the reference's) or
+        GDPT_SCREENED_POISSON (L2, weight alpha on the primal image, Jacobi iterations)."""
+        _check(lib().cupbr_renderer_set_gdpt_reconstruction(self._h, int(mode), float(alpha), int(iterations)))
 
     def set_traversal_counting(self, enable: bool):
         """Counts BVH4 node visits and primitive tests (reported by stats())."""

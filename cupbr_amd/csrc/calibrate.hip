@@ -1,0 +1,59 @@
+// Measurement aid: the VALU issue ceiling of the device. The trace kernels are bound by VALU
+// instruction issue at partial lane occupancy, so their roofline is stated against the rate at
+// which the SIMDs retire wave64 VALU instructions; this kernel measures that rate with every lane
+// active (eight independent FP32 FMA chains per lane, eight waves per SIMD).
+#include "device.h"
+
+namespace cupbr {
+namespace {
+
+constexpr int kCalBlock = 256;
+
+__global__ __launch_bounds__(kCalBlock) void k_valu_peak(float* __restrict__ out, uint32_t iters, float m, float c) {
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    float a0 = (float)gtid, a1 = a0 + 1.0f, a2 = a0 + 2.0f, a3 = a0 + 3.0f;
+    float a4 = a0 + 4.0f, a5 = a0 + 5.0f, a6 = a0 + 6.0f, a7 = a0 + 7.0f;
+    for (uint32_t k = 0; k < iters; ++k) {
+        a0 = __builtin_fmaf(a0, m, c);
+        a1 = __builtin_fmaf(a1, m, c);
+        a2 = __builtin_fmaf(a2, m, c);
+        a3 = __builtin_fmaf(a3, m, c);
+        a4 = __builtin_fmaf(a4, m, c);
+        a5 = __builtin_fmaf(a5, m, c);
+        a6 = __builtin_fmaf(a6, m, c);
+        a7 = __builtin_fmaf(a7, m, c);
+    }
+    out[gtid] = ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
+}
+
+}  // namespace
+}  // namespace cupbr
+
+using namespace cupbr;
+
+extern "C" int cupbr_measure_valu_peak(uint32_t iterations, double* milliseconds, uint64_t* fma_instructions) {
+    if (!milliseconds || !fma_instructions || iterations == 0)
+        return set_error(CUPBR_ERR_INVALID_ARGUMENT, "null output or zero iterations");
+    const int blocks = device_cu_count() * 8;  // 32 waves per CU: 8 per SIMD
+    const size_t n = (size_t)blocks * kCalBlock;
+    DevBuf out;
+    if (int rc = out.ensure(n * sizeof(float))) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CUPBR_HIP_CHECK(hipEventCreate(&e0));
+    CUPBR_HIP_CHECK(hipEventCreate(&e1));
+    // one untimed launch, then the timed one
+    hipLaunchKernelGGL(k_valu_peak, dim3(blocks), dim3(kCalBlock), 0, nullptr, out.as<float>(), iterations, 0.999f,
+                       1e-3f);
+    CUPBR_HIP_CHECK(hipEventRecord(e0, nullptr));
+    hipLaunchKernelGGL(k_valu_peak, dim3(blocks), dim3(kCalBlock), 0, nullptr, out.as<float>(), iterations, 0.999f,
+                       1e-3f);
+    CUPBR_HIP_CHECK(hipEventRecord(e1, nullptr));
+    CUPBR_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    CUPBR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *milliseconds = ms;
+    *fma_instructions = (uint64_t)(n / 64) * iterations * 8;  // wave64 FMA instructions of the timed launch
+    return CUPBR_OK;
+}

@@ -98,6 +98,59 @@ int cupbr_camera_look_at(cupbr_camera* cam, const float eye[3], const float targ
     return CUPBR_OK;
 }
 
+int cupbr_camera_controller_init(cupbr_camera_controller* ctl, uint32_t width, uint32_t height) {
+    CHECK_ARG(ctl && height, "invalid controller arguments");
+    ctl->aspect_ratio = (float)width / (float)height;
+    ctl->pitch = 0.0f;
+    ctl->yaw = 3.14159f / 2.0f;
+    ctl->first_mouse = 1;
+    ctl->last_x = ctl->last_y = 0.0f;
+    return CUPBR_OK;
+}
+
+// Camera::processInput, Camera.cpp:13-90 (operation order kept: the axes must follow the
+// reference's for the same inputs)
+int cupbr_camera_process_input(cupbr_camera* cam, cupbr_camera_controller* ctl, const cupbr_input_state* in,
+                               float delta_time) {
+    CHECK_ARG(cam && ctl && in, "null argument");
+    bool moved = false;
+    float speed = 4.0f;
+    if (in->keys & CUPBR_KEY_LEFT_SHIFT) speed *= 4;
+    if (in->keys & CUPBR_KEY_LEFT_CONTROL) speed /= 4;
+    v3 pos = ld3(cam->position);
+    const v3 xa = ld3(cam->x_axis), za = ld3(cam->z_axis);
+    if (in->keys & CUPBR_KEY_D) { pos += speed * xa * delta_time; moved = true; }
+    if (in->keys & CUPBR_KEY_A) { pos = pos - speed * xa * delta_time; moved = true; }
+    if (in->keys & CUPBR_KEY_S) { pos = pos - speed * za * delta_time; moved = true; }
+    if (in->keys & CUPBR_KEY_W) { pos += speed * za * delta_time; moved = true; }
+    st3(cam->position, pos);
+    const float cx = (float)in->cursor_x, cy = (float)in->cursor_y;
+    if (ctl->first_mouse) {
+        ctl->last_x = cx;
+        ctl->last_y = cy;
+        ctl->first_mouse = 0;
+    }
+    const float sensitivity = 0.002f;
+    float xoff = ctl->last_x - cx, yoff = ctl->last_y - cy;
+    if (!(safe_float_equal(xoff, 0.0f) && safe_float_equal(yoff, 0.0f))) moved = true;
+    ctl->last_x = cx;
+    ctl->last_y = cy;
+    xoff *= sensitivity;
+    yoff *= sensitivity;
+    ctl->yaw += xoff;
+    ctl->pitch += yoff;
+    if (ctl->pitch > 3.14159f / 2.0f) ctl->pitch = 3.14159f / 2.0f;
+    if (ctl->pitch < -3.14159f / 2.0f) ctl->pitch = -3.14159f / 2.0f;
+    const v3 front = mk(std::cos(ctl->yaw) * std::cos(ctl->pitch), std::sin(ctl->pitch),
+                        std::sin(ctl->yaw) * std::cos(ctl->pitch));
+    const v3 xn = ctl->aspect_ratio * normalize(mk(front.z, 0.0f, -front.x));
+    st3(cam->z_axis, front);
+    st3(cam->x_axis, xn);
+    st3(cam->y_axis, cross(front, xn) / ctl->aspect_ratio);
+    cam->moved = moved ? 1 : 0;
+    return CUPBR_OK;
+}
+
 // ------------------------------------------------------------------ scene
 int cupbr_scene_load_xml(const char* path, cupbr_scene** out) {
     CHECK_ARG(path && out, "null argument");
@@ -474,6 +527,7 @@ int cupbr_renderer_get_stats(const cupbr_renderer* cr, cupbr_render_stats* out) 
         r->stats.extension_prim_tests = c.ext_prims;
         r->stats.shadow_node_visits = c.shadow_nodes;
         r->stats.shadow_prim_tests = c.shadow_prims;
+        r->stats.refill_launches = c.refill_launches;
     }
     r->stats.cache_queries = r->nrc_queries_total;
     r->stats.cache_training_rows = r->nrc_rows_total;
@@ -487,6 +541,19 @@ int cupbr_renderer_get_stats(const cupbr_renderer* cr, cupbr_render_stats* out) 
         r->stats.stage_launches[st] += 1;
     }
     *out = r->stats;
+    return CUPBR_OK;
+}
+
+int cupbr_renderer_set_gdpt_reconstruction(cupbr_renderer* r, int32_t mode, float alpha, uint32_t iterations) {
+    CHECK_ARG(r, "null renderer");
+    CHECK_ARG(mode == CUPBR_GDPT_TRIMMED_MEAN || mode == CUPBR_GDPT_SCREENED_POISSON, "unknown reconstruction mode");
+    if (mode == CUPBR_GDPT_SCREENED_POISSON) {
+        CHECK_ARG(std::isfinite(alpha) && alpha >= 0.0f, "alpha must be finite and >= 0");
+        CHECK_ARG(iterations <= (1u << 20), "at most 2^20 iterations");
+        r->gdpt_alpha = alpha;
+        r->gdpt_iterations = iterations;
+    }
+    r->gdpt_mode = mode;
     return CUPBR_OK;
 }
 

@@ -516,6 +516,7 @@ int grid_render_device(const cupbr_scene* cs, const cupbr_camera* cam, const cup
     const uint32_t n_px = w * h;
     const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(spp, kMaxGridSamples / n_px));
     if (s->grid_samples.ensure((size_t)n_px * chunk * 12)) return CUPBR_ERR_OUT_OF_MEMORY;
+    if (int rc = scene_stream_acquire(s, stream)) return rc;
     for (uint32_t s0 = 0; s0 < spp; s0 += chunk) {
         const uint32_t cnt = std::min(chunk, spp - s0);
         int blocks;
@@ -529,6 +530,7 @@ int grid_render_device(const cupbr_scene* cs, const cupbr_camera* cam, const cup
                            s->grid_samples.as<float>(), n_px, cnt, s0 == 0 ? 1 : 0, s0 + cnt == spp ? 1 : 0, spp);
         CUPBR_HIP_CHECK(hipGetLastError());
     }
+    if (int rc = scene_stream_release(s, stream)) return rc;
     // all samples of this call stay in grid_samples when they fit one chunk: an adjoint with the
     // same arguments may take the paths' radiance from there
     auto& f = s->grid_forward;
@@ -570,6 +572,7 @@ int grid_adjoint_device(const cupbr_scene* cs, const cupbr_camera* cam, const cu
     int blocks;
     uint32_t* spill;
     if (grid_launch_shape(s, (size_t)w * h * spp, &blocks, &spill)) return CUPBR_ERR_OUT_OF_MEMORY;
+    if (int rc = scene_stream_acquire(s, stream)) return rc;
     // private gradient copies (cleared by a kernel, not a memset, so that a captured graph replays it)
     const size_t n_vox = (size_t)m->nx * m->ny * m->nz;
     const size_t copy_floats = (n_vox + 3) & ~(size_t)3;  // 16-byte aligned copies
@@ -593,7 +596,7 @@ int grid_adjoint_device(const cupbr_scene* cs, const cupbr_camera* cam, const cu
                            stream, grad_rho, target, n_vox, copy_floats);
         CUPBR_HIP_CHECK(hipGetLastError());
     }
-    return CUPBR_OK;
+    return scene_stream_release(s, stream);
 }
 
 int grid_validate(const cupbr_scene* s, const cupbr_camera* cam, const cupbr_grid_medium* m, uint32_t w, uint32_t h,

@@ -425,6 +425,35 @@ __global__ void k_gdpt_opt(const float* __restrict__ recon, const float* __restr
                                 median4(v1.z, v2.z, v3_.z, v4.z)));
 }
 
+// Screened-Poisson (L2) reconstruction, one Jacobi iteration over the interior pixels (no
+// counterpart in the reference): (a2 + 4) I(p) = a2 B(p) + (I(L) + gx(L)) + (I(D) + gy(D)) +
+// (I(R) - gx(p)) + (I(U) - gy(p)), i.e. the minimiser of a2 |I - B|^2 + |dx I - gx|^2 + |dy I - gy|^2
+// with the gradients of init_kernel read as forward differences, as the trimmed mean reads them.
+// The border of out stays zero.
+__global__ void k_gdpt_l2(const float* __restrict__ recon, const float* __restrict__ base,
+                          const float* __restrict__ gx, const float* __restrict__ gy, float* __restrict__ out,
+                          uint32_t w, uint32_t h, float a2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t x = i % w, y = i / w;
+    if (x == 0 || y == 0 || x == w - 1 || y == h - 1) return;
+    const v3 v1 = ld3(recon + 3 * (size_t)(i - 1)) + ld3(gx + 3 * (size_t)(i - 1));
+    const v3 v2 = ld3(recon + 3 * (size_t)(i - w)) + ld3(gy + 3 * (size_t)(i - w));
+    const v3 v3_ = ld3(recon + 3 * (size_t)(i + 1)) - ld3(gx + 3 * (size_t)i);
+    const v3 v4 = ld3(recon + 3 * (size_t)(i + w)) - ld3(gy + 3 * (size_t)i);
+    const v3 b = ld3(base + 3 * (size_t)i);
+    const float den = a2 + 4.0f;
+    st3(out + 3 * (size_t)i, (a2 * b + ((v1 + v2) + (v3_ + v4))) / den);
+}
+
+// zero border of an image (the trimmed-mean and L2 iterates keep a zero border)
+__global__ void k_zero_border(float* img, uint32_t w, uint32_t h) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t x = i % w, y = i / w;
+    if (x == 0 || y == 0 || x == w - 1 || y == h - 1) st3(img + 3 * (size_t)i, splat(0.0f));
+}
+
 // ------------------------------------------------------------------ ray tracer
 // raytracing_kernel, RayTracer.cu:11-54: primary hit, every light as a point light, shadow rays
 __global__ __launch_bounds__(kIntBlock) void k_raytrace(DevScene S, cupbr_camera cam, uint32_t w, uint32_t h,
@@ -521,10 +550,25 @@ int render_gdpt(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uin
                        im[CUPBR_IMAGE_GRADIENT_X], im[CUPBR_IMAGE_GRADIENT_Y], w, h);
     float* a = im[CUPBR_IMAGE_OUTPUT];
     float* b = r->temp.as<float>();
-    for (int it = 0; it < 50; ++it) {  // ping-pong; 50 is even so the result lands in the output image
-        hipLaunchKernelGGL(k_gdpt_opt, dim3(blocks), dim3(kIntBlock), 0, r->stream, a, im[CUPBR_IMAGE_GRADIENT_X],
-                           im[CUPBR_IMAGE_GRADIENT_Y], b, w, h);
-        std::swap(a, b);
+    if (r->gdpt_mode == CUPBR_GDPT_TRIMMED_MEAN) {
+        for (int it = 0; it < 50; ++it) {  // ping-pong; 50 is even so the result lands in the output image
+            hipLaunchKernelGGL(k_gdpt_opt, dim3(blocks), dim3(kIntBlock), 0, r->stream, a, im[CUPBR_IMAGE_GRADIENT_X],
+                               im[CUPBR_IMAGE_GRADIENT_Y], b, w, h);
+            std::swap(a, b);
+        }
+    } else {
+        // the iterates hold a zero border (temp's border is never written by the Jacobi kernel)
+        hipLaunchKernelGGL(k_zero_border, dim3(blocks), dim3(kIntBlock), 0, r->stream, a, w, h);
+        hipLaunchKernelGGL(k_zero_border, dim3(blocks), dim3(kIntBlock), 0, r->stream, b, w, h);
+        const float a2 = r->gdpt_alpha * r->gdpt_alpha;
+        for (uint32_t it = 0; it < r->gdpt_iterations; ++it) {
+            hipLaunchKernelGGL(k_gdpt_l2, dim3(blocks), dim3(kIntBlock), 0, r->stream, a, im[CUPBR_IMAGE_BASE],
+                               im[CUPBR_IMAGE_GRADIENT_X], im[CUPBR_IMAGE_GRADIENT_Y], b, w, h, a2);
+            std::swap(a, b);
+        }
+        if (a != im[CUPBR_IMAGE_OUTPUT])
+            CUPBR_HIP_CHECK(hipMemcpyAsync(im[CUPBR_IMAGE_OUTPUT], a, (size_t)npix * 12, hipMemcpyDeviceToDevice,
+                                           r->stream));
     }
     CUPBR_HIP_CHECK(hipGetLastError());
     return CUPBR_OK;

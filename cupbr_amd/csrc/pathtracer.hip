@@ -402,6 +402,7 @@ void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o, const float
     // with the two tails of the refill (extension rays, then shadow rays). The decision is the
     // same for every wave (n_ext and n_sh are read at the start).
     const bool refill = !kCount && n_ext + n_sh >= 4u * gridDim.x * kPtBlock;
+    if (refill && gtid == 0) c->refill_launches += 1;
     if (refill) {
         trace_queue<false>(
             S, n_ext, g0, c->work, st,

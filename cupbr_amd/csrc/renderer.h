@@ -31,6 +31,7 @@ struct Counters {
     unsigned long long extension_rays;
     unsigned long long shadow_rays;
     unsigned long long ext_nodes, ext_prims, shadow_nodes, shadow_prims;  // traversal counting
+    unsigned long long refill_launches;  // trace launches that handed rays to idle lanes
 };
 
 // HIP events recorded between consecutive launches on the renderer's stream;
@@ -79,6 +80,9 @@ struct cupbr_renderer {
     uint32_t frame_index = 0;
     uint32_t max_trace_depth = 5;   // PBRenderer.cu:59
     bool russian_roulette = false;  // uninitialised in the reference (PBRenderer.cu:51); off here
+    int32_t gdpt_mode = CUPBR_GDPT_TRIMMED_MEAN;  // reconstruction of GRADIENTDOMAIN frames
+    float gdpt_alpha = 0.2f;                      // screened-Poisson weight of the primal image
+    uint32_t gdpt_iterations = 50;                // screened-Poisson Jacobi iterations
     cupbr::DevBuf images[cupbr::kNumImages];
     cupbr::DevBuf temp;
     cupbr::Wavefront wf;

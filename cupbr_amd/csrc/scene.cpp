@@ -89,6 +89,24 @@ void scene_release_device(cupbr_scene* s) {
     s->allocations.clear();
     s->dev = DevScene{};
     s->uploaded = false;
+    if (s->stream_event) (void)hipEventDestroy(s->stream_event);
+    s->stream_event = nullptr;
+    s->stream_event_recorded = false;
+}
+
+int scene_stream_acquire(cupbr_scene* s, hipStream_t stream) {
+    if (!s->stream_event) {
+        CUPBR_HIP_CHECK(hipEventCreateWithFlags(&s->stream_event, hipEventDisableTiming));
+        s->stream_event_recorded = false;
+    }
+    if (s->stream_event_recorded) CUPBR_HIP_CHECK(hipStreamWaitEvent(stream, s->stream_event, 0));
+    return CUPBR_OK;
+}
+
+int scene_stream_release(cupbr_scene* s, hipStream_t stream) {
+    CUPBR_HIP_CHECK(hipEventRecord(s->stream_event, stream));
+    s->stream_event_recorded = true;
+    return CUPBR_OK;
 }
 
 int scene_upload(const cupbr_scene* cs) {

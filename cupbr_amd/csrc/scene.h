@@ -35,6 +35,8 @@ struct cupbr_scene {
     std::vector<void*> allocations;
     int device = 0;
     cupbr::DevBuf spill;               // traversal stack spill for the batch query kernels
+    hipEvent_t stream_event = nullptr; // end of the last kernel that used the scene's own buffers
+    bool stream_event_recorded = false;
     cupbr::DevBuf grid_spill;          // traversal stack spill of the grid-medium kernels
     cupbr::DevBuf grid_samples;        // per-sample radiance of the grid-medium forward pass
     cupbr::DevBuf grid_counts;         // 4 x u64 work counters of the grid-medium kernels
@@ -62,6 +64,12 @@ int scene_finalize(cupbr_scene* s);
 int scene_upload(const cupbr_scene* s);
 void scene_release_device(cupbr_scene* s);
 
+// The batch-query spill stack and the grid kernels' spill, sample and gradient buffers belong to
+// the scene, not to a call: a call on one stream first waits for the scene's last kernel on any
+// stream (acquire), and marks the end of its own kernels (release).
+int scene_stream_acquire(cupbr_scene* s, hipStream_t stream);
+int scene_stream_release(cupbr_scene* s, hipStream_t stream);
+
 // XML / OBJ / RGBE loading (loader.cpp)
 int load_scene_xml(const std::string& path, cupbr_scene* s);
 int load_rgbe(const std::string& path, std::vector<float>& rgb, uint32_t& w, uint32_t& h);
@@ -71,6 +79,13 @@ int trace_closest_device(const cupbr_scene* s, const float* rays, size_t n, int3
                          hipStream_t stream);
 int trace_visibility_device(const cupbr_scene* s, const float* rays, const float* dist, size_t n, uint8_t* vis,
                             hipStream_t stream);
+// the closest-hit kernel of trace_closest_device on a given grid of 256-lane blocks, with a spill
+// stack of (stack_entries - 32) words per lane at a stride of blocks * 256 (nullptr when
+// stack_entries <= 32)
+constexpr int kTraceBlock = 256;
+constexpr int kTraceLds = 32;
+int trace_closest_launch(const cupbr_scene* s, const float* rays, size_t n, int32_t* obj, int32_t* tri, float* t,
+                         uint32_t* spill, int blocks, hipStream_t stream);
 
 // heterogeneous medium + adjoint (grid.hip)
 int grid_set_counting(cupbr_scene* s, bool enable);

@@ -5,9 +5,6 @@
 
 namespace cupbr {
 
-constexpr int kTraceBlock = 256;
-constexpr int kTraceLds = 32;
-
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(DevScene S, const float* __restrict__ rays, size_t n,
                                                                int32_t* __restrict__ obj, int32_t* __restrict__ tri,
                                                                float* __restrict__ tout, uint32_t* spill) {
@@ -45,6 +42,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_visibility(DevScene S, co
     }
 }
 
+int trace_closest_launch(const cupbr_scene* s, const float* rays, size_t n, int32_t* obj, int32_t* tri, float* t,
+                         uint32_t* spill, int blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_trace_closest, dim3(blocks), dim3(kTraceBlock), 0, stream, s->dev, rays, n, obj, tri, t, spill);
+    CUPBR_HIP_CHECK(hipGetLastError());
+    return CUPBR_OK;
+}
+
 static int trace_grid(cupbr_scene* s, size_t n, uint32_t** spill) {
     int blocks = (int)std::min<size_t>((n + kTraceBlock - 1) / kTraceBlock, (size_t)device_cu_count() * 16);
     blocks = std::max(blocks, 1);
@@ -64,11 +68,11 @@ int trace_closest_device(const cupbr_scene* cs, const float* rays, size_t n, int
     cupbr_scene* s = const_cast<cupbr_scene*>(cs);
     if (int rc = scene_upload(s)) return rc;
     uint32_t* spill;
+    if (int rc = scene_stream_acquire(s, stream)) return rc;
     int blocks = trace_grid(s, n, &spill);
     if (blocks < 0) return CUPBR_ERR_OUT_OF_MEMORY;
-    hipLaunchKernelGGL(k_trace_closest, dim3(blocks), dim3(kTraceBlock), 0, stream, s->dev, rays, n, obj, tri, t, spill);
-    CUPBR_HIP_CHECK(hipGetLastError());
-    return CUPBR_OK;
+    if (int rc = trace_closest_launch(s, rays, n, obj, tri, t, spill, blocks, stream)) return rc;
+    return scene_stream_release(s, stream);
 }
 
 int trace_visibility_device(const cupbr_scene* cs, const float* rays, const float* dist, size_t n, uint8_t* vis,
@@ -77,11 +81,12 @@ int trace_visibility_device(const cupbr_scene* cs, const float* rays, const floa
     cupbr_scene* s = const_cast<cupbr_scene*>(cs);
     if (int rc = scene_upload(s)) return rc;
     uint32_t* spill;
+    if (int rc = scene_stream_acquire(s, stream)) return rc;
     int blocks = trace_grid(s, n, &spill);
     if (blocks < 0) return CUPBR_ERR_OUT_OF_MEMORY;
     hipLaunchKernelGGL(k_trace_visibility, dim3(blocks), dim3(kTraceBlock), 0, stream, s->dev, rays, dist, n, vis, spill);
     CUPBR_HIP_CHECK(hipGetLastError());
-    return CUPBR_OK;
+    return scene_stream_release(s, stream);
 }
 
 }  // namespace cupbr

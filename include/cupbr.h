@@ -168,6 +168,9 @@ typedef struct cupbr_render_stats {
     uint64_t shadow_node_visits, shadow_prim_tests;
     /* with a radiance cache attached: cache queries answered and training rows collected */
     uint64_t cache_queries, cache_training_rows;
+    /* path tracer trace launches (after bounce 0) whose rays were handed to a wave's idle lanes
+     * as they finished; shorter launches trace 64-ray chunks, one ray per lane */
+    uint64_t refill_launches;
 } cupbr_render_stats;
 
 typedef struct cupbr_scene cupbr_scene;
@@ -178,6 +181,11 @@ typedef struct cupbr_tonemapper cupbr_tonemapper;
 /* ------------------------------------------------------------------ errors / device */
 const char* cupbr_last_error(void);
 int cupbr_device_name(char* out, size_t n);
+/* Measurement aid, no counterpart in the reference: runs a kernel of independent FP32 FMAs with
+ * every lane active at 8 waves per SIMD (twice; the second launch is timed with HIP events) and
+ * returns its time and its wave64 FMA instruction count, i.e. the VALU issue ceiling against which
+ * the trace kernels' roofline is stated. */
+int cupbr_measure_valu_peak(uint32_t iterations, double* milliseconds, uint64_t* fma_instructions);
 
 /* ------------------------------------------------------------------ camera */
 /* Camera(width, height), lib/src/DataStructure/Camera.cpp:6-11: origin, looking +z, 90 deg vertical FOV */
@@ -185,6 +193,35 @@ int cupbr_camera_init(cupbr_camera* cam, uint32_t width, uint32_t height);
 /* extension: pinhole with pose and vertical field of view (degrees), same axis convention */
 int cupbr_camera_look_at(cupbr_camera* cam, const float eye[3], const float target[3], const float up[3],
                          float vfov_degrees, uint32_t width, uint32_t height);
+
+/* Camera::processInput(GLFWwindow*, const float&), lib/src/DataStructure/Camera.cpp:13-90, with
+ * the window's key and cursor polling replaced by an input snapshot (the caller reads them from
+ * its own window system). The controller holds the state the reference keeps in Camera's private
+ * members (Camera.h:78-85): aspect ratio, pitch, yaw, first-mouse flag, last cursor position.
+ * WASD move along the x and z axes at 4 units/s (x4 with left shift, /4 with left control); the
+ * cursor offset times 0.002 turns yaw and pitch (pitch clamped to +-3.14159/2); the axes are then
+ * rebuilt from yaw and pitch, and moved is set when a key moved the camera or the cursor moved. */
+typedef struct cupbr_camera_controller {
+    float aspect_ratio;
+    float pitch;
+    float yaw;
+    int32_t first_mouse;
+    float last_x, last_y;
+} cupbr_camera_controller;
+#define CUPBR_KEY_W 1u
+#define CUPBR_KEY_A 2u
+#define CUPBR_KEY_S 4u
+#define CUPBR_KEY_D 8u
+#define CUPBR_KEY_LEFT_SHIFT 16u
+#define CUPBR_KEY_LEFT_CONTROL 32u
+typedef struct cupbr_input_state {
+    uint32_t keys;            /* CUPBR_KEY_* bits of the keys held down (glfwGetKey == GLFW_PRESS) */
+    double cursor_x, cursor_y; /* glfwGetCursorPos */
+} cupbr_input_state;
+/* the state of Camera(width, height) (Camera.cpp:6-11, Camera.h:78-85) */
+int cupbr_camera_controller_init(cupbr_camera_controller* ctl, uint32_t width, uint32_t height);
+int cupbr_camera_process_input(cupbr_camera* cam, cupbr_camera_controller* ctl, const cupbr_input_state* input,
+                               float delta_time);
 
 /* ------------------------------------------------------------------ scene */
 /* Scene SceneLoader::loadFromFile(const std::string&), lib/src/Scene/SceneLoader.cpp:140-454.
@@ -266,6 +303,17 @@ int cupbr_renderer_resume(cupbr_renderer* r, const float* host_rgb, size_t n_flo
 int cupbr_renderer_set_profiling(cupbr_renderer* r, int32_t enable);
 int cupbr_renderer_set_traversal_counting(cupbr_renderer* r, int32_t enable);
 int cupbr_renderer_synchronize(cupbr_renderer* r);
+
+/* Reconstruction of GRADIENTDOMAIN frames. CUPBR_GDPT_TRIMMED_MEAN is the reference's: 50 iterations
+ * of the trimmed mean of the four neighbour estimates (GradientDomain.cu:303-350,392-401; alpha and
+ * iterations are ignored). CUPBR_GDPT_SCREENED_POISSON has no counterpart in the reference: `iterations`
+ * Jacobi iterations of the L2 screened-Poisson problem, minimising
+ * alpha^2 |I - base|^2 + |dx I - gradient_x|^2 + |dy I - gradient_y|^2 over the interior pixels (the
+ * gradients of init_kernel, GradientDomain.cu:248-273, as forward differences; border pixels zero).
+ * Defaults: trimmed mean, alpha 0.2, 50 iterations. */
+#define CUPBR_GDPT_TRIMMED_MEAN 0
+#define CUPBR_GDPT_SCREENED_POISSON 1
+int cupbr_renderer_set_gdpt_reconstruction(cupbr_renderer* r, int32_t mode, float alpha, uint32_t iterations);
 
 /* ------------------------------------------------------------------ post processing (PostProcessor.h:12-52) */
 int cupbr_post_create(cupbr_post** out);

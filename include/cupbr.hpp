@@ -58,7 +58,10 @@ struct Image {
 // addition for scripted poses. moved() restarts accumulation (PBRenderer.cu:174-177).
 class Camera {
 public:
-    Camera(uint32_t width, uint32_t height) { check(cupbr_camera_init(&c_, width, height)); }
+    Camera(uint32_t width, uint32_t height) {
+        check(cupbr_camera_init(&c_, width, height));
+        check(cupbr_camera_controller_init(&ctl_, width, height));
+    }
     static Camera lookAt(const float eye[3], const float target[3], const float up[3], float vfov_degrees,
                          uint32_t width, uint32_t height) {
         Camera cam(width, height);
@@ -67,6 +70,11 @@ public:
     }
     bool moved() const { return c_.moved != 0; }
     void setMoved(bool m) { c_.moved = m ? 1 : 0; }
+    // processInput(GLFWwindow*, const float&), Camera.cpp:13-90: the window's key and cursor state
+    // is passed as a snapshot (keys: CUPBR_KEY_* bits held down)
+    void processInput(const cupbr_input_state& input, float delta_time) {
+        check(cupbr_camera_process_input(&c_, &ctl_, &input, delta_time));
+    }
     const cupbr_camera* get() const { return &c_; }
     cupbr_camera* get() { return &c_; }
 
@@ -75,6 +83,7 @@ private:
         if (rc != CUPBR_OK) throw std::runtime_error(cupbr_last_error());
     }
     cupbr_camera c_{};
+    cupbr_camera_controller ctl_{};
 };
 
 // Scene (Scene.h:15-34) as a handle; owned until SceneLoader::destroyScene.

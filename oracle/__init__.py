@@ -59,6 +59,7 @@ def lib():
         L.orc_gdpt_reconstruct.argtypes = [C.c_uint32, C.c_uint32, P, C.c_int, P, P, P]
         L.orc_grid.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                C.c_uint32, C.c_int, P, P, P, P]
+        L.orc_gdpt_reconstruct_l2.argtypes = [C.c_uint32, C.c_uint32, P, C.c_float, C.c_int, P]
         L.orc_bloom.argtypes = [P, C.c_uint32, C.c_uint32, P, P]
         L.orc_tonemap.argtypes = [P, C.c_uint64, C.c_int, C.c_float, P]
         L.orc_brdf.argtypes = [P, P, P, P, P]
@@ -220,6 +221,16 @@ def gdpt_reconstruct(bufs5, iters=50):
     gy = np.zeros((h, w, 3), np.float32)
     lib().orc_gdpt_reconstruct(w, h, _p(b), iters, _p(recon), _p(gx), _p(gy))
     return recon, gx, gy
+
+
+def gdpt_reconstruct_l2(bufs5, alpha=0.2, iters=50):
+    """Screened-Poisson (L2) reconstruction of the 5 gradient-domain buffers: `iters` Jacobi
+    iterations of (a^2 + 4) I = a^2 B + sum of the four neighbour estimates (FP32)."""
+    b = _c(bufs5, np.float32)
+    _, h, w, _ = b.shape
+    recon = np.zeros((h, w, 3), np.float32)
+    lib().orc_gdpt_reconstruct_l2(w, h, _p(b), float(alpha), int(iters), _p(recon))
+    return recon
 
 
 def bloom(hdr, threshold_curve):

@@ -1442,6 +1442,49 @@ void orc_gdpt_reconstruct(uint32_t w, uint32_t h, const float* bufs5, int iters,
         for (int c = 0; c < 3; ++c) { recon[3 * i + c] = r[i][c]; gx[3 * i + c] = x[i][c]; gy[3 * i + c] = y[i][c]; }
 }
 
+// Screened-Poisson (L2) reconstruction, the gradient-domain mode with no counterpart in the
+// reference (it keeps init_kernel's gradients, GradientDomain.cu:248-273, and border handling):
+// Jacobi iterations on (a^2 + 4) I(p) = a^2 B(p) + (I(L) + gx(L)) + (I(D) + gy(D)) + (I(R) - gx(p))
+// + (I(U) - gy(p)) over the interior pixels, the border held at zero. FP32, the device kernel's
+// operation order: (a2 * B + ((v1 + v2) + (v3 + v4))) / (a2 + 4).
+void orc_gdpt_reconstruct_l2(uint32_t w, uint32_t h, const float* bufs5, float alpha, int iters, float* recon) {
+    const size_t n = (size_t)w * h;
+    std::vector<V3<float>> b[5];
+    for (int k = 0; k < 5; ++k) {
+        b[k].resize(n);
+        for (size_t i = 0; i < n; ++i) b[k][i] = V3<float>(bufs5[k * 3 * n + 3 * i], bufs5[k * 3 * n + 3 * i + 1], bufs5[k * 3 * n + 3 * i + 2]);
+    }
+    std::vector<V3<float>> gx(n, V3<float>(0)), gy(n, V3<float>(0)), cur = b[0], nxt(n, V3<float>(0));
+    for (uint32_t y = 1; y + 1 < h; ++y)
+        for (uint32_t x = 1; x + 1 < w; ++x) {
+            size_t i = (size_t)y * w + x;
+            gx[i] = b[1][i] + b[2][i];
+            gy[i] = b[3][i] + b[4][i];
+        }
+    // the border of the iterate is zero, as in the trimmed-mean mode
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x)
+            if (x == 0 || y == 0 || x + 1 == w || y + 1 == h) cur[(size_t)y * w + x] = V3<float>(0);
+    const float a2 = alpha * alpha;
+    const float den = a2 + 4.0f;
+    for (int it = 0; it < iters; ++it) {
+        for (uint32_t y = 1; y + 1 < h; ++y)
+            for (uint32_t x = 1; x + 1 < w; ++x) {
+                size_t i = (size_t)y * w + x;
+                V3<float> v1 = cur[i - 1] + gx[i - 1];
+                V3<float> v2 = cur[i - w] + gy[i - w];
+                V3<float> v3 = cur[i + 1] - gx[i];
+                V3<float> v4 = cur[i + w] - gy[i];
+                V3<float> r;
+                for (int c = 0; c < 3; ++c) r[c] = (a2 * b[0][i][c] + ((v1[c] + v2[c]) + (v3[c] + v4[c]))) / den;
+                nxt[i] = r;
+            }
+        std::swap(cur, nxt);
+    }
+    for (size_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) recon[3 * i + c] = cur[i][c];
+}
+
 // PostProcessor::bloom, PostProcessor.cpp:125-142 + PyramidConstructor.cu:9-169, 6 levels, FP32.
 // Returns the output buffer and, optionally, every pyramid level (down then up, levels 0..5).
 void orc_bloom(const float* hdr, uint32_t w, uint32_t h, const float* thr, float* out) {

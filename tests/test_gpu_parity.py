@@ -254,9 +254,9 @@ def test_batches_of_frames_equal_frame_by_frame_calls():
     assert np.array_equal(ia, ib)
 
 
-@pytest.mark.parametrize("name,w,h,spp", [("mixed_env", 67, 45, 4), ("million", 203, 117, 4),
-                                          ("mixed_env", 1536, 1024, 4), ("million", 640, 480, 16)])
-def test_fast_paths_equal_per_ray_traversal(name, w, h, spp):
+@pytest.mark.parametrize("name,w,h,spp,refills", [("mixed_env", 67, 45, 4, False), ("million", 203, 117, 4, False),
+                                                  ("mixed_env", 1536, 1024, 4, True), ("million", 640, 480, 16, True)])
+def test_fast_paths_equal_per_ray_traversal(name, w, h, spp, refills):
     """Bounce 0 traces each 64-ray chunk of camera rays as one packet (wave-uniform node and
     primitive loads, a child entered when any lane's interval overlaps it), and launches of at
     least 4 rays per lane of the persistent trace grid (about 2 M rays; the last two cases) hand
@@ -276,9 +276,16 @@ def test_fast_paths_equal_per_ray_traversal(name, w, h, spp):
         r.set_traversal_counting(counting)
         r.render(cam, spp)
         imgs.append(r.get_output_image())
+        st = r.stats()
         if counting:
-            rays = r.stats()["extension_rays"]
-    print(f"{name} {w}x{h} {spp} spp: {rays} extension rays", flush=True)
+            rays = st["extension_rays"]
+            assert st["refill_launches"] == 0
+        else:
+            refill_launches = st["refill_launches"]
+    print(f"{name} {w}x{h} {spp} spp: {rays} extension rays, {refill_launches} refill launches", flush=True)
+    # the large cases must exercise the refill branch (launches of >= 4 rays per lane), the small
+    # ones the 64-ray chunks
+    assert (refill_launches > 0) == refills
     assert np.isfinite(imgs[0]).all() and float(imgs[0].mean()) > 0
     assert np.array_equal(imgs[0], imgs[1])
 
@@ -334,6 +341,33 @@ def test_gradient_domain_same_seed_parity():
     np.testing.assert_allclose(r.get_output_image(), recon_own, rtol=1e-5, atol=1e-6)
     recon, _, _ = oracle.gdpt_reconstruct(bufs.astype(np.float32), 50)
     assert rel_mse(r.get_output_image(), recon) <= 1e-6
+
+
+@pytest.mark.parametrize("alpha,iters", [(0.2, 50), (0.5, 7)])
+def test_gradient_domain_screened_poisson_reconstruction(alpha, iters):
+    """The screened-Poisson (L2) mode on the GPU's own five buffers equals the oracle's Jacobi
+    iterations (deterministic FP32; an odd count ends in the ping-pong buffer and is copied to the
+    output); the buffers themselves are those of the trimmed-mean mode."""
+    desc, ca = scenes.cornell_box()
+    w, h = 40, 32
+    cam = _camera(ca, w, h)
+    scene = cb.Scene.from_desc(desc)
+    r = cb.PBRenderer(cb.RenderingMethod.GRADIENTDOMAIN)
+    r.set_output_size(w, h)
+    r.register_scene(scene)
+    r.set_method(cb.RenderingMethod.GRADIENTDOMAIN)
+    r.set_gdpt_reconstruction(cb.GDPT_SCREENED_POISSON, alpha, iters)
+    r.render(cam, 4)
+    names = [cb.Image.BASE, cb.Image.GRADIENT_X_FORWARD, cb.Image.GRADIENT_X_BACKWARD, cb.Image.GRADIENT_Y_FORWARD,
+             cb.Image.GRADIENT_Y_BACKWARD]
+    own = np.stack([r.get_image(im) for im in names]).astype(np.float32)
+    want = oracle.gdpt_reconstruct_l2(own, alpha, iters)
+    got = r.get_output_image()
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6)
+    assert (got[0] == 0).all() and (got[:, 0] == 0).all()
+    t = _render(scene, cam, w, h, 4, method=cb.RenderingMethod.GRADIENTDOMAIN)
+    for im in names:
+        assert np.array_equal(t.get_image(im), r.get_image(im))
 
 
 def test_raytracer_point_light_closed_form():

@@ -833,3 +833,83 @@ def test_russian_roulette_keeps_the_furnace_mean():
     v = (s2[..., 0].sum() / (256 * n) - m ** 2) / (256 * n)
     assert v > 0  # roulette makes the samples random
     assert abs(m - full) < 4 * math.sqrt(v), (m, full, math.sqrt(v))
+
+
+# --------------------------------------------------------------------------- screened-Poisson reconstruction
+
+
+def _l2_direct_solve(bufs5, alpha):
+    """FP64 minimiser of alpha^2 |I - B|^2 + sum over pixel edges with an interior end of
+    (I(q) - I(p) - g(p))^2 (g = init_kernel's gradients, GradientDomain.cu:248-273, as forward
+    differences), border pixels fixed at 0, by a direct sparse solve of the normal equations."""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import spsolve
+
+    b = np.asarray(bufs5, np.float64)
+    _, h, w, _ = b.shape
+    base = b[0]
+    gx = np.zeros_like(base)
+    gy = np.zeros_like(base)
+    gx[1:-1, 1:-1] = b[1][1:-1, 1:-1] + b[2][1:-1, 1:-1]
+    gy[1:-1, 1:-1] = b[3][1:-1, 1:-1] + b[4][1:-1, 1:-1]
+    inner = np.zeros((h, w), bool)
+    inner[1:-1, 1:-1] = True
+    idx = -np.ones((h, w), int)
+    idx[inner] = np.arange(inner.sum())
+    n = int(inner.sum())
+    rows, cols, vals, tgt = [], [], [], []  # D I - g over the edges, rows of D
+    e = 0
+    for y in range(h):
+        for x in range(w):
+            for (yy, xx, g) in ((y, x + 1, gx), (y + 1, x, gy)):
+                if yy >= h or xx >= w or not (inner[y, x] or inner[yy, xx]):
+                    continue
+                if inner[y, x]:
+                    rows.append(e), cols.append(idx[y, x]), vals.append(-1.0)
+                if inner[yy, xx]:
+                    rows.append(e), cols.append(idx[yy, xx]), vals.append(1.0)
+                tgt.append(g[y, x])
+                e += 1
+    D = sp.csr_matrix((vals, (rows, cols)), shape=(e, n))
+    A = (alpha ** 2 * sp.identity(n) + D.T @ D).tocsc()
+    out = np.zeros((h, w, 3))
+    for c in range(3):
+        rhs = alpha ** 2 * base[..., c][inner] + D.T @ np.array([t[c] for t in tgt])
+        out[..., c][inner] = spsolve(A, rhs)
+    return out
+
+
+def test_screened_poisson_converges_to_the_direct_solve():
+    """The oracle's Jacobi iterations of the screened-Poisson reconstruction converge to the FP64
+    direct solve of the same least-squares problem (a restatement with a wrong sign, neighbour or
+    weight converges elsewhere)."""
+    rng = np.random.default_rng(50)
+    h, w = 14, 18
+    bufs = np.zeros((5, h, w, 3), np.float32)
+    bufs[:, 1:-1, 1:-1] = rng.normal(size=(5, h - 2, w - 2, 3)).astype(np.float32)
+    bufs[0] = np.abs(bufs[0]) + 0.5
+    for alpha in (0.2, 1.0):
+        want = _l2_direct_solve(bufs, alpha)
+        got = oracle.gdpt_reconstruct_l2(bufs, alpha, 6000)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        assert err < 2e-5, (alpha, err)
+
+
+def test_screened_poisson_fixed_point_and_primal_limit():
+    """A ramp with consistent forward differences is a fixed point away from the zero border; with
+    zero gradients and a large alpha the reconstruction tends to the primal image."""
+    h, w = 24, 32
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+    base = np.stack([0.5 + 0.01 * xx + 0.02 * yy] * 3, -1).astype(np.float32)
+    gx = np.zeros_like(base)
+    gy = np.zeros_like(base)
+    gx[:, :-1] = base[:, 1:] - base[:, :-1]
+    gy[:-1] = base[1:] - base[:-1]
+    bufs = np.stack([base, gx / 2, gx / 2, gy / 2, gy / 2])
+    recon = oracle.gdpt_reconstruct_l2(bufs, 0.2, 1)
+    np.testing.assert_allclose(recon[2:-2, 2:-2], base[2:-2, 2:-2], rtol=1e-5)
+    rng = np.random.default_rng(51)
+    b0 = np.zeros((5, h, w, 3), np.float32)
+    b0[0, 1:-1, 1:-1] = rng.uniform(0.5, 2.0, (h - 2, w - 2, 3))
+    recon = oracle.gdpt_reconstruct_l2(b0, 100.0, 50)
+    np.testing.assert_allclose(recon[1:-1, 1:-1], b0[0, 1:-1, 1:-1], rtol=2e-3)
