@@ -913,3 +913,45 @@ def test_screened_poisson_fixed_point_and_primal_limit():
     b0[0, 1:-1, 1:-1] = rng.uniform(0.5, 2.0, (h - 2, w - 2, 3))
     recon = oracle.gdpt_reconstruct_l2(b0, 100.0, 50)
     np.testing.assert_allclose(recon[1:-1, 1:-1], b0[0, 1:-1, 1:-1], rtol=2e-3)
+
+
+def test_volume_single_scattering_inside_an_object():
+    """Scattering inside a VOLUME sphere (no interface), depth 2 (entry, then one event): the event's
+    NEE first traces to the sphere's boundary along the light direction, attenuates by
+    exp(-sigma_t s_exit) and the HG phase HG(g, d . l) (VolumeRenderer.cu:152-172), then the light's
+    I / r^2. With a grey medium the weight after scattering is 1, so
+    E[L] = int_0^{2R} sigma_s exp(-sigma_t t) HG(g, d . l) I exp(-sigma_t s_exit(t)) / r(t)^2 dt
+    along the diameter, by quadrature, within 4 sigma."""
+    from scipy.integrate import quad
+    R, ss, sa, g, I = 1.0, 0.6, 0.3, 0.5, 3.0
+    C = np.array([0.0, 0.0, 3.0])
+    Lp = np.array([0.4, 2.5, 3.3])
+    b = scenes.SceneBuilder()
+    b.sphere(tuple(C), R, scenes.material(scenes.VOLUME, sigma_s=(ss,) * 3, sigma_a=(sa,) * 3, g=g, interface=0))
+    b.point_light(tuple(Lp), (I, I, I))
+    b.volume = np.array([0, 0, 0, 0, 0, 0, 0], np.float32)
+    S = oracle.OracleScene(b.desc())
+    cam = dict(position=[0, 0, 0], x_axis=[1e-7, 0, 0], y_axis=[0, 1e-7, 0], z_axis=[0, 0, 1])
+    n = 40000
+    _, s, s2 = S.render(cam, 2, 2, 0, n, depth=2, method=oracle.VOLUME, precision=64, stats=True)
+    mean = s[..., 0].sum() / (4 * n)
+    var = (s2[..., 0].sum() / (4 * n) - mean ** 2) / (4 * n)
+    f32 = lambda x: float(np.float32(x))
+    st, gg = f32(ss) + f32(sa), f32(g)
+    d = np.array([0.0, 0.0, 1.0])
+
+    def integrand(t):
+        P = C - R * d + t * d
+        v = Lp - P
+        r = np.linalg.norm(v)
+        l = v / r
+        # distance to the sphere boundary from P along l (P inside): |P + s l - C| = R
+        pc = P - C
+        bq = pc @ l
+        s_exit = -bq + math.sqrt(bq * bq - (pc @ pc - R * R))
+        cos = d @ l
+        hg = (1 - gg * gg) / (4 * f32(3.14159)) * (1 + gg * gg - 2 * gg * cos) ** -1.5
+        return f32(ss) * math.exp(-st * t) * hg * I * math.exp(-st * s_exit) / r ** 2
+
+    want = quad(integrand, 0, 2 * R, limit=200)[0]
+    assert abs(mean - want) < 4 * math.sqrt(var), (mean, want, math.sqrt(var))
