@@ -251,8 +251,9 @@ def measure_counters(args, log):
     def bytes_of(disp, name):
         if not disp:
             return None, None
-        first = next((i for i, d in enumerate(disp) if "cupbr" in d["kernel"]), len(disp))
-        cal = sorted(d.get(name, 0.0) for d in disp[:first] if d.get(name, 0.0) > 1000)
+        # the copy kernels of --copy-calibration (torch elementwise kernels: a fill, then three
+        # copies; the fill reads nothing, so reads are taken from the dispatches that read)
+        cal = sorted(d.get(name, 0.0) for d in disp if "elementwise" in d["kernel"] and d.get(name, 0.0) > 1000)
         scale = (COPY_MIB * 1024.0) / cal[len(cal) // 2] if cal else None
         tr = [d.get(name, 0.0) for d in disp if is_trace(d["kernel"])]
         if not tr or scale is None:
@@ -606,15 +607,20 @@ def bench_grid(args, log):
 def bench_nrc(args, log):
     """Equal-time comparison on the 1080p / 1M-triangle scene (depth 8).
 
-    reference  plain PT, 4096 spp (64 calls of 64 spp; the running mean over 4096 frames)
+    reference  plain PT, 4096 spp, FP64-accumulated: each frame f is rendered alone (the running mean
+               restarted at frame index f from a zero image, so the image is sample_f / (f + 1)) and
+               its samples are summed in FP64
     cache      online training over 300 warm-up frames in calls of s frames (--nrc-spp, default 4;
                2^16 training paths per frame, 16 Adam steps of 2^14 rows after each call), then
                `steps` timed calls of s spp from frame 0 (inference, training rows and the call's
                training included)
-    plain PT   timed at 1, 2, 4 and 8 spp; its time is fitted as a + b * spp (least squares) and
-               its rel-MSE follows c / spp (unbiased, measured at the same counts); at the cache
-               call's time T it affords s* = (T - a) / b samples, rel-MSE c / s*
-    value      rel-MSE(plain PT at time T) / rel-MSE(cache, s spp); higher is better
+    plain PT   timed and rendered at 1..12 spp (one call each); equal time = the cache call's time T:
+               value = rel-MSE(plain PT at the smallest spp whose measured time is >= T) /
+               rel-MSE(cache), i.e. plain PT is given at least the cache's time; the ratio against
+               the largest spp with time <= T is reported beside it
+    ceiling    rel-MSE of the primary vertex's direct light alone (plain PT of depth 1 against its own
+               FP64 reference): a cache queried from the second vertex on cannot remove it, so
+               rel-MSE(cache at T) >= rel-MSE(depth 1, 1 spp) x (cost of a cached sample / T)
     roofline   the inference kernel (matrix cores): 2 * 16,576 flops per query / launch time"""
     import numpy as np
     import torch
@@ -629,11 +635,11 @@ def bench_nrc(args, log):
     scene.upload()
     cam = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], W, H)
 
-    def renderer(cache=None):
+    def renderer(cache=None, depth=DEPTH):
         r = cb.PBRenderer(cb.RenderingMethod.PATHTRACER)
         r.set_output_size(W, H)
         r.register_scene(scene)
-        r.set_max_trace_depth(DEPTH)
+        r.set_max_trace_depth(depth)
         if cache is not None:
             r.set_radiance_cache(cache, training_paths=1 << 16, train_steps=16, batch=1 << 14, learning_rate=1e-3)
         return r
@@ -649,24 +655,31 @@ def bench_nrc(args, log):
         r.synchronize()
         return (time.perf_counter() - t0) * 1e3
 
+    def fp64_reference(depth, n):
+        """mean of frames 0..n-1 summed in FP64, each frame's samples recovered exactly: from a zero
+        image at frame index f the running mean is a * L with a = 1/(f + 1) in FP32 (PathTracer.cu:162-168)"""
+        R = renderer(depth=depth)
+        zero = np.zeros((H, W, 3), np.float32)
+        acc = np.zeros((H, W, 3), np.float64)
+        for f in range(n):
+            R.resume(zero, f)
+            R.render(cam, 1)
+            img = R.get_output_image().astype(np.float64)
+            acc += img if f == 0 else img / float(np.float32(1.0) / (np.float32(f) + np.float32(1.0)))
+        return acc / n
+
     t0 = time.perf_counter()
-    R = renderer()
-    for _ in range(64):
-        R.render(cam, 64)
-    ref = R.get_output_image().astype(np.float64)
-    log(f"reference: 4096 spp in {time.perf_counter() - t0:.1f}s")
-    del R
+    ref = fp64_reference(DEPTH, 4096)
+    log(f"reference: 4096 spp, FP64-accumulated frame by frame, in {time.perf_counter() - t0:.1f}s")
 
     P = renderer()
     frame_ms(P, 8)
-    pt_spp, pt_ms, pt_err = [1, 2, 4, 8], [], []
+    pt_spp = list(range(1, 13))
+    pt_ms, pt_err = [], []
     for k in pt_spp:
         pt_ms.append(float(np.median([frame_ms(P, k) for _ in range(3)])))
         pt_err.append(rel_mse(P.get_output_image(), ref))
-    b_fit, a_fit = np.polyfit(pt_spp, pt_ms, 1)
-    c_fit = float(np.mean([e * k for e, k in zip(pt_err, pt_spp)]))
-    log("plain PT: " + ", ".join(f"{k} spp {t:.2f} ms rel-MSE {e:.4f}" for k, t, e in zip(pt_spp, pt_ms, pt_err)) +
-        f"; time {a_fit:.2f} + {b_fit:.2f} ms/spp, rel-MSE {c_fit:.4f}/spp")
+    log("plain PT: " + ", ".join(f"{k} spp {t:.2f} ms {e:.4f}" for k, t, e in zip(pt_spp, pt_ms, pt_err)))
 
     cache = cb.RadianceCache((-10.0, -10.0, -10.0), (110.0, 110.0, 110.0), seed=1)
     N = renderer(cache)
@@ -682,16 +695,28 @@ def bench_nrc(args, log):
         errs.append(rel_mse(N.get_output_image(), ref))
     st = N.stats()
     t_nrc, e_nrc = float(np.median(times)), float(np.mean(errs))
-    s_eq = max((t_nrc - a_fit) / b_fit, 1e-6)
-    e_pt = c_fit / s_eq
-    ratio = e_pt / e_nrc
-    k = max(1, int(round(s_eq)))
-    P.reset()
-    P.render(cam, k)
-    e_pt_k = rel_mse(P.get_output_image(), ref)
+    hi = next((i for i, t in enumerate(pt_ms) if t >= t_nrc), len(pt_ms) - 1)
+    lo = max(0, hi - 1) if pt_ms[hi] > t_nrc else hi
+    ratio = pt_err[hi] / e_nrc
+    ratio_lo = pt_err[lo] / e_nrc
+    b_fit, a_fit = np.polyfit(pt_spp, pt_ms, 1)
     log(f"cache: {S_NRC} spp in {t_nrc:.2f} ms, rel-MSE {e_nrc:.4f} ({st['cache_queries']} queries, "
-        f"{st['cache_training_rows']} training rows); plain PT in that time: {s_eq:.2f} spp, rel-MSE {e_pt:.4f} "
-        f"(rendered at {k} spp: {e_pt_k:.4f}); equal-time ratio {ratio:.2f}")
+        f"{st['cache_training_rows']} training rows); plain PT at {pt_spp[hi]} spp ({pt_ms[hi]:.2f} ms, at least equal "
+        f"time): {pt_err[hi]:.4f}, ratio {ratio:.2f}; at {pt_spp[lo]} spp ({pt_ms[lo]:.2f} ms): {pt_err[lo]:.4f}, "
+        f"ratio {ratio_lo:.2f}")
+
+    # variance decomposition: the primary vertex's direct light alone (depth 1)
+    ref1 = fp64_reference(1, 1024)
+    D1 = renderer(depth=1)
+    t_d1 = float(np.median([frame_ms(D1, 1) for _ in range(5)]))
+    e_d1 = rel_mse(D1.get_output_image(), ref1)
+    e_pt1 = pt_err[0]
+    c_nrc = t_nrc / S_NRC  # per cached sample, training included
+    c_pt = b_fit  # marginal cost of a plain PT sample
+    bound = (e_pt1 / c_pt * c_nrc) / e_d1 if e_d1 > 0 else float("nan")
+    log(f"decomposition: plain PT 1 spp rel-MSE {e_pt1:.4f}, of which the primary vertex's direct light (depth 1, "
+        f"{t_d1:.2f} ms) {e_d1:.4f} ({e_d1 / e_pt1:.0%}); a cache from the second vertex on, at {c_nrc:.2f} ms per "
+        f"sample against {c_pt:.2f} ms per plain sample, is bounded by an equal-time ratio of {bound:.2f}")
 
     # inference kernel alone, timed with HIP events on torch's stream
     q = int(st["cache_queries"])
@@ -722,12 +747,16 @@ def bench_nrc(args, log):
         "dtype": "bf16",
         "data": "synthetic (seeded procedural scene, cupbr_amd.scenes.million_triangles)",
         "config": {"workload": f"PT+NEE {W}x{H}, depth {DEPTH}, {S_NRC} spp per call with a 64->4x64->3 radiance "
-                               f"cache trained online after each call; plain PT at equal time; reference 4096 spp",
+                               f"cache trained online after each call; plain PT at (at least) equal time, measured; "
+                               f"reference 4096 spp FP64-accumulated",
                    "cache_spp": S_NRC, "rel_mse_cache": round(e_nrc, 5), "cache_call_ms": round(t_nrc, 3),
-                   "rel_mse_pt_equal_time": round(e_pt, 5), "pt_spp_equal_time": round(s_eq, 3),
-                   "pt_ms_fit": [round(float(a_fit), 3), round(float(b_fit), 3)], "pt_rel_mse_per_spp": round(c_fit, 5),
+                   "pt_equal_time": {"spp": pt_spp[hi], "ms": round(pt_ms[hi], 3), "rel_mse": round(pt_err[hi], 5)},
+                   "pt_less_time": {"spp": pt_spp[lo], "ms": round(pt_ms[lo], 3), "rel_mse": round(pt_err[lo], 5),
+                                    "ratio": round(ratio_lo, 3)},
                    "pt_measured": {str(k_): [round(t_, 3), round(e_, 5)] for k_, t_, e_ in zip(pt_spp, pt_ms, pt_err)},
-                   "rel_mse_pt_rounded_spp": [k, round(e_pt_k, 5)]},
+                   "pt_ms_fit": [round(float(a_fit), 3), round(float(b_fit), 3)],
+                   "direct_only_rel_mse_1spp": round(e_d1, 5), "pt_rel_mse_1spp": round(e_pt1, 5),
+                   "ceiling_ratio": round(bound, 3)},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 5), "traffic": None, "kernel": "k_nrc_infer<false>",
                      "avg_launch_ms": round(inf_ms, 4), "queries_per_launch": int(x.shape[0])},
@@ -737,12 +766,12 @@ def bench_nrc(args, log):
 
 
 def bench_integrator(args, log):
-    """The one-thread-per-pixel integrators: GDPT (C2: Cornell box 512x512, 128 spp, depth 5; every
-    sample traces the base path and 4 neighbour paths, then 50 reconstruction iterations) or the
-    volume renderer (homogeneous media: two scattering spheres in a glass-walled box and a scene
-    medium, 512x512, 64 spp, depth 8). One step = reset + one render call. Roofline: the
-    algorithmic BVH bytes of all traversals of a step (counted in an instrumented pass) over the
-    render call's device time (HIP events on the renderer's stream)."""
+    """The other integrators: GDPT (Cornell box 512x512, 128 spp, depth 5; every sample traces the
+    base path and 4 neighbour paths on the wavefront queues, then 50 reconstruction iterations) or
+    the volume renderer (one thread per pixel; homogeneous media: two scattering spheres in a
+    glass-walled box and a scene medium, 512x512, 64 spp, depth 8). One step = reset + one render
+    call. Roofline: the algorithmic BVH bytes of all traversals of a step (counted in an
+    instrumented pass) over the render call's device time (HIP events on the renderer's stream)."""
     import numpy as np
 
     import cupbr_amd as cb
@@ -829,7 +858,7 @@ def bench_integrator(args, log):
                    "mrays_per_s": round(rays * args.steps / t_total / 1e6, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_gdpt (all launches of a render call)" if args.workload == "gdpt" else
+                     "kernel": "gradient-domain wavefront (all launches of a render call)" if args.workload == "gdpt" else
                                "k_volume (all launches of a render call)",
                      "bytes_per_step": int(bytes_step), "device_ms_per_step": round(dev_ms / args.steps, 3)},
         "cpu_baseline": cpu,

@@ -28,7 +28,8 @@ __all__ = [
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libcupbr.so")
+# CUPBR_LIBRARY names another build of the library (same ABI) to load instead of the in-tree one
+LIB_PATH = os.environ.get("CUPBR_LIBRARY") or os.path.join(_HERE, "lib", "libcupbr.so")
 
 
 class CupbrError(RuntimeError):

@@ -1,7 +1,8 @@
-// Volume, gradient-domain and ray-tracing integrators (one thread per pixel).
+// Volume and ray-tracing integrators (one thread per pixel) and the gradient-domain reconstruction.
 //
 //   volume   volume_kernel + handleMediumInteraction, lib/src/Renderer/VolumeRenderer.cu:11-329
-//   gdpt     gdpt_kernel, init_kernel, optimization_kernel, lib/src/Renderer/GradientDomain.cu:110-402
+//   gdpt     init_kernel, optimization_kernel, lib/src/Renderer/GradientDomain.cu:248-402 (the
+//            paths of gdpt_kernel run in wavefront form, pathtracer.hip)
 //   raytracer raytracing_kernel, lib/src/Renderer/RayTracer.cu:11-65
 // They share the BVH4 traversal and the shading code of the path tracer.
 #include "renderer.h"
@@ -12,10 +13,9 @@ namespace cupbr {
 
 constexpr int kIntBlock = 256;
 constexpr int kIntLds = 32;
-// The volume and gradient-domain kernels hold a whole path (and its shading) per thread; left to
-// itself the compiler uses 143-165 VGPRs (2-3 waves per SIMD). Capping them at 128 VGPRs
-// (4 waves per SIMD) spills a few values and runs 1.2x (volume) and 1.4x (gradient domain)
-// faster; a 96-VGPR cap (5 waves) spills far more and is slower.
+// The volume kernel holds a whole path (and its shading) per thread; left to itself the compiler
+// uses ~160 VGPRs (2-3 waves per SIMD). Capping it at 128 VGPRs (4 waves per SIMD) spills a few
+// values and runs 1.2x faster; a 96-VGPR cap (5 waves) spills far more and is slower.
 #define CUPBR_INT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 8)))
 
 // kCount: BVH4 node visits and primitive tests of closest-hit (te) and visibility (ts) queries
@@ -282,108 +282,8 @@ __global__ __launch_bounds__(kIntBlock) CUPBR_INT_OCCUPANCY void k_volume(DevSce
 }
 
 // ------------------------------------------------------------------ gradient-domain
-// collect_radiance, GradientDomain.cu:110-158. The base path records which vertices exist
-// (valid bits); a shift path stops at the first vertex the base path does not have (:127).
-template <class TR>
-__device__ v3 gdpt_collect(TR& T, v3 o, v3 d, uint32_t seed, uint32_t max_depth, uint32_t* valid_out,
-                           uint32_t base_valid, bool shift) {
-    const DevScene& S = T.S;
-    v3 radiance = splat(0.0f), weight = splat(1.0f), out_dir = splat(0.0f);
-    const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
-    uint32_t valid = 0;
-    uint32_t depth = 0;
-    do {
-        bool next_valid = false;
-        if (shift && !((base_valid >> depth) & 1u)) break;
-        TraceResult hr;
-        if (!T.closest(o, d, hr)) {
-            if (S.use_env) radiance += weight * env_lookup(S, d);
-            break;
-        }
-        valid |= 1u << depth;
-        const v3 P = o + hr.t * d;
-        v3 N;
-        uint32_t obj;
-        hit_geometry(S, hr.prim, P, N, obj);
-        const Mat m = load_mat(S.mats + obj);
-        const v3 inc = -1.0f * d;
-        radiance += weight * m.albedo_e;
-        if (m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f) {
-            weight = splat(0.0f);
-        } else {
-            v3 ldir, le;
-            float dist;
-            if (nee_sample(S, m, seed, P, inc, N, ldir, le, dist)) {
-                if (T.visible(P + 0.001f * ldir, normalize(ldir), dist))
-                    radiance += (nl * fmaxf(0.0f, dot(N, ldir))) * brdf(m, inc, ldir, N) * le * weight;
-            }
-        }
-        const DirPdf s = sample_direction(m, seed, inc, N);
-        if (!(norm(s.d) == 0.0f)) {
-            weight = weight * fabsf(dot(s.d, N)) * brdf(m, inc, s.d, N) / s.pdf;
-            out_dir = s.d;
-            next_valid = true;
-        }
-        o = P + 0.01f * out_dir;
-        d = normalize(out_dir);
-        if (!next_valid) break;
-        ++depth;
-    } while (depth < max_depth);
-    if (valid_out) *valid_out = valid;
-    return radiance;
-}
-
-// gdpt_kernel, GradientDomain.cu:160-246: images base, gxf, gxb, gyf, gyb
-template <bool kCount>
-__global__ __launch_bounds__(kIntBlock) CUPBR_INT_OCCUPANCY void k_gdpt(DevScene S, cupbr_camera cam, uint32_t w, uint32_t h,
-                                                    uint32_t frame0, uint32_t nspp, uint32_t max_depth,
-                                                    float* __restrict__ base, float* __restrict__ gxf,
-                                                    float* __restrict__ gxb, float* __restrict__ gyf,
-                                                    float* __restrict__ gyb, Counters* c, uint32_t* spill) {
-    __shared__ uint32_t lds[kIntLds * kIntBlock];
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    LaneStack<kIntLds> st{lds + threadIdx.x, kIntBlock, spill ? spill + gtid : nullptr, gridDim.x * blockDim.x};
-    TracerT<kCount> T{S, st};
-    const uint32_t npix = w * h;
-    if (gtid < npix) {
-        const uint32_t px = gtid % w, py = gtid / w;
-        if (px != 0 && px != w - 1 && py != 0 && py != h - 1) {
-            const size_t i3 = 3 * (size_t)gtid;
-            for (uint32_t s = 0; s < nspp; ++s) {
-                const uint32_t f = frame0 + s;
-                // The five camera rays are drawn from one seed chain in the order base, left, right,
-                // up, down, and each path continues from the chain as it stood after its own ray.
-                // Drawing each ray just before its path keeps that order and needs one ray in
-                // registers instead of five; one loop body serves the base and the shifted paths.
-                uint32_t chain = tea4(gtid, f);
-                uint32_t valid = 0;
-                v3 Lb = splat(0.0f);
-#pragma unroll 1
-                for (uint32_t k = 0; k < 5; ++k) {
-                    const int dx = k == 1 ? -1 : (k == 2 ? 1 : 0);
-                    const int dy = k == 3 ? 1 : (k == 4 ? -1 : 0);
-                    v3 o, d;
-                    launch_ray((uint32_t)((int)px + dx), (uint32_t)((int)py + dy), w, h, cam, chain, o, d);
-                    const v3 L = gdpt_collect(T, o, d, chain, max_depth, k == 0 ? &valid : nullptr, valid, k != 0);
-                    if (k == 0) {
-                        Lb = L;
-                        running_mean(base + i3, Lb, f);
-                    } else if (k == 1) {
-                        running_mean(gxf + i3, 0.5f * (Lb - L), f);
-                    } else if (k == 2) {
-                        running_mean(gxb + i3, 0.5f * (L - Lb), f);
-                    } else if (k == 3) {
-                        running_mean(gyb + i3, 0.5f * (L - Lb), f);
-                    } else {
-                        running_mean(gyf + i3, 0.5f * (Lb - L), f);
-                    }
-                }
-            }
-        }
-    }
-    add_ray_counts(c, T.ext, T.shadow);
-    add_visit_counts(c, T);
-}
+// The paths of gdpt_kernel (GradientDomain.cu:110-246) run in wavefront form (pathtracer.hip,
+// gdpt_paths_wavefront); the reconstruction kernels follow.
 
 // init_kernel, GradientDomain.cu:248-273
 __global__ void k_gdpt_init(float* recon, const float* base, const float* gxf, const float* gxb, const float* gyf,
@@ -529,19 +429,11 @@ int render_volume(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, u
 int render_gdpt(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp) {
     const uint32_t w = r->width, h = r->height, npix = w * h;
     const uint32_t blocks = (npix + kIntBlock - 1) / kIntBlock;
-    uint32_t* spill;
-    Counters* c;
-    int rc = prepare(r, blocks * kIntBlock, &spill, &c);
-    if (rc) return rc;
+    int rc;
     float* im[kNumImages];
     for (int k = 0; k < kNumImages; ++k) im[k] = r->images[k].as<float>();
-    for (uint32_t s = 0; s < spp; s += 2) {
-        hipLaunchKernelGGL(r->counting ? k_gdpt<true> : k_gdpt<false>, dim3(blocks), dim3(kIntBlock), 0, r->stream,
-                           r->scene->dev, cam, w, h, frame0 + s,
-                           std::min(2u, spp - s), std::max(1u, r->max_trace_depth), im[CUPBR_IMAGE_BASE],
-                           im[CUPBR_IMAGE_GRADIENT_X_FORWARD], im[CUPBR_IMAGE_GRADIENT_X_BACKWARD],
-                           im[CUPBR_IMAGE_GRADIENT_Y_FORWARD], im[CUPBR_IMAGE_GRADIENT_Y_BACKWARD], c, spill);
-    }
+    // the five paths of every (pixel, frame) in wavefront form (pathtracer.hip)
+    if ((rc = gdpt_paths_wavefront(r, cam, frame0, spp))) return rc;
     // reconstruction (:380-401); a frame's reconstruction depends only on the current running
     // means, so it runs once after the call's last frame
     hipLaunchKernelGGL(k_gdpt_init, dim3(blocks), dim3(kIntBlock), 0, r->stream, im[CUPBR_IMAGE_OUTPUT],

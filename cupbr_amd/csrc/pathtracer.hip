@@ -509,6 +509,187 @@ __global__ __launch_bounds__(kPtBlock) void k_accumulate(float* __restrict__ img
     st3(img + 3 * (size_t)pix, acc);
 }
 
+// ------------------------------------------------------------------ gradient-domain paths
+// gdpt_kernel (GradientDomain.cu:160-246) on the same queues. A group is one interior pixel of one
+// frame: its base path and four shifted paths (left, right, up, down), path ids 5g + k. Each
+// follows collect_radiance (:110-158) on its own part of the group's seed chain; a shifted path
+// stops at the first depth where the base path has no vertex (:126-127). All five advance bounce
+// by bounce: the base path's shade of bounce b records the queue slot of its bounce-(b+1) ray (or
+// none) in slot[(b+1) & 1][g], and a shifted path's shade of bounce b+1 reads that ray's hit, one
+// launch later. A shifted path alive at bounce b passed the check at b-1, so the base path was
+// shaded at b-1 and wrote the slot it reads: no stale entries, no reads of what the same launch
+// writes. Queue slots of a frame's groups follow the 8x8 tiles of the interior pixels.
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+__global__ __launch_bounds__(kPtBlock) void k_gdpt_raygen(cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0,
+                                                          uint32_t n_int, uint32_t n_groups, float4* __restrict__ q_o,
+                                                          float4* __restrict__ q_d, float4* __restrict__ weight,
+                                                          float4* __restrict__ rad, uint32_t* __restrict__ slot0,
+                                                          Counters* c) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) {
+        c->queue_count[0].v = 5u * n_groups;
+        for (int k = 0; k < 8; ++k) c->work[k].v = c->swork[k].v = 0;
+        c->shadow_count[0].v = 0;
+        c->shadow_count[1].v = 0;
+    }
+    if (g >= n_groups) return;
+    const uint32_t s = g / n_int, gi = g - s * n_int;
+    uint32_t ix, iy;
+    tile_pixel(gi, w - 2u, h - 2u, ix, iy);
+    const uint32_t px = ix + 1u, py = iy + 1u;
+    // the five camera rays from one chain, base, left, right, up, down; each path continues from
+    // the chain as it stood after its own ray (:180-213)
+    uint32_t chain = tea4(py * w + px, frame0 + s);
+#pragma unroll 1
+    for (uint32_t k = 0; k < 5; ++k) {
+        const uint32_t nx = k == 1 ? px - 1u : (k == 2 ? px + 1u : px);
+        const uint32_t ny = k == 3 ? py + 1u : (k == 4 ? py - 1u : py);
+        v3 o, d;
+        launch_ray(nx, ny, w, h, cam, chain, o, d);
+        const uint32_t pid = 5u * g + k;
+        q_o[pid] = make_float4(o.x, o.y, o.z, 0.0f);
+        q_d[pid] = make_float4(d.x, d.y, d.z, __uint_as_float(pid));
+        weight[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(chain));
+        rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    slot0[g] = 5u * g;
+}
+
+// one bounce of collect_radiance (:121-156) for every queued path
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void k_gdpt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, const float4* __restrict__ q_o,
+                  const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
+                  float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
+                  float4* __restrict__ s_c, float4* __restrict__ weight, float4* __restrict__ rad, Counters* c,
+                  const uint32_t* __restrict__ slot_now, uint32_t* __restrict__ slot_next) {
+    __shared__ uint32_t s_append[kPtWaves + 1];
+    const uint32_t n = c->queue_count[cur].v;
+    if (blockIdx.x == 0 && threadIdx.x < 8) c->work[threadIdx.x].v = c->swork[threadIdx.x].v = 0;
+    const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
+    for (uint32_t base = blockIdx.x * kPtBlock; base < n; base += gridDim.x * kPtBlock) {
+        const uint32_t i = base + threadIdx.x;
+        bool want_next = false, want_shadow = false;
+        float4 no = {}, nd = {}, so = {}, sd = {}, sc = {};
+        if (i < n) {
+            const float4 o4 = q_o[i], d4 = q_d[i];
+            const float2 hh = q_hit[i];
+            const uint32_t p = __float_as_uint(d4.w);
+            const uint32_t g = p / 5u, k = p - 5u * g;
+            bool go = true;
+            if (k != 0) {  // :126-127: the base path needs a vertex at this depth
+                const uint32_t bs = slot_now[g];
+                go = bs != kNoSlot && __float_as_uint(q_hit[bs].y) != kNoHit;
+            }
+            if (go) {
+                const float4 W = weight[p];
+                const float4 R = rad[p];
+                v3 w3 = mk(W.x, W.y, W.z);
+                uint32_t seed = __float_as_uint(W.w);
+                v3 L = mk(R.x, R.y, R.z);
+                const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+                const uint32_t prim = __float_as_uint(hh.y);
+                if (prim == kNoHit) {  // :128-137
+                    if (S.use_env) L += w3 * env_lookup(S, d);
+                } else {
+                    const v3 P = o + hh.x * d;
+                    v3 N;
+                    uint32_t obj;
+                    hit_geometry(S, prim, P, N, obj);
+                    const Mat m = load_mat(S.mats + obj);
+                    const v3 inc = -1.0f * d;
+                    L += w3 * m.albedo_e;  // emissiveIllumintationGDPT :39-44
+                    if (m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f) {  // directIlluminationGDPT :46-88
+                        w3 = splat(0.0f);
+                    } else {
+                        v3 ldir, le;
+                        float dist = 0.0f;
+                        if (nee_sample(S, m, seed, P, inc, N, ldir, le, dist)) {
+                            const v3 contrib = (nl * fmaxf(0.0f, dot(N, ldir))) * brdf(m, inc, ldir, N) * le * w3;
+                            if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
+                                const v3 sorg = P + 0.001f * ldir;
+                                const v3 sdir = normalize(ldir);
+                                want_shadow = true;
+                                so = make_float4(sorg.x, sorg.y, sorg.z, dist);
+                                sd = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(p));
+                                sc = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+                            }
+                        }
+                    }
+                    // indirectIlluminationGDPT :90-108 (a zero weight keeps the path going: the
+                    // base path's later vertices decide how far its shifted paths go)
+                    const DirPdf sp = sample_direction(m, seed, inc, N);
+                    if (!(norm(sp.d) == 0.0f)) {
+                        w3 = w3 * fabsf(dot(sp.d, N)) * brdf(m, inc, sp.d, N) / sp.pdf;
+                        if (bounce + 1 < max_depth) {  // :154-156
+                            const v3 norg = P + 0.01f * sp.d;
+                            const v3 ndir = normalize(sp.d);
+                            want_next = true;
+                            no = make_float4(norg.x, norg.y, norg.z, 0.0f);
+                            nd = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(p));
+                        }
+                    }
+                    weight[p] = make_float4(w3.x, w3.y, w3.z, __uint_as_float(seed));
+                }
+                rad[p] = make_float4(L.x, L.y, L.z, R.w);
+            }
+            if (k == 0) slot_next[g] = kNoSlot;  // overwritten below when the base path goes on
+        }
+        const uint32_t ns = block_append(want_next, &c->queue_count[cur ^ 1].v, s_append);
+        if (want_next) {
+            nq_o[ns] = no;
+            nq_d[ns] = nd;
+            const uint32_t p = __float_as_uint(nd.w);
+            const uint32_t g = p / 5u;
+            if (p == 5u * g) slot_next[g] = ns;
+        }
+        const uint32_t ss = block_append(want_shadow, &c->shadow_count[cur].v, s_append);
+        if (want_shadow) {
+            s_o[ss] = so;
+            s_d[ss] = sd;
+            s_c[ss] = sc;
+        }
+    }
+}
+
+// gradients and running means of the five buffers (:214-245), frame by frame in order
+__global__ __launch_bounds__(kPtBlock) void k_gdpt_accumulate(float* __restrict__ base, float* __restrict__ gxf,
+                                                              float* __restrict__ gxb, float* __restrict__ gyf,
+                                                              float* __restrict__ gyb, const float4* __restrict__ rad,
+                                                              uint32_t w, uint32_t h, uint32_t n_int, uint32_t frame0,
+                                                              uint32_t nspp) {
+    const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= n_int) return;
+    uint32_t ix, iy;
+    tile_pixel(gi, w - 2u, h - 2u, ix, iy);
+    const size_t i3 = 3 * ((size_t)(iy + 1u) * w + ix + 1u);
+    v3 acc[5] = {ld3(base + i3), ld3(gxf + i3), ld3(gxb + i3), ld3(gyf + i3), ld3(gyb + i3)};
+    for (uint32_t s = 0; s < nspp; ++s) {
+        const size_t p = 5 * ((size_t)s * n_int + gi);
+        v3 L[5];
+        for (int k = 0; k < 5; ++k) {
+            const float4 R = rad[p + k];
+            L[k] = mk(R.x, R.y, R.z);
+        }
+        const v3 val[5] = {L[0], 0.5f * (L[0] - L[1]), 0.5f * (L[2] - L[0]), 0.5f * (L[0] - L[4]),
+                           0.5f * (L[3] - L[0])};
+        const uint32_t f = frame0 + s;
+        for (int k = 0; k < 5; ++k) {
+            if (f > 0) {
+                const float a = 1.0f / ((float)f + 1.0f);
+                acc[k] = (1.0f - a) * acc[k] + a * val[k];
+            } else {
+                acc[k] = val[k];
+            }
+        }
+    }
+    st3(base + i3, acc[0]);
+    st3(gxf + i3, acc[1]);
+    st3(gxb + i3, acc[2]);
+    st3(gyf + i3, acc[3]);
+    st3(gyb + i3, acc[4]);
+}
+
 static int ensure_wavefront(cupbr_renderer* r, uint32_t paths, uint32_t grid_threads) {
     Wavefront& wf = r->wf;
     int rc;
@@ -671,6 +852,77 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
             }
             if ((rc = stage_mark(r, 3))) return rc;
         }
+    }
+    CUPBR_HIP_CHECK(hipGetLastError());
+    return CUPBR_OK;
+}
+
+int gdpt_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp) {
+    const uint32_t w = r->width, h = r->height;
+    if (w < 3 || h < 3 || spp == 0) return CUPBR_OK;  // no interior pixel: the buffers stay as they are
+    const uint32_t n_int = (w - 2u) * (h - 2u);
+    // paths in flight, in whole frames (5 per interior pixel), bounded as in the path tracer
+    const uint32_t target = 1u << 27;
+    uint32_t cap = 1u << 27;
+    {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
+            cap = (uint32_t)std::min<size_t>(cap, std::max<size_t>(total_b / 8 / 200, 1u << 20));
+    }
+    const uint32_t per_frame = 5u * n_int;
+    uint32_t batch = std::max<uint32_t>(1, target / per_frame);
+    batch = std::min(batch, std::max<uint32_t>(1, cap / per_frame));
+    batch = std::min(batch, spp);
+    const int blocks = device_cu_count() * 8;
+    const uint32_t grid_threads = (uint32_t)blocks * kPtBlock;
+    int rc = ensure_wavefront(r, batch * per_frame, grid_threads);
+    if (rc) return rc;
+    Wavefront& wf = r->wf;
+    if ((rc = wf.gdpt_slot.ensure((size_t)2 * batch * n_int * sizeof(uint32_t)))) return rc;
+    uint32_t* slots[2] = {wf.gdpt_slot.as<uint32_t>(), wf.gdpt_slot.as<uint32_t>() + (size_t)batch * n_int};
+    Counters* c = wf.counters.as<Counters>();
+    CUPBR_HIP_CHECK(hipMemsetAsync(c, 0, sizeof(Counters), r->stream));
+    uint32_t* spill = r->scene->stack_entries > (uint32_t)kPtLds ? wf.spill.as<uint32_t>() : nullptr;
+    const DevScene& S = r->scene->dev;
+    float* im[kNumImages];
+    for (int k = 0; k < kNumImages; ++k) im[k] = r->images[k].as<float>();
+    const uint32_t bounces = std::max<uint32_t>(1, r->max_trace_depth);
+    auto trace = r->counting ? k_pt_trace<true> : k_pt_trace<false>;
+    const bool packet = !r->counting && r->scene->stack_entries <= kPacketStack && S.n_nodes > 0;
+    if ((rc = stage_mark(r, -1))) return rc;
+    for (uint32_t s0 = 0; s0 < spp; s0 += batch) {
+        const uint32_t nb = std::min(batch, spp - s0);
+        const uint32_t groups = nb * n_int, npath = 5u * groups;
+        hipLaunchKernelGGL(k_gdpt_raygen, dim3((groups + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
+                           h, frame0 + s0, n_int, groups, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
+                           wf.weight.as<float4>(), wf.radiance.as<float4>(), slots[0], c);
+        if ((rc = stage_mark(r, 3))) return rc;
+        for (uint32_t b = 0; b <= bounces; ++b) {
+            const int cur = (int)(b & 1);
+            if (b == 0 && packet) {
+                const uint32_t nblk = (npath + kPtBlock - 1) / kPtBlock;
+                hipLaunchKernelGGL(k_pt_trace_camera, dim3((nblk + 7u) / 8u * 8u), dim3(kPtBlock), 0, r->stream, S,
+                                   npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(), wf.q_hit.as<float2>(), c);
+            } else {
+                hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
+                                   wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
+                                   wf.s_d[cur ^ 1].as<float4>(), wf.s_c[cur ^ 1].as<float4>(),
+                                   wf.radiance.as<float4>(), c, spill, grid_threads);
+            }
+            if ((rc = stage_mark(r, 0))) return rc;
+            if (b == bounces) break;
+            hipLaunchKernelGGL(k_gdpt_shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
+                               wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(),
+                               wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(), wf.s_o[cur].as<float4>(),
+                               wf.s_d[cur].as<float4>(), wf.s_c[cur].as<float4>(), wf.weight.as<float4>(),
+                               wf.radiance.as<float4>(), c, slots[b & 1], slots[(b + 1) & 1]);
+            if ((rc = stage_mark(r, 1))) return rc;
+        }
+        hipLaunchKernelGGL(k_gdpt_accumulate, dim3((n_int + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream,
+                           im[CUPBR_IMAGE_BASE], im[CUPBR_IMAGE_GRADIENT_X_FORWARD], im[CUPBR_IMAGE_GRADIENT_X_BACKWARD],
+                           im[CUPBR_IMAGE_GRADIENT_Y_FORWARD], im[CUPBR_IMAGE_GRADIENT_Y_BACKWARD],
+                           wf.radiance.as<float4>(), w, h, n_int, frame0 + s0, nb);
+        if ((rc = stage_mark(r, 3))) return rc;
     }
     CUPBR_HIP_CHECK(hipGetLastError());
     return CUPBR_OK;

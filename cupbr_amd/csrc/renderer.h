@@ -67,6 +67,7 @@ struct Wavefront {
                                     // direction.xyz+path | contribution
     DevBuf counters;
     DevBuf spill;
+    DevBuf gdpt_slot;  // gradient-domain groups: queue slot of the base path's ray, by bounce parity
     uint32_t spill_entries = 0;
 };
 
@@ -115,6 +116,9 @@ inline int stage_mark(cupbr_renderer* r, int stage) { return r->profiling ? r->t
 int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp);
 int render_volume(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp);
 int render_gdpt(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp);
+// the gradient-domain paths of frames [frame0, frame0+spp) in wavefront form: running means of the
+// base and the four gradient buffers (pathtracer.hip)
+int gdpt_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp);
 int render_raytracer(cupbr_renderer* r, const cupbr_camera& cam);
 
 }  // namespace cupbr
