@@ -186,6 +186,7 @@ def lib():
             "cupbr_scene_get_reference_order": (C.c_int, [P, P, U32]),
             "cupbr_trace_closest": (C.c_int, [P, P, SZ, P, P, P]),
             "cupbr_trace_closest_device": (C.c_int, [P, P, SZ, P, P, P, P]),
+            "cupbr_trace_closest_device_noalloc": (C.c_int, [P, P, SZ, P, P, P, P]),
             "cupbr_trace_visibility": (C.c_int, [P, P, P, SZ, P]),
             "cupbr_trace_visibility_device": (C.c_int, [P, P, P, SZ, P, P]),
             "cupbr_renderer_create": (C.c_int, [I32, P]),

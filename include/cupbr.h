@@ -260,6 +260,17 @@ int cupbr_trace_closest(const cupbr_scene* scene, const float* rays, size_t n, i
                         int32_t* triangle_id, float* t);
 int cupbr_trace_closest_device(const cupbr_scene* scene, const float* rays, size_t n, int32_t* object_id,
                                int32_t* triangle_id, float* t, void* hip_stream);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc). Same results as
+ * cupbr_trace_closest_device; the traversal-stack spill lives in a fixed per-device buffer of the
+ * library (16 MiB, held until the process ends) instead of a scene-owned allocation, and the grid
+ * is cut to the blocks whose stacks fit it. Nothing is allocated: the scene must already be
+ * resident on the current device (cupbr_scene_upload, or any GPU call on it) — otherwise
+ * CUPBR_ERR_INVALID_ARGUMENT — and a BVH deeper than the buffer allows (more than 16,416 stack
+ * entries per ray) gives CUPBR_ERR_BUFFER_SIZE. Calls that overlap on one device (other streams
+ * or threads) share the buffer and race; order them. */
+int cupbr_trace_closest_device_noalloc(const cupbr_scene* scene, const float* rays, size_t n, int32_t* object_id,
+                                       int32_t* triangle_id, float* t, void* hip_stream);
 /* Tracing::traceVisibility(Scene&, const float&, const Ray&), TracingDetail.h:118-183 */
 int cupbr_trace_visibility(const cupbr_scene* scene, const float* rays, const float* distance, size_t n,
                            uint8_t* visible);
