@@ -50,10 +50,11 @@ def main():
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nrc-spp", type=int, default=4, help="samples per render call of the radiance-cache workload")
-    ap.add_argument("--workload", choices=["pt", "grid", "nrc", "gdpt", "volume"], default="pt",
+    ap.add_argument("--workload", choices=["pt", "cornell", "grid", "nrc", "gdpt", "volume"], default="pt",
                     help="pt: the headline PT+NEE frame; grid: one gradient evaluation (forward + path-replay "
                          "adjoint) of the 64^3 heterogeneous medium at 256x256, 64 spp; nrc: equal-time rel-MSE of "
                          "1-spp frames with the neural radiance cache against plain PT on the 1080p/1M-triangle scene; "
+                         "cornell: PT+NEE of the 34-triangle Cornell box at 256x256, 16 spp, depth 5; "
                          "gdpt: gradient-domain frame (5 paths per sample + 50 reconstruction iterations) of the "
                          "Cornell box at 512x512, 128 spp; volume: the homogeneous-media renderer at 512x512, 64 spp")
     ap.add_argument("--no-counters", action="store_true",
@@ -94,7 +95,7 @@ def main():
         return bench_grid(args, log)
     if args.workload == "nrc":
         return bench_nrc(args, log)
-    if args.workload in ("gdpt", "volume"):
+    if args.workload in ("gdpt", "volume", "cornell"):
         return bench_integrator(args, log)
     return bench_pt(args, log)
 
@@ -618,9 +619,11 @@ def bench_nrc(args, log):
                value = rel-MSE(plain PT at the smallest spp whose measured time is >= T) /
                rel-MSE(cache), i.e. plain PT is given at least the cache's time; the ratio against
                the largest spp with time <= T is reported beside it
-    ceiling    rel-MSE of the primary vertex's direct light alone (plain PT of depth 1 against its own
-               FP64 reference): a cache queried from the second vertex on cannot remove it, so
-               rel-MSE(cache at T) >= rel-MSE(depth 1, 1 spp) x (cost of a cached sample / T)
+    ceiling    the squared error of the primary vertex's direct light alone (plain PT of depth 1
+               against its own FP64 reference, normalised by the full reference as the metric is):
+               a cache queried from the second vertex on cannot remove it, so a cached sample has at
+               least that error, and the equal-time ratio is at most
+               rel-MSE(PT, 1 spp) / cost(PT sample) x cost(cached sample) / error(depth 1, 1 spp)
     roofline   the inference kernel (matrix cores): 2 * 16,576 flops per query / launch time"""
     import numpy as np
     import torch
@@ -709,7 +712,8 @@ def bench_nrc(args, log):
     ref1 = fp64_reference(1, 1024)
     D1 = renderer(depth=1)
     t_d1 = float(np.median([frame_ms(D1, 1) for _ in range(5)]))
-    e_d1 = rel_mse(D1.get_output_image(), ref1)
+    # the direct part's squared error, normalised as the metric normalises the full image's
+    e_d1 = float(np.mean((D1.get_output_image().astype(np.float64) - ref1) ** 2 / (ref ** 2 + 1e-2)))
     e_pt1 = pt_err[0]
     c_nrc = t_nrc / S_NRC  # per cached sample, training included
     c_pt = b_fit  # marginal cost of a plain PT sample
@@ -779,7 +783,12 @@ def bench_integrator(args, log):
     from cupbr_amd import scenes
 
     W = H = 512
-    if args.workload == "gdpt":
+    if args.workload == "cornell":
+        W = H = 256
+        desc, ca = scenes.cornell_box(W, H)
+        cam = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], W, H)
+        method, spp, depth, paths = cb.RenderingMethod.PATHTRACER, 16, 5, 1
+    elif args.workload == "gdpt":
         desc, ca = scenes.cornell_box(W, H)
         cam = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], W, H)
         method, spp, depth, paths = cb.RenderingMethod.GRADIENTDOMAIN, 128, 5, 5
@@ -821,14 +830,22 @@ def bench_integrator(args, log):
     log(f"{args.workload}: {value:.1f} Msamples/s ({paths} paths per sample), {t_total / args.steps * 1e3:.1f} ms/step, "
         f"device {dev_ms / args.steps:.1f} ms/step, {rays * args.steps / t_total / 1e6:.0f} Mrays/s, "
         f"{bytes_step / max(rays, 1):.0f} algorithmic B/ray")
+    extra = {}
+    if args.workload == "cornell":  # the frame against the FP64 oracle with the same seeds (untimed)
+        img = r.get_output_image().astype(np.float64)
+        r64 = oracle.OracleScene(desc, precisions=(64,)).render(cam.as_dict(), W, H, 0, spp, depth=depth, precision=64)
+        extra["rel_mse_vs_fp64_oracle"] = float(np.mean((img - r64) ** 2 / (r64 ** 2 + 1e-2)))
+        log(f"cornell: rel-MSE against the FP64 oracle (same seeds) {extra['rel_mse_vs_fp64_oracle']:.2e}")
     cpu = None
     if not args.no_cpu_baseline:
         hc = host_cpu()
         threads = hc["threads_used"]
         ref = oracle.OracleScene(desc, precisions=(32,))
-        cs = 128 if args.workload == "gdpt" else 640  # ~10 s of CPU work on 16 threads
+        cs = {"gdpt": 128, "volume": 640, "cornell": 256}[args.workload]  # ~10 s of CPU work on 16 threads
         t0 = time.perf_counter()
-        if args.workload == "gdpt":
+        if args.workload == "cornell":
+            ref.render(cam.as_dict(), W, H, 0, cs, depth=depth, precision=32, threads=threads)
+        elif args.workload == "gdpt":
             bufs = ref.gdpt(cam.as_dict(), W, H, 0, cs, depth=depth, precision=32, threads=threads)
             oracle.gdpt_reconstruct(bufs.astype(np.float32), 50)
         else:
@@ -851,15 +868,18 @@ def bench_integrator(args, log):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": (f"gradient-domain PT, Cornell box {W}x{H}, {spp} spp, depth {depth}, 5 paths per sample "
-                                f"+ 50 reconstruction iterations") if args.workload == "gdpt" else
-                               f"volume renderer (homogeneous media), {W}x{H}, {spp} spp, depth {depth}",
+        "config": {"workload": {"gdpt": f"gradient-domain PT, Cornell box {W}x{H}, {spp} spp, depth {depth}, 5 paths "
+                                         f"per sample + 50 reconstruction iterations",
+                                "volume": f"volume renderer (homogeneous media), {W}x{H}, {spp} spp, depth {depth}",
+                                "cornell": f"PT+NEE, Cornell box (34 triangles), {W}x{H}, {spp} spp, depth {depth}"}[
+                                    args.workload],
                    "width": W, "height": H, "spp": spp, "max_depth": depth,
-                   "mrays_per_s": round(rays * args.steps / t_total / 1e6, 1)},
+                   "mrays_per_s": round(rays * args.steps / t_total / 1e6, 1), **extra},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "gradient-domain wavefront (all launches of a render call)" if args.workload == "gdpt" else
-                               "k_volume (all launches of a render call)",
+                     "kernel": {"gdpt": "gradient-domain wavefront (all launches of a render call)",
+                                "volume": "volume wavefront (all launches of a render call)",
+                                "cornell": "path tracer wavefront (all launches of a render call)"}[args.workload],
                      "bytes_per_step": int(bytes_step), "device_ms_per_step": round(dev_ms / args.steps, 3)},
         "cpu_baseline": cpu,
     }

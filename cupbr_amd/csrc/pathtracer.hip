@@ -24,6 +24,17 @@
 #include "shading.h"
 #include "traverse.h"
 
+// Lanes of the refill loop that must be at a leaf before a wave runs a leaf step (inner-node steps
+// otherwise): 16 of 64 measured best (1080p, 1M triangles: 8 / 16 / 24 / 32 -> 901 / 932 / 920 /
+// 843 Msamples/s; leaf and inner steps taken together every iteration: 889)
+#ifndef CUPBR_LEAF_BATCH
+#define CUPBR_LEAF_BATCH 16
+#endif
+// idle lanes of a wave that trigger a refill from the ray queue
+#ifndef CUPBR_REFILL_IDLE
+#define CUPBR_REFILL_IDLE 32
+#endif
+
 namespace cupbr {
 
 constexpr int kPtBlock = 256;
@@ -102,6 +113,8 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
         c->shadow_count[1].v = 0;
         c->query_count.v = 0;
         c->row_count.v = 0;
+        c->iqueue_count[0].v = 0;
+        c->iqueue_count[1].v = 0;
     }
     if (p >= npath) return;  // p: queue slot
     const uint32_t npix = w * h;
@@ -316,7 +329,7 @@ __device__ __forceinline__ void shadow_result(bool occluded, const float4 d4, co
 // One queue of the trace kernel, n rays, with per-lane refill: the queue is split into 8
 // contiguous groups, group h's rays handed out by counter[h]. A wave starts on group g0 (its
 // block's XCD), moves to the next group once its counter has passed the group's end, and gives
-// fresh rays to its idle lanes whenever at least 32 of them are idle (or all are), so that a
+// fresh rays to its idle lanes whenever at least CUPBR_REFILL_IDLE (32) of them are idle (or all are), so that a
 // lane whose ray ended does not wait for the slowest ray of a 64-ray chunk. finish(i, hit, r)
 // consumes the result of ray i.
 template <bool kAny, class Load, class Finish>
@@ -328,9 +341,12 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
     bool active = false, pool = true, hit = false;
     uint32_t my = 0;
     RayState rs;
+#if CUPBR_SPECULATIVE
+    uint32_t pend = 0;
+#endif
     for (;;) {
         const unsigned long long idle = __ballot(!active);
-        if (pool && (__popcll(idle) >= 32 || idle == ~0ull)) {
+        if (pool && (__popcll(idle) >= CUPBR_REFILL_IDLE || idle == ~0ull)) {
             const uint32_t need = (uint32_t)__popcll(idle);
             const uint32_t leader = (uint32_t)__builtin_ctzll(idle);
             uint32_t base = 0;
@@ -344,6 +360,9 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
                     float dist;
                     load(my, o, d, dist);
                     active = ray_begin<kAny>(S, rs, o, d, dist, hit);
+#if CUPBR_SPECULATIVE
+                    pend = 0;
+#endif
                     if (!active) finish(my, hit, rs.best);
                 }
             }
@@ -356,11 +375,30 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
                 }
             }
         }
-        if (__ballot(active) == 0ull) {
+        const unsigned long long am = __ballot(active);
+        if (am == 0ull) {
             if (!pool) break;
             continue;
         }
-        if (active && !ray_step<kAny>(S, rs, st, hit)) {
+        // Wave-uniform phases: a primitive step for the lanes at a leaf once at least
+        // CUPBR_LEAF_BATCH of them are there (or no lane has an inner node left), an inner-node step
+        // for the others otherwise. A lane whose ray reached a leaf waits during inner phases, so
+        // that the primitive test runs for many lanes at once instead of a few in every iteration
+        // (before: leaf code in 97% of iterations for 9 of 64 lanes).
+#if CUPBR_SPECULATIVE
+        const bool want_leaf = active && (pend != 0u || (rs.cur != kChildEmpty && (rs.cur & kLeafBit)));
+        const bool want_inner = active && !(rs.cur & kLeafBit);
+        const uint32_t n_leaf = (uint32_t)__popcll(__ballot(want_leaf));
+        const bool leaf_phase = n_leaf >= (uint32_t)CUPBR_LEAF_BATCH || __ballot(want_inner) == 0ull;
+        const bool go = leaf_phase ? want_leaf : want_inner;
+        if (go && !ray_step_spec<kAny>(S, rs, st, pend, leaf_phase, hit)) {
+#else
+        const bool at_leaf = active && (rs.cur & kLeafBit);
+        const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
+        const bool leaf_phase = n_leaf >= (uint32_t)CUPBR_LEAF_BATCH || n_leaf == (uint32_t)__popcll(am);
+        const bool go = leaf_phase ? at_leaf : (active && !at_leaf);
+        if (go && !ray_step<kAny>(S, rs, st, hit)) {
+#endif
             finish(my, hit, rs.best);
             active = false;
         }
@@ -385,6 +423,7 @@ void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o, const float
     const uint32_t n_sh = c->shadow_count[cur ^ 1].v;
     if (gtid == 0) {
         c->queue_count[cur ^ 1].v = 0;
+        c->iqueue_count[cur ^ 1].v = 0;
         c->shadow_count[cur].v = 0;
         c->extension_rays += n_ext;
         c->shadow_rays += n_sh;
@@ -471,6 +510,7 @@ void k_pt_trace_camera(DevScene S, uint32_t n, const float4* __restrict__ q_o, c
                        float2* __restrict__ q_hit, Counters* c) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // as k_pt_trace for cur = 0 (no shadow rays yet)
         c->queue_count[1].v = 0;
+        c->iqueue_count[1].v = 0;
         c->shadow_count[0].v = 0;
         c->extension_rays += n;
     }
@@ -690,6 +730,210 @@ __global__ __launch_bounds__(kPtBlock) void k_gdpt_accumulate(float* __restrict_
     st3(gyb + i3, acc[4]);
 }
 
+// ------------------------------------------------------------------ volume renderer paths
+// volume_kernel (VolumeRenderer.cu:214-310) on the same queues, one iteration of its loop per
+// bounce. A path's medium state lives in rad[p].w: bit 31 = inside an object, bits 0..30 = that
+// object (the last interface crossed). Rays outside are traced by the trace kernels against the
+// scene; rays inside go to the inside queue and are traced by the shade kernel against their
+// object alone (Tracing::traceRay(scene, ray, index), TracingDetail.h:68-116), whose primitives it
+// loops over. A path whose weight becomes zero (absorbed, back-facing, Russian roulette) stops:
+// every later term would be multiplied by that weight.
+constexpr uint32_t kInsideBit = 0x80000000u;
+
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_vol_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int rr, const float4* __restrict__ q_o,
+                 const float4* __restrict__ q_d, const float2* __restrict__ q_hit, const float4* __restrict__ iq_o,
+                 const float4* __restrict__ iq_d, float4* __restrict__ nq_o, float4* __restrict__ nq_d,
+                 float4* __restrict__ niq_o, float4* __restrict__ niq_d, float4* __restrict__ s_o,
+                 float4* __restrict__ s_d, float4* __restrict__ s_c, float4* __restrict__ weight,
+                 float4* __restrict__ rad, Counters* c) {
+    __shared__ uint32_t s_append[kPtWaves + 1];
+    const uint32_t n_out = c->queue_count[cur].v, n_in = c->iqueue_count[cur].v, n = n_out + n_in;
+    if (blockIdx.x == 0 && threadIdx.x < 8) c->work[threadIdx.x].v = c->swork[threadIdx.x].v = 0;
+    const float nl = (float)(S.n_lights + (S.use_env ? 1u : 0u));
+    unsigned long long inside_traces = 0;
+    for (uint32_t base = blockIdx.x * kPtBlock; base < n; base += gridDim.x * kPtBlock) {
+        const uint32_t i = base + threadIdx.x;
+        bool want_next = false, want_shadow = false, next_inside = false;
+        float4 no = {}, nd = {}, so = {}, sd = {}, sc = {};
+        if (i < n) {
+            const bool from_inside = i >= n_out;
+            const float4 o4 = from_inside ? iq_o[i - n_out] : q_o[i];
+            const float4 d4 = from_inside ? iq_d[i - n_out] : q_d[i];
+            const uint32_t p = __float_as_uint(d4.w);
+            const float4 W = weight[p];
+            const float4 R = rad[p];
+            v3 w3 = mk(W.x, W.y, W.z);
+            uint32_t seed = __float_as_uint(W.w);
+            v3 L = mk(R.x, R.y, R.z);
+            uint32_t vstate = __float_as_uint(R.w);
+            bool inside = (vstate & kInsideBit) != 0u;
+            uint32_t obj_index = vstate & ~kInsideBit;
+            const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+            TraceResult hr;
+            if (from_inside) {
+                ++inside_traces;
+                object_trace(S, obj_index, o, d, hr);
+            } else {
+                const float2 hh = q_hit[i];
+                hr.t = hh.x;
+                hr.prim = __float_as_uint(hh.y);
+            }
+            const v3 inc = -1.0f * d;
+            if (hr.prim == kNoHit) {  // :253-260
+                if (S.use_env) L += w3 * env_lookup(S, d);
+            } else {
+                const v3 P = o + hr.t * d;
+                v3 N;
+                uint32_t obj;
+                hit_geometry(S, hr.prim, P, N, obj);
+                const Mat m = load_mat(S.mats + obj);
+                const int32_t vol_obj = inside ? (int32_t)obj : -1;  // the payload's volume (:277)
+                bool next_valid = false, alive = true, medium_event = false;
+                v3 out_dir = splat(0.0f), ray_start = splat(0.0f);
+                // handleMediumInteraction :88-212
+                const VolCoef vc = vol_coef(S, vol_obj);
+                const v3 st = vc.sa + vc.ss;
+                uint32_t ch = (uint32_t)(rnd(seed) * 3.0f);
+                ch = min(ch, 2u);
+                const float stc = comp(st, (int)ch);
+                if (!safe_float_equal(stc, 0.0f)) {
+                    const float t = -logf(1.0f - rnd(seed)) / stc;
+                    if (t < hr.t) {
+                        medium_event = true;
+                        const v3 ep = o + t * d;
+                        const float sprob = comp(vc.ss, (int)ch) / stc;
+                        if (rnd(seed) < sprob) {
+                            float pdf = 0.0f;
+                            for (int k = 0; k < 3; ++k) pdf += comp(st, k) * expf(-1.0f * comp(st, k) * t);
+                            pdf /= 3.0f;
+                            w3 = w3 * vc.ss / sprob * vexp(-1.0f * st * t) / pdf;
+                            v3 ldir, le;
+                            float dist;
+                            if (nee_sample(S, m, seed, ep, inc, N, ldir, le, dist)) {
+                                const v3 sdir = normalize(ldir);
+                                v3 sorg, att = splat(1.0f);
+                                if (inside) {  // to the object's boundary first (:159-171)
+                                    ++inside_traces;
+                                    TraceResult ge;
+                                    float gd = 0.0f;
+                                    v3 gp = ep;
+                                    if (object_trace(S, obj_index, ep, sdir, ge)) {
+                                        gd = ge.t;
+                                        gp = ep + ge.t * sdir;
+                                    }
+                                    att = vexp(-1.0f * st * gd) * hg_phase(vc.g, -dot(inc, ldir));
+                                    sorg = gp + 0.001f * ldir;
+                                    dist -= gd;
+                                } else {
+                                    sorg = ep + 0.001f * ldir;
+                                }
+                                const v3 sst = ld3(S.volume.sigma_a) + ld3(S.volume.sigma_s);
+                                const v3 contrib = nl * vexp(-1.0f * sst * fminf(dist, 100000000.0f)) * att * le * w3;
+                                if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
+                                    want_shadow = true;
+                                    so = make_float4(sorg.x, sorg.y, sorg.z, dist);
+                                    sd = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(p));
+                                    sc = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+                                }
+                            }
+                            const DirPdf hs = hg_sample(vc.g, -1.0f * inc, seed);
+                            out_dir = hs.d;
+                            ray_start = ep;
+                            next_valid = true;
+                        } else {
+                            alive = false;  // absorbed (:120-125)
+                        }
+                    } else {
+                        float pdf = 0.0f;
+                        for (int k = 0; k < 3; ++k) pdf += expf(-comp(st, k) * hr.t);
+                        pdf /= 3.0f;
+                        w3 = w3 * vexp(-1.0f * st * hr.t) / pdf;
+                        ray_start = P;
+                    }
+                }
+                if (alive && !medium_event) {
+                    if (m.type == CUPBR_VOLUME_MATERIAL) {  // medium interface :269-278
+                        const DirPdf sdp = sample_direction(m, seed, inc, N);
+                        out_dir = sdp.d;
+                        ray_start = P;
+                        const bool refl = dot(inc, N) * dot(out_dir, N) > 0;
+                        inside = refl ? inside : !inside;
+                        obj_index = obj;
+                        next_valid = true;
+                    } else {
+                        if (m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f) {  // :24-70
+                            w3 = splat(0.0f);
+                        } else {
+                            v3 ldir, le;
+                            float dist;
+                            if (nee_sample(S, m, seed, P, inc, N, ldir, le, dist)) {
+                                const v3 contrib = (nl * fmaxf(0.0f, dot(N, ldir))) *
+                                                   vexp(-1.0f * (vc.ss + vc.sa) * fminf(dist, 100000000.0f)) *
+                                                   brdf(m, inc, ldir, N) * le * w3;
+                                if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
+                                    const v3 sorg = P + 0.001f * ldir;
+                                    const v3 sdir = normalize(ldir);
+                                    want_shadow = true;
+                                    so = make_float4(sorg.x, sorg.y, sorg.z, dist);
+                                    sd = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(p));
+                                    sc = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+                                }
+                            }
+                        }
+                        const DirPdf sdp = sample_direction(m, seed, inc, N);  // :72-86
+                        if (!(norm(sdp.d) == 0.0f)) {
+                            w3 = w3 * fabsf(dot(sdp.d, N)) * brdf(m, inc, sdp.d, N) / sdp.pdf;
+                            out_dir = sdp.d;
+                            ray_start = P;
+                            next_valid = true;
+                        }
+                    }
+                    if (rr) {  // :280-289
+                        const float alpha = clampf(fmaxf(w3.x, fmaxf(w3.y, w3.z)), 0.0f, 1.0f);
+                        if (rnd(seed) > alpha || safe_float_equal(alpha, 0.0f)) {
+                            next_valid = false;
+                        } else {
+                            w3 = w3 / alpha;
+                        }
+                    }
+                }
+                if (alive && next_valid && bounce + 1 < max_depth && !(w3.x == 0.0f && w3.y == 0.0f && w3.z == 0.0f)) {
+                    const v3 norg = ray_start + 0.001f * out_dir;  // :298
+                    const v3 ndir = normalize(out_dir);
+                    want_next = true;
+                    next_inside = inside;
+                    no = make_float4(norg.x, norg.y, norg.z, 0.0f);
+                    nd = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(p));
+                }
+                weight[p] = make_float4(w3.x, w3.y, w3.z, __uint_as_float(seed));
+            }
+            vstate = (inside ? kInsideBit : 0u) | obj_index;
+            rad[p] = make_float4(L.x, L.y, L.z, __uint_as_float(vstate));
+        }
+        const bool app_out = want_next && !next_inside, app_in = want_next && next_inside;
+        const uint32_t ns = block_append(app_out, &c->queue_count[cur ^ 1].v, s_append);
+        if (app_out) {
+            nq_o[ns] = no;
+            nq_d[ns] = nd;
+        }
+        const uint32_t ni = block_append(app_in, &c->iqueue_count[cur ^ 1].v, s_append);
+        if (app_in) {
+            niq_o[ni] = no;
+            niq_d[ni] = nd;
+        }
+        const uint32_t ss = block_append(want_shadow, &c->shadow_count[cur].v, s_append);
+        if (want_shadow) {
+            s_o[ss] = so;
+            s_d[ss] = sd;
+            s_c[ss] = sc;
+        }
+    }
+    // object traces of this launch count as closest-hit rays
+    for (int off = 32; off > 0; off >>= 1) inside_traces += __shfl_xor(inside_traces, off);
+    if (__lane_id() == 0 && inside_traces) atomicAdd(&c->extension_rays, inside_traces);
+}
+
 static int ensure_wavefront(cupbr_renderer* r, uint32_t paths, uint32_t grid_threads) {
     Wavefront& wf = r->wf;
     int rc;
@@ -852,6 +1096,80 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
             }
             if ((rc = stage_mark(r, 3))) return rc;
         }
+    }
+    CUPBR_HIP_CHECK(hipGetLastError());
+    return CUPBR_OK;
+}
+
+int volume_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp) {
+    const uint32_t w = r->width, h = r->height, npix = w * h;
+    if (npix == 0 || spp == 0) return CUPBR_OK;
+    const uint32_t target = 1u << 27;
+    uint32_t cap = 1u << 27;
+    {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
+            cap = (uint32_t)std::min<size_t>(cap, std::max<size_t>(total_b / 8 / 264, 1u << 20));
+    }
+    uint32_t batch = std::max<uint32_t>(1, (target + npix - 1) / npix);
+    batch = std::min(batch, std::max<uint32_t>(1, cap / npix));
+    batch = std::min(batch, spp);
+    const int blocks = device_cu_count() * 8;
+    const uint32_t grid_threads = (uint32_t)blocks * kPtBlock;
+    int rc = ensure_wavefront(r, batch * npix, grid_threads);
+    if (rc) return rc;
+    Wavefront& wf = r->wf;
+    if (wf.iq_capacity < batch * npix) {
+        for (int k = 0; k < 2; ++k) {
+            if ((rc = wf.iq_o[k].ensure((size_t)batch * npix * 16)) || (rc = wf.iq_d[k].ensure((size_t)batch * npix * 16)))
+                return rc;
+        }
+        wf.iq_capacity = batch * npix;
+    }
+    Counters* c = wf.counters.as<Counters>();
+    CUPBR_HIP_CHECK(hipMemsetAsync(c, 0, sizeof(Counters), r->stream));
+    uint32_t* spill = r->scene->stack_entries > (uint32_t)kPtLds ? wf.spill.as<uint32_t>() : nullptr;
+    const DevScene& S = r->scene->dev;
+    float* img = r->images[CUPBR_IMAGE_OUTPUT].as<float>();
+    const uint32_t bounces = std::max<uint32_t>(1, r->max_trace_depth);
+    auto trace = r->counting ? k_pt_trace<true> : k_pt_trace<false>;
+    const bool packet = !r->counting && r->scene->stack_entries <= kPacketStack && S.n_nodes > 0;
+    if ((rc = stage_mark(r, -1))) return rc;
+    for (uint32_t s0 = 0; s0 < spp; s0 += batch) {
+        const uint32_t nb = std::min(batch, spp - s0);
+        const uint32_t npath = nb * npix;
+        // the path tracer's ray generation: camera rays in 8x8 tiles, seed tea<4>(pixel, frame)
+        // (VolumeRenderer.cu:229-234); rad[p].w = 0: outside every object
+        hipLaunchKernelGGL(k_pt_raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
+                           h, frame0 + s0, npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
+                           wf.weight.as<float4>(), wf.radiance.as<float4>(), c, 0u);
+        if ((rc = stage_mark(r, 3))) return rc;
+        for (uint32_t b = 0; b <= bounces; ++b) {
+            const int cur = (int)(b & 1);
+            if (b == 0 && packet) {
+                const uint32_t nblk = (npath + kPtBlock - 1) / kPtBlock;
+                hipLaunchKernelGGL(k_pt_trace_camera, dim3((nblk + 7u) / 8u * 8u), dim3(kPtBlock), 0, r->stream, S,
+                                   npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(), wf.q_hit.as<float2>(), c);
+            } else {
+                hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
+                                   wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
+                                   wf.s_d[cur ^ 1].as<float4>(), wf.s_c[cur ^ 1].as<float4>(),
+                                   wf.radiance.as<float4>(), c, spill, grid_threads);
+            }
+            if ((rc = stage_mark(r, 0))) return rc;
+            if (b == bounces) break;
+            hipLaunchKernelGGL(k_vol_shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
+                               (int)r->russian_roulette, wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),
+                               wf.q_hit.as<float2>(), wf.iq_o[cur].as<float4>(), wf.iq_d[cur].as<float4>(),
+                               wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(),
+                               wf.iq_o[cur ^ 1].as<float4>(), wf.iq_d[cur ^ 1].as<float4>(), wf.s_o[cur].as<float4>(),
+                               wf.s_d[cur].as<float4>(), wf.s_c[cur].as<float4>(), wf.weight.as<float4>(),
+                               wf.radiance.as<float4>(), c);
+            if ((rc = stage_mark(r, 1))) return rc;
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, img,
+                           wf.radiance.as<float4>(), npix, frame0 + s0, nb);
+        if ((rc = stage_mark(r, 3))) return rc;
     }
     CUPBR_HIP_CHECK(hipGetLastError());
     return CUPBR_OK;

@@ -28,6 +28,7 @@ struct Counters {
     CounterLine query_count;  // radiance-cache queries of the batch
     CounterLine query_early;  // queries that existed after the second bounce's shade
     CounterLine row_count;    // radiance-cache training rows of the batch
+    CounterLine iqueue_count[2];  // volume renderer: rays inside an object (bounce parity)
     unsigned long long extension_rays;
     unsigned long long shadow_rays;
     unsigned long long ext_nodes, ext_prims, shadow_nodes, shadow_prims;  // traversal counting
@@ -68,6 +69,8 @@ struct Wavefront {
     DevBuf counters;
     DevBuf spill;
     DevBuf gdpt_slot;  // gradient-domain groups: queue slot of the base path's ray, by bounce parity
+    DevBuf iq_o[2], iq_d[2];  // volume renderer: rays inside an object (traced against that object)
+    uint32_t iq_capacity = 0;
     uint32_t spill_entries = 0;
 };
 
@@ -119,6 +122,8 @@ int render_gdpt(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uin
 // the gradient-domain paths of frames [frame0, frame0+spp) in wavefront form: running means of the
 // base and the four gradient buffers (pathtracer.hip)
 int gdpt_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp);
+// the volume renderer's frames [frame0, frame0+spp) in wavefront form (pathtracer.hip)
+int volume_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp);
 int render_raytracer(cupbr_renderer* r, const cupbr_camera& cam);
 
 }  // namespace cupbr

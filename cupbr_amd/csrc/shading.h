@@ -257,4 +257,25 @@ __device__ __forceinline__ void tile_pixel(uint32_t j, uint32_t w, uint32_t h, u
     }
 }
 
+// coefficients of the current medium of the volume renderer: the scene's (obj < 0) or an
+// object's interior (Volume.h:15-21; VolumeRenderer.cu:88-98)
+struct VolCoef {
+    v3 sa, ss;
+    float g;
+};
+__device__ __forceinline__ VolCoef vol_coef(const DevScene& S, int32_t obj) {
+    VolCoef c;
+    if (obj < 0) {
+        c.sa = ld3(S.volume.sigma_a);
+        c.ss = ld3(S.volume.sigma_s);
+        c.g = S.volume.g;
+    } else {
+        const cupbr_volume& v = S.mats[obj].volume;
+        c.sa = ld3(v.sigma_a);
+        c.ss = ld3(v.sigma_s);
+        c.g = v.g;
+    }
+    return c;
+}
+
 }  // namespace cupbr

@@ -20,6 +20,10 @@
 
 #include "scene_types.h"
 
+#ifndef CUPBR_SPECULATIVE
+#define CUPBR_SPECULATIVE 0
+#endif
+
 namespace cupbr {
 
 // Triangle::computeRayIntersection, lib/src/Geometry/TriangleDetail.h:21-57.
@@ -314,13 +318,27 @@ __device__ __forceinline__ bool ray_begin(const DevScene& S, RayState& r, v3 o, 
     return S.n_nodes != 0;
 }
 
-// One node or leaf of a started ray; false when its traversal has ended.
+// One inner node or one primitive of a started ray; false when its traversal has ended. A leaf
+// takes one step per primitive, so that lanes at leaves of different sizes do not hold the wave
+// for the largest one.
 template <bool kAny, int kLds>
 __device__ __forceinline__ bool ray_step(const DevScene& S, RayState& r, LaneStack<kLds>& st, bool& hit) {
     if (r.cur & kLeafBit) {
-        if (visit_leaf<kAny>(S, r.cur, r.o, r.d, r.best.t, r.best)) {
-            hit = true;
-            return false;
+        // one primitive per step: the leaf word keeps the rest of the leaf (first + 1, count - 1)
+        const uint32_t first = leaf_first(r.cur);
+        const PrimRec pr = S.prims[first];
+        if (kAny) {
+            float t;
+            if (prim_test(pr, r.o, r.d, t) && t <= r.best.t && object_box_ok(S, pr, r.o, r.d)) {
+                hit = true;
+                return false;
+            }
+        } else {
+            closest_prim(S, pr, first, r.o, r.d, r.best);
+        }
+        if (leaf_count(r.cur) > 1u) {
+            r.cur += 15u;  // first + 1 (bits 4..30), count - 1 (bits 0..3)
+            return true;
         }
     } else {
         const float lim = r.best.t * 1.0001f + 1e-6f;  // the query distance of an any-hit ray
@@ -330,6 +348,49 @@ __device__ __forceinline__ bool ray_step(const DevScene& S, RayState& r, LaneSta
     r.cur = st.get(--r.sp);
     return true;
 }
+#if CUPBR_SPECULATIVE
+// Speculative form of ray_step: a leaf reached during an inner-node step is parked in `pend` and
+// the lane goes on with the next stack entry, so that it keeps stepping inner nodes until it meets
+// a second leaf. leaf_step: test one primitive of the parked leaf (or of cur when none is parked);
+// otherwise step the inner node cur. cur == kChildEmpty: the stack is exhausted. Returns false
+// when the traversal has ended.
+template <bool kAny, int kLds>
+__device__ __forceinline__ bool ray_step_spec(const DevScene& S, RayState& r, LaneStack<kLds>& st, uint32_t& pend,
+                                              bool leaf_step, bool& hit) {
+    if (leaf_step) {
+        const bool parked = pend != 0u;
+        const uint32_t lw = parked ? pend : r.cur;
+        const uint32_t first = leaf_first(lw);
+        const PrimRec pr = S.prims[first];
+        if (kAny) {
+            float t;
+            if (prim_test(pr, r.o, r.d, t) && t <= r.best.t && object_box_ok(S, pr, r.o, r.d)) {
+                hit = true;
+                return false;
+            }
+        } else {
+            closest_prim(S, pr, first, r.o, r.d, r.best);
+        }
+        const uint32_t next = leaf_count(lw) > 1u ? lw + 15u : 0u;
+        if (parked) {
+            pend = next;
+        } else if (next) {
+            r.cur = next;
+        } else {
+            r.cur = r.sp == 0 ? kChildEmpty : st.get(--r.sp);
+        }
+    } else {
+        const float lim = r.best.t * 1.0001f + 1e-6f;
+        if (!visit_inner<kAny>(S, r.cur, r.o, r.ix, r.iy, r.iz, lim, st, r.sp))
+            r.cur = r.sp == 0 ? kChildEmpty : st.get(--r.sp);
+        if (pend == 0u && r.cur != kChildEmpty && (r.cur & kLeafBit)) {
+            pend = r.cur;
+            r.cur = r.sp == 0 ? kChildEmpty : st.get(--r.sp);
+        }
+    }
+    return !(r.cur == kChildEmpty && pend == 0u);
+}
+#endif
 #undef CUPBR_CSWAP
 
 // a load whose address is the same in every lane, issued as a scalar load (records of the
@@ -469,6 +530,36 @@ __device__ __forceinline__ void traverse_packet(const DevScene& S, v3 o, v3 d, b
         traverse_packet_impl<true>(S, o, d, valid, px != 0, py != 0, pz != 0, best);
     else
         traverse_packet_impl<false>(S, o, d, valid, false, false, false, best);
+}
+
+// Tracing::traceRay(scene, ray, index), TracingDetail.h:68-116: one object's primitives in order,
+// no box test; a mesh keeps its first strictly-closer triangle (MeshDetail.h:28-35)
+__device__ __forceinline__ bool object_trace(const DevScene& S, uint32_t obj, v3 o, v3 d, TraceResult& r) {
+    r.t = __builtin_inff();
+    r.prim = kNoHit;
+    r.rank = 0xFFFFFFFFu;
+    const ObjInfo& oi = S.objs[obj];
+    for (uint32_t k = 0; k < oi.rank_count; ++k) {
+        const uint32_t prim = S.rank_prim[oi.rank_base + k];
+        if (prim == kNoHit) continue;
+        float t;
+        bool h;
+        if (prim & kUnbBit) {
+            h = plane_quad_test(S.unb[prim & ~kUnbBit], o, d, t);
+        } else {
+            const PrimRec pr = S.prims[prim];
+            if ((__float_as_uint(pr.c.w) & 3u) == kPrimTriangle)
+                h = mt_test(mk(pr.a.x, pr.a.y, pr.a.z), mk(pr.a.w, pr.b.x, pr.b.y), mk(pr.b.z, pr.b.w, pr.c.x), o, d, t);
+            else
+                h = sphere_test(mk(pr.a.x, pr.a.y, pr.a.z), pr.a.w, o, d, t);
+        }
+        if (h && t < r.t) {
+            r.t = t;
+            r.prim = prim;
+            r.rank = oi.rank_base + k;
+        }
+    }
+    return r.prim != kNoHit;
 }
 
 // object id and triangle-in-mesh index of a closest-hit result
