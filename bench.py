@@ -50,6 +50,10 @@ def main():
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nrc-spp", type=int, default=4, help="samples per render call of the radiance-cache workload")
+    ap.add_argument("--nrc-camera-roughness", default="0.1,inf",
+                    help="comma list of camera-hit roughness thresholds of the radiance cache (inf: the camera hit "
+                         "is never a cache vertex); the first one gives the reported value, the others are reported "
+                         "beside it")
     ap.add_argument("--workload", choices=["pt", "cornell", "grid", "nrc", "gdpt", "volume"], default="pt",
                     help="pt: the headline PT+NEE frame; grid: one gradient evaluation (forward + path-replay "
                          "adjoint) of the 64^3 heterogeneous medium at 256x256, 64 spp; nrc: equal-time rel-MSE of "
@@ -58,7 +62,8 @@ def main():
                          "gdpt: gradient-domain frame (5 paths per sample + 50 reconstruction iterations) of the "
                          "Cornell box at 512x512, 128 spp; volume: the homogeneous-media renderer at 512x512, 64 spp")
     ap.add_argument("--no-counters", action="store_true",
-                    help="skip the three rocprofv3 child runs that measure the trace kernels' VALU and HBM counters")
+                    help="skip the three rocprofv3 child runs that measure VALU and HBM counters (pt: of the "
+                         "trace kernels; cornell, gdpt, volume: of all kernels of a render call)")
     ap.add_argument("--counter-dir", default=None, help="directory for the rocprofv3 child runs' output")
     ap.add_argument("--counters-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -205,7 +210,8 @@ def _run_counter_child(args, counters, log, copy=False):
     out_dir = tempfile.mkdtemp(prefix="cupbr_pmc_", dir=args.counter_dir or None)
     cmd = [prof, "--pmc", *counters, "--output-format", "csv", "-d", out_dir, "-o", "c", "--", sys.executable,
            os.path.join(ROOT, "bench.py"), "--counters-child", "--spp", str(args.spp), "--width", str(args.width),
-           "--height", str(args.height), "--depth", str(args.depth)] + (["--copy-calibration", str(COPY_MIB)] if copy else [])
+           "--height", str(args.height), "--depth", str(args.depth), "--workload", args.workload] + \
+        (["--copy-calibration", str(COPY_MIB)] if copy else [])
     env = dict(os.environ, TMPDIR="/tmp")
     t0 = time.time()
     p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, env=env, start_new_session=True)
@@ -224,13 +230,21 @@ def _run_counter_child(args, counters, log, copy=False):
     return _per_dispatch(rows)
 
 
-def measure_counters(args, log):
-    """VALU and HBM counters of the trace kernels from three rocprofv3 runs of one step each."""
+def is_trace_kernel(k):
+    return "k_pt_trace_camera" in k or "k_pt_trace<false>" in k
+
+
+def is_render_kernel(k):
+    return "cupbr::" in k and "k_valu_peak" not in k
+
+
+def measure_counters(args, log, is_trace=is_trace_kernel):
+    """VALU and HBM counters of the dispatches whose kernel name satisfies is_trace (the trace
+    kernels by default) from three rocprofv3 runs of one step each."""
     res = {}
     valu = _run_counter_child(args, VALU_COUNTERS, log)
     fetch = _run_counter_child(args, ["FETCH_SIZE"], log, copy=True)
     write = _run_counter_child(args, ["WRITE_SIZE"], log, copy=True)
-    is_trace = lambda k: "k_pt_trace_camera" in k or "k_pt_trace<false>" in k  # noqa: E731
     if valu:
         tr = [d for d in valu if is_trace(d["kernel"])]
         mk = [d for d in valu if "k_valu_peak" in d["kernel"]]
@@ -239,7 +253,7 @@ def measure_counters(args, log):
             res["valu_instructions"] = sum(d.get("SQ_INSTS_VALU", 0.0) for d in tr)
             act = sum(d.get("SQ_ACTIVE_INST_VALU", 0.0) for d in tr)
             res["lane_utilisation"] = sum(d.get("SQ_THREAD_CYCLES_VALU", 0.0) for d in tr) / max(64 * act, 1.0)
-            res["per_dispatch"] = [{"kernel": "k_pt_trace_camera" if "camera" in d["kernel"] else "k_pt_trace<false>",
+            res["per_dispatch"] = [{"kernel": d["kernel"].split("(")[0].replace("void ", ""),
                                     "valu_instructions": d.get("SQ_INSTS_VALU", 0.0),
                                     "lane_utilisation": d.get("SQ_THREAD_CYCLES_VALU", 0.0) /
                                     max(64 * d.get("SQ_ACTIVE_INST_VALU", 0.0), 1.0),
@@ -259,10 +273,14 @@ def measure_counters(args, log):
         tr = [d.get(name, 0.0) for d in disp if is_trace(d["kernel"])]
         if not tr or scale is None:
             return None, scale
-        return sum(tr) * 1024.0 * scale / len(tr), scale
+        return sum(tr) * 1024.0 * scale, scale
 
-    res["fetch_bytes_per_launch"], res["fetch_scale"] = bytes_of(fetch, "FETCH_SIZE")
-    res["write_bytes_per_launch"], res["write_scale"] = bytes_of(write, "WRITE_SIZE")
+    fb, res["fetch_scale"] = bytes_of(fetch, "FETCH_SIZE")
+    wb, res["write_scale"] = bytes_of(write, "WRITE_SIZE")
+    n = max(res.get("trace_dispatches", 0), 1)
+    res["fetch_bytes_per_launch"] = None if fb is None else fb / n
+    res["write_bytes_per_launch"] = None if wb is None else wb / n
+    res["fetch_bytes_total"], res["write_bytes_total"] = fb, wb
     return res
 
 
@@ -273,6 +291,9 @@ def counters_child(args):
     from cupbr_amd import scenes
 
     cb.measure_valu_peak(VALU_FMA_ITERS)
+    work = None
+    if args.workload in ("cornell", "gdpt", "volume"):
+        work = integrator_setup(args)  # scene build and upload before the copy and the render call
     if args.copy_calibration > 0:
         import torch
 
@@ -283,6 +304,9 @@ def counters_child(args):
             torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
+    if work is not None:
+        work["step"]()
+        return
     W, H = args.width, args.height
     desc, ca = scenes.million_triangles(W, H)
     scene = cb.Scene.from_desc(desc)
@@ -684,29 +708,49 @@ def bench_nrc(args, log):
         pt_err.append(rel_mse(P.get_output_image(), ref))
     log("plain PT: " + ", ".join(f"{k} spp {t:.2f} ms {e:.4f}" for k, t, e in zip(pt_spp, pt_ms, pt_err)))
 
-    cache = cb.RadianceCache((-10.0, -10.0, -10.0), (110.0, 110.0, 110.0), seed=1)
-    N = renderer(cache)
-    t0 = time.perf_counter()
     warm = 300
-    for _ in range(warm // S_NRC):
-        N.render(cam, S_NRC)
-    N.synchronize()
-    log(f"cache: {warm} training frames in calls of {S_NRC} in {time.perf_counter() - t0:.1f}s")
-    times, errs = [], []
-    for _ in range(args.steps):
-        times.append(frame_ms(N, S_NRC))
-        errs.append(rel_mse(N.get_output_image(), ref))
-    st = N.stats()
-    t_nrc, e_nrc = float(np.median(times)), float(np.mean(errs))
-    hi = next((i for i, t in enumerate(pt_ms) if t >= t_nrc), len(pt_ms) - 1)
-    lo = max(0, hi - 1) if pt_ms[hi] > t_nrc else hi
-    ratio = pt_err[hi] / e_nrc
-    ratio_lo = pt_err[lo] / e_nrc
+
+    def run_cache(camera_min_roughness):
+        cache = cb.RadianceCache((-10.0, -10.0, -10.0), (110.0, 110.0, 110.0), seed=1)
+        N = renderer(cache)
+        N.set_radiance_cache_camera_hit(camera_min_roughness)
+        t0 = time.perf_counter()
+        for _ in range(warm // S_NRC):
+            N.render(cam, S_NRC)
+        N.synchronize()
+        t_warm = time.perf_counter() - t0
+        times, errs, imgs = [], [], []
+        for _ in range(args.steps):
+            times.append(frame_ms(N, S_NRC))
+            imgs.append(N.get_output_image().astype(np.float64))
+            errs.append(rel_mse(imgs[-1], ref))
+        st = N.stats()
+        t_c, e_c = float(np.median(times)), float(np.mean(errs))
+        # the error of the calls' mean image: their variance divided by the number of calls, plus
+        # the cache's bias (each call has fresh seeds; the cache moves a little between calls)
+        e_mean = rel_mse(np.mean(imgs, axis=0), ref)
+        hi = next((i for i, t in enumerate(pt_ms) if t >= t_c), len(pt_ms) - 1)
+        lo = max(0, hi - 1) if pt_ms[hi] > t_c else hi
+        res = {"camera_min_roughness": camera_min_roughness, "cache_call_ms": t_c, "rel_mse_cache": e_c,
+               "rel_mse_mean_of_calls": e_mean,
+               "queries": int(st["cache_queries"]), "training_rows": int(st["cache_training_rows"]),
+               "hi": hi, "lo": lo, "ratio": pt_err[hi] / e_c, "ratio_lo": pt_err[lo] / e_c, "cache": cache}
+        log(f"cache (camera hit: {'never' if camera_min_roughness == float('inf') else f'roughness >= {camera_min_roughness}'}): "
+            f"{warm} training frames in calls of {S_NRC} in {t_warm:.1f}s; {S_NRC} spp in {t_c:.2f} ms, rel-MSE "
+            f"{e_c:.4f} ({res['queries']} queries, {res['training_rows']} training rows); plain PT at {pt_spp[hi]} spp "
+            f"({pt_ms[hi]:.2f} ms, at least equal time): {pt_err[hi]:.4f}, ratio {res['ratio']:.2f}; at {pt_spp[lo]} "
+            f"spp ({pt_ms[lo]:.2f} ms): {pt_err[lo]:.4f}, ratio {res['ratio_lo']:.2f}; mean of the {args.steps} "
+            f"calls' images: rel-MSE {e_mean:.4f}")
+        return res
+
+    modes = [float(x) for x in args.nrc_camera_roughness.split(",")]
+    runs = [run_cache(m) for m in modes]
+    main_run = runs[0]
+    second = next((r_ for r_ in runs if r_["camera_min_roughness"] == float("inf")), None) or run_cache(float("inf"))
+    cache, st = main_run["cache"], {"cache_queries": main_run["queries"]}
+    t_nrc, e_nrc, hi, lo = main_run["cache_call_ms"], main_run["rel_mse_cache"], main_run["hi"], main_run["lo"]
+    ratio, ratio_lo = main_run["ratio"], main_run["ratio_lo"]
     b_fit, a_fit = np.polyfit(pt_spp, pt_ms, 1)
-    log(f"cache: {S_NRC} spp in {t_nrc:.2f} ms, rel-MSE {e_nrc:.4f} ({st['cache_queries']} queries, "
-        f"{st['cache_training_rows']} training rows); plain PT at {pt_spp[hi]} spp ({pt_ms[hi]:.2f} ms, at least equal "
-        f"time): {pt_err[hi]:.4f}, ratio {ratio:.2f}; at {pt_spp[lo]} spp ({pt_ms[lo]:.2f} ms): {pt_err[lo]:.4f}, "
-        f"ratio {ratio_lo:.2f}")
 
     # variance decomposition: the primary vertex's direct light alone (depth 1)
     ref1 = fp64_reference(1, 1024)
@@ -715,12 +759,13 @@ def bench_nrc(args, log):
     # the direct part's squared error, normalised as the metric normalises the full image's
     e_d1 = float(np.mean((D1.get_output_image().astype(np.float64) - ref1) ** 2 / (ref ** 2 + 1e-2)))
     e_pt1 = pt_err[0]
-    c_nrc = t_nrc / S_NRC  # per cached sample, training included
+    c_nrc = second["cache_call_ms"] / S_NRC  # per cached sample of the second-vertex cache, training included
     c_pt = b_fit  # marginal cost of a plain PT sample
     bound = (e_pt1 / c_pt * c_nrc) / e_d1 if e_d1 > 0 else float("nan")
     log(f"decomposition: plain PT 1 spp rel-MSE {e_pt1:.4f}, of which the primary vertex's direct light (depth 1, "
         f"{t_d1:.2f} ms) {e_d1:.4f} ({e_d1 / e_pt1:.0%}); a cache from the second vertex on, at {c_nrc:.2f} ms per "
-        f"sample against {c_pt:.2f} ms per plain sample, is bounded by an equal-time ratio of {bound:.2f}")
+        f"sample against {c_pt:.2f} ms per plain sample, is bounded by an equal-time ratio of {bound:.2f} (measured "
+        f"{second['ratio']:.2f}); only a cache vertex at the camera hit can go past it")
 
     # inference kernel alone, timed with HIP events on torch's stream
     q = int(st["cache_queries"])
@@ -753,6 +798,10 @@ def bench_nrc(args, log):
         "config": {"workload": f"PT+NEE {W}x{H}, depth {DEPTH}, {S_NRC} spp per call with a 64->4x64->3 radiance "
                                f"cache trained online after each call; plain PT at (at least) equal time, measured; "
                                f"reference 4096 spp FP64-accumulated",
+                   "cache_vertices": "the camera hit when Lambertian or GGX with roughness >= "
+                                     f"{main_run['camera_min_roughness']}, and every diffuse/glossy vertex after it"
+                                     if main_run["camera_min_roughness"] != float("inf") else
+                                     "every diffuse/glossy vertex after the camera hit",
                    "cache_spp": S_NRC, "rel_mse_cache": round(e_nrc, 5), "cache_call_ms": round(t_nrc, 3),
                    "pt_equal_time": {"spp": pt_spp[hi], "ms": round(pt_ms[hi], 3), "rel_mse": round(pt_err[hi], 5)},
                    "pt_less_time": {"spp": pt_spp[lo], "ms": round(pt_ms[lo], 3), "rel_mse": round(pt_err[lo], 5),
@@ -760,7 +809,15 @@ def bench_nrc(args, log):
                    "pt_measured": {str(k_): [round(t_, 3), round(e_, 5)] for k_, t_, e_ in zip(pt_spp, pt_ms, pt_err)},
                    "pt_ms_fit": [round(float(a_fit), 3), round(float(b_fit), 3)],
                    "direct_only_rel_mse_1spp": round(e_d1, 5), "pt_rel_mse_1spp": round(e_pt1, 5),
-                   "ceiling_ratio": round(bound, 3)},
+                   "second_vertex_cache": {"ratio": round(second["ratio"], 3),
+                                           "rel_mse_cache": round(second["rel_mse_cache"], 5),
+                                           "cache_call_ms": round(second["cache_call_ms"], 3),
+                                           "ceiling_ratio": round(bound, 3)},
+                   "modes": [{"camera_min_roughness": None if r_["camera_min_roughness"] == float("inf") else
+                              r_["camera_min_roughness"], "ratio": round(r_["ratio"], 3),
+                              "rel_mse_cache": round(r_["rel_mse_cache"], 5),
+                              "rel_mse_mean_of_calls": round(r_["rel_mse_mean_of_calls"], 5),
+                              "cache_call_ms": round(r_["cache_call_ms"], 3)} for r_ in runs]},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 5), "traffic": None, "kernel": "k_nrc_infer<false>",
                      "avg_launch_ms": round(inf_ms, 4), "queries_per_launch": int(x.shape[0])},
@@ -769,17 +826,13 @@ def bench_nrc(args, log):
     print(json.dumps(out), flush=True)
 
 
-def bench_integrator(args, log):
-    """The other integrators: GDPT (Cornell box 512x512, 128 spp, depth 5; every sample traces the
-    base path and 4 neighbour paths on the wavefront queues, then 50 reconstruction iterations) or
-    the volume renderer (one thread per pixel; homogeneous media: two scattering spheres in a
-    glass-walled box and a scene medium, 512x512, 64 spp, depth 8). One step = reset + one render
-    call. Roofline: the algorithmic BVH bytes of all traversals of a step (counted in an
-    instrumented pass) over the render call's device time (HIP events on the renderer's stream)."""
-    import numpy as np
-
+def integrator_setup(args):
+    """Scene, camera and renderer of the Cornell-box path tracer (256x256, 16 spp, depth 5), the
+    gradient-domain workload (Cornell box 512x512, 128 spp, depth 5; every sample traces the base
+    path and 4 neighbour paths on the wavefront queues, then 50 reconstruction iterations) or the
+    volume workload (homogeneous media: two scattering spheres in a glass-walled box and a scene
+    medium, 512x512, 64 spp, depth 8); step() = reset + one render call."""
     import cupbr_amd as cb
-    import oracle
     from cupbr_amd import scenes
 
     W = H = 512
@@ -809,6 +862,24 @@ def bench_integrator(args, log):
         r.render(cam, spp)
         r.synchronize()
 
+    return {"desc": desc, "cam": cam, "scene": scene, "renderer": r, "W": W, "H": H, "spp": spp, "depth": depth,
+            "paths": paths, "step": step}
+
+
+def bench_integrator(args, log):
+    """The Cornell-box path tracer, the gradient-domain and the volume workloads (integrator_setup).
+    Roofline: VALU issue of all kernels of a render call, from counters of one render call in
+    rocprofv3 child runs, over the call's device time (HIP events on the renderer's stream); the
+    algorithmic BVH bytes of all traversals (counted in an instrumented pass) are reported beside
+    it: these scenes' BVHs stay in L1/L2."""
+    import numpy as np
+
+    import oracle
+
+    wk = integrator_setup(args)
+    desc, cam, r, step = wk["desc"], wk["cam"], wk["renderer"], wk["step"]
+    W, H, spp, depth, paths = wk["W"], wk["H"], wk["spp"], wk["depth"], wk["paths"]
+
     r.set_traversal_counting(True)
     step()
     sc = r.stats()
@@ -818,18 +889,51 @@ def bench_integrator(args, log):
     for _ in range(args.warmup):
         step()
     dev_ms = 0.0
+    step_s = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        t1 = time.perf_counter()
         step()
+        step_s.append(time.perf_counter() - t1)
         dev_ms += r.stats()["milliseconds"]
     t_total = time.perf_counter() - t0
-    samples = W * H * spp * args.steps
-    value = samples / t_total / 1e6
-    achieved = bytes_step * args.steps / (dev_ms * 1e-3) / 1e9
+    t_med = float(np.median(step_s))
+    value = W * H * spp / t_med / 1e6  # median step
+    alg_gbs = bytes_step * args.steps / (dev_ms * 1e-3) / 1e9
+    dev_step_ms = dev_ms / args.steps
     rays = sc["extension_rays"] + sc["shadow_rays"]
-    log(f"{args.workload}: {value:.1f} Msamples/s ({paths} paths per sample), {t_total / args.steps * 1e3:.1f} ms/step, "
-        f"device {dev_ms / args.steps:.1f} ms/step, {rays * args.steps / t_total / 1e6:.0f} Mrays/s, "
+    log(f"{args.workload}: {value:.1f} Msamples/s ({paths} paths per sample; median step {t_med * 1e3:.2f} ms), "
+        f"device {dev_step_ms:.2f} ms/step, {rays / t_med / 1e6:.0f} Mrays/s, "
         f"{bytes_step / max(rays, 1):.0f} algorithmic B/ray")
+    kernel = {"gdpt": "gradient-domain wavefront + reconstruction (all kernels of a render call)",
+              "volume": "volume wavefront (all kernels of a render call)",
+              "cornell": "path tracer wavefront (all kernels of a render call)"}[args.workload]
+    roof = {"bound": "valu", "achieved": None, "peak": round(VALU_PEAK_TOPS, 3), "unit": "TOP/s", "frac": None,
+            "traffic": None, "kernel": kernel, "op": "one VALU instruction on one active lane",
+            "device_ms_per_step": round(dev_step_ms, 3), "algorithmic_bvh_bytes_per_step": int(bytes_step),
+            "algorithmic_bvh_gbs": round(alg_gbs, 1)}
+    if not args.no_counters:
+        c = measure_counters(args, log, is_render_kernel)
+        t_dev = dev_step_ms * 1e-3  # the child's dispatches are one render call
+        if c.get("valu_instructions"):
+            issue = c["valu_instructions"] / t_dev
+            lane = c["lane_utilisation"]
+            achieved = issue * 64 * lane / 1e12
+            roof.update({"achieved": round(achieved, 3), "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                         "lane_utilisation": round(lane, 4),
+                         "valu_issue_of_peak": round(issue * 64 / 1e12 / VALU_PEAK_TOPS, 4),
+                         "valu_instructions_per_step": int(c["valu_instructions"]), "dispatches": c["trace_dispatches"]})
+            log(f"{args.workload} VALU: {c['valu_instructions'] / 1e9:.2f} G wave instructions per render call in "
+                f"{c['trace_dispatches']} dispatches, lane utilisation {lane:.3f}, issue "
+                f"{issue * 64 / 1e12 / VALU_PEAK_TOPS:.3f} of peak; useful {achieved:.2f} TOP/s = "
+                f"{achieved / VALU_PEAK_TOPS:.3f} of peak")
+        if c.get("fetch_bytes_total") is not None and c.get("write_bytes_total") is not None:
+            traffic = c["fetch_bytes_total"] + c["write_bytes_total"]
+            hbm_gbs = traffic / t_dev / 1e9
+            roof.update({"traffic": int(traffic), "traffic_per": "render call", "hbm_gbs": round(hbm_gbs, 1),
+                         "hbm_of_peak": round(hbm_gbs / HBM_PEAK_GBS, 4)})
+            log(f"{args.workload} HBM traffic: {traffic / 1e9:.3f} GB per render call = {hbm_gbs:.0f} GB/s = "
+                f"{hbm_gbs / HBM_PEAK_GBS:.3f} of HBM peak")
     extra = {}
     if args.workload == "cornell":  # the frame against the FP64 oracle with the same seeds (untimed)
         img = r.get_output_image().astype(np.float64)
@@ -868,19 +972,14 @@ def bench_integrator(args, log):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": {"gdpt": f"gradient-domain PT, Cornell box {W}x{H}, {spp} spp, depth {depth}, 5 paths "
+        "config": {"value_from": "median step time","workload": {"gdpt": f"gradient-domain PT, Cornell box {W}x{H}, {spp} spp, depth {depth}, 5 paths "
                                          f"per sample + 50 reconstruction iterations",
                                 "volume": f"volume renderer (homogeneous media), {W}x{H}, {spp} spp, depth {depth}",
                                 "cornell": f"PT+NEE, Cornell box (34 triangles), {W}x{H}, {spp} spp, depth {depth}"}[
                                     args.workload],
                    "width": W, "height": H, "spp": spp, "max_depth": depth,
-                   "mrays_per_s": round(rays * args.steps / t_total / 1e6, 1), **extra},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": {"gdpt": "gradient-domain wavefront (all launches of a render call)",
-                                "volume": "volume wavefront (all launches of a render call)",
-                                "cornell": "path tracer wavefront (all launches of a render call)"}[args.workload],
-                     "bytes_per_step": int(bytes_step), "device_ms_per_step": round(dev_ms / args.steps, 3)},
+                   "mrays_per_s": round(rays / t_med / 1e6, 1), **extra},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

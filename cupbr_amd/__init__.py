@@ -53,6 +53,7 @@ class RenderingMethod(enum.IntEnum):  # PBRenderer.h:14-22
 class ToneMappingType(enum.IntEnum):  # ToneMapper.h:13-17
     REINHARD = 0
     GAMMA = 1
+    ACES = 2  # not in the reference: Narkowicz's ACES filmic fit, then gamma 1/2.2
 
 
 class GeometryType(enum.IntEnum):  # Geometry.h:15-23
@@ -236,6 +237,7 @@ def lib():
             "cupbr_nrc_train_device": (C.c_int, [P, P, U32, U32, U32, C.c_float, P, P]),
             "cupbr_nrc_matrix_core_product": (C.c_int, [P, P, P]),
             "cupbr_renderer_set_radiance_cache": (C.c_int, [P, P, U32, U32, U32, C.c_float]),
+            "cupbr_renderer_set_radiance_cache_camera_hit": (C.c_int, [P, C.c_float]),
             "cupbr_renderer_save_pfm": (C.c_int, [P, I32, C.c_char_p]),
             "cupbr_pick": (C.c_int, [P, P, U32, U32, U32, U32, P, P]),
             "cupbr_scene_update_material": (C.c_int, [P, I32, P]),
@@ -847,6 +849,12 @@ class PBRenderer:
         self._cache = cache
         _check(lib().cupbr_renderer_set_radiance_cache(self._h, None if cache is None else cache.handle,
                                                        training_paths, train_steps, batch, learning_rate))
+
+    def set_radiance_cache_camera_hit(self, min_roughness: float = float("inf")):
+        """Paths also end into the cache at the camera hit on Lambertian surfaces (min_roughness <= 1)
+        and GGX surfaces of roughness >= min_roughness; inf = never (the default). See
+        cupbr_renderer_set_radiance_cache_camera_hit."""
+        _check(lib().cupbr_renderer_set_radiance_cache_camera_hit(self._h, float(min_roughness)))
 
     def set_gdpt_reconstruction(self, mode: int = 0, alpha: float = 0.2, iterations: int = 50):
         """GRADIENTDOMAIN reconstruction: GDPT_TRIMMED_MEAN (the reference's) or

@@ -692,7 +692,7 @@ int cupbr_post_get_buffer(const cupbr_post* p, float* host, size_t n) {
 int cupbr_tonemapper_create(int32_t type, cupbr_tonemapper** out) {
     CHECK_ARG(out, "null output handle");
     *out = nullptr;
-    CHECK_ARG(type == CUPBR_REINHARD || type == CUPBR_GAMMA, "unknown tone mapping type");
+    CHECK_ARG(type == CUPBR_REINHARD || type == CUPBR_GAMMA || type == CUPBR_ACES, "unknown tone mapping type");
     auto* t = new (std::nothrow) cupbr_tonemapper();
     if (!t) return set_error(CUPBR_ERR_OUT_OF_MEMORY, "out of host memory");
     t->type = type;
@@ -764,7 +764,7 @@ int cupbr_tonemapper_get_render_buffer(const cupbr_tonemapper* t, uint8_t* host,
 
 int cupbr_tonemapper_set_type(cupbr_tonemapper* t, int32_t type) {
     CHECK_ARG(t, "null tone mapper");
-    CHECK_ARG(type == CUPBR_REINHARD || type == CUPBR_GAMMA, "unknown tone mapping type");
+    CHECK_ARG(type == CUPBR_REINHARD || type == CUPBR_GAMMA || type == CUPBR_ACES, "unknown tone mapping type");
     t->type = type;
     return CUPBR_OK;
 }
@@ -967,5 +967,13 @@ int cupbr_renderer_set_radiance_cache(cupbr_renderer* r, cupbr_nrc* cache, uint3
     r->nrc_steps = train_steps;
     r->nrc_batch = batch;
     r->nrc_lr = lr;
+    return CUPBR_OK;
+}
+
+int cupbr_renderer_set_radiance_cache_camera_hit(cupbr_renderer* r, float min_roughness) {
+    CHECK_ARG(r, "null renderer");
+    CHECK_ARG(!std::isnan(min_roughness), "min_roughness is NaN");
+    CUPBR_HIP_CHECK(hipStreamSynchronize(r->stream));
+    r->nrc_camera_min_roughness = min_roughness;
     return CUPBR_OK;
 }

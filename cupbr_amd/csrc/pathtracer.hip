@@ -140,6 +140,9 @@ struct NrcShade {
     float4* queries;  // 5 float4 per query
     float4* pending;  // 6 float4 per row
     uint32_t max_rows;
+    // the camera hit is also a cache vertex when its material is Lambertian or GGX with roughness
+    // >= camera_min_roughness (+inf: never)
+    float camera_min_roughness;
 };
 
 template <bool kNrc>
@@ -182,8 +185,11 @@ __device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bo
                 L += w3 * m.albedo_e;  // emissiveIllumintation :20-26
                 bool alive = true;
                 bool cached = false;
-                if (kNrc && bounce >= 1 && (m.type == CUPBR_LAMBERT || m.type == CUPBR_PHONG || m.type == CUPBR_GGX) &&
-                    !(m.type != CUPBR_GLASS && dot(N, inc) <= 0.0f)) {
+                const bool cache_vertex =
+                    bounce >= 1 ? (m.type == CUPBR_LAMBERT || m.type == CUPBR_PHONG || m.type == CUPBR_GGX)
+                                : ((m.type == CUPBR_LAMBERT && nrc.camera_min_roughness <= 1.0f) ||
+                                   (m.type == CUPBR_GGX && m.roughness >= nrc.camera_min_roughness));
+                if (kNrc && cache_vertex && !(dot(N, inc) <= 0.0f)) {
                     rec[0] = make_float4(P.x, P.y, P.z, inc.x);
                     rec[1] = make_float4(inc.y, inc.z, N.x, N.y);
                     rec[2] = make_float4(N.z, m.roughness, m.albedo_d.x, m.albedo_d.y);
@@ -341,9 +347,6 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
     bool active = false, pool = true, hit = false;
     uint32_t my = 0;
     RayState rs;
-#if CUPBR_SPECULATIVE
-    uint32_t pend = 0;
-#endif
     for (;;) {
         const unsigned long long idle = __ballot(!active);
         if (pool && (__popcll(idle) >= CUPBR_REFILL_IDLE || idle == ~0ull)) {
@@ -360,9 +363,6 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
                     float dist;
                     load(my, o, d, dist);
                     active = ray_begin<kAny>(S, rs, o, d, dist, hit);
-#if CUPBR_SPECULATIVE
-                    pend = 0;
-#endif
                     if (!active) finish(my, hit, rs.best);
                 }
             }
@@ -385,20 +385,11 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
         // for the others otherwise. A lane whose ray reached a leaf waits during inner phases, so
         // that the primitive test runs for many lanes at once instead of a few in every iteration
         // (before: leaf code in 97% of iterations for 9 of 64 lanes).
-#if CUPBR_SPECULATIVE
-        const bool want_leaf = active && (pend != 0u || (rs.cur != kChildEmpty && (rs.cur & kLeafBit)));
-        const bool want_inner = active && !(rs.cur & kLeafBit);
-        const uint32_t n_leaf = (uint32_t)__popcll(__ballot(want_leaf));
-        const bool leaf_phase = n_leaf >= (uint32_t)CUPBR_LEAF_BATCH || __ballot(want_inner) == 0ull;
-        const bool go = leaf_phase ? want_leaf : want_inner;
-        if (go && !ray_step_spec<kAny>(S, rs, st, pend, leaf_phase, hit)) {
-#else
         const bool at_leaf = active && (rs.cur & kLeafBit);
         const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
         const bool leaf_phase = n_leaf >= (uint32_t)CUPBR_LEAF_BATCH || n_leaf == (uint32_t)__popcll(am);
         const bool go = leaf_phase ? at_leaf : (active && !at_leaf);
         if (go && !ray_step<kAny>(S, rs, st, hit)) {
-#endif
             finish(my, hit, rs.best);
             active = false;
         }
@@ -998,14 +989,15 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     // counts define the algorithmic bytes of a trace launch)
     const bool packet = !r->counting && r->scene->stack_entries <= kPacketStack && S.n_nodes > 0;
     auto shade = nrc ? k_pt_shade_nrc : k_pt_shade;
-    NrcShade ns{nullptr, nullptr, 0};
+    NrcShade ns{nullptr, nullptr, 0, __builtin_inff()};
     uint32_t train_thresh = 0;
     if (nrc) {
         if ((rc = r->nrc_queries.ensure((size_t)batch * npix * 80)) ||
             (rc = r->nrc_pending.ensure((size_t)r->nrc_max_rows * 96)) ||
             (rc = r->nrc_rows.ensure((size_t)r->nrc_max_rows * 80)))
             return rc;
-        ns = NrcShade{r->nrc_queries.as<float4>(), r->nrc_pending.as<float4>(), r->nrc_max_rows};
+        ns = NrcShade{r->nrc_queries.as<float4>(), r->nrc_pending.as<float4>(), r->nrc_max_rows,
+                      r->nrc_camera_min_roughness};
         if (!r->side_stream) {
             if (hipStreamCreateWithFlags(&r->side_stream, hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&r->ev_queries, hipEventDisableTiming) != hipSuccess ||

@@ -8,9 +8,12 @@
 //     down0  = threshold(hdr),  down1(q) = box2x2(down0)(2q)   (edge-clamped)
 // This file computes exactly that, in the reference's FP32 operation order, in
 // one pass per 64x8 tile: the tile's 34x6 block of down1 (with its one-texel
-// halo) is built in LDS from L2-resident hdr reads, then every output pixel
-// reads its own hdr value once. HBM traffic is one FP32 RGB read and one write
-// per pixel (+ 4 bytes when the RGBA8 tone map is fused in).
+// halo) is built in LDS from hdr reads, then every output pixel reads its own
+// hdr value again (an L1/L2 hit). A tile's reads reach 2 pixels into each
+// neighbouring tile, so tiles are assigned to XCDs in contiguous row-major
+// bands: the neighbours run on the same XCD at about the same time and their
+// shared lines come from that XCD's L2. HBM traffic is then about one FP32 RGB
+// read and one write per pixel (+ 4 bytes when the RGBA8 tone map is fused in).
 #include <cstdio>
 #include <vector>
 
@@ -42,6 +45,10 @@ __device__ __forceinline__ uchar4 tonemap_px(v3 c, int32_t type, float exposure)
         if (type == CUPBR_REINHARD) {
             const float base = (float)(1.0 - (double)expf(-cc[k] * exposure));
             m[k] = powf(base, k == 2 ? (float)(1.0f / 2.2) : 1.0f / 2.2f);
+        } else if (type == CUPBR_ACES) {  // Narkowicz's ACES filmic fit, then gamma 1/2.2
+            const float x = cc[k] * exposure;
+            const float f = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+            m[k] = powf(clampf(f, 0.0f, 1.0f), 1.0f / 2.2f);
         } else {
             const float x = cc[k];
             m[k] = x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
@@ -59,7 +66,13 @@ __global__ __launch_bounds__(256) void k_bloom(const float* __restrict__ hdr, fl
                                                uchar4* __restrict__ rgba, uint32_t w, uint32_t h, float4 curve,
                                                int32_t type, float exposure) {
     __shared__ float d1[3][kD1H][kD1W];
-    const int x0 = blockIdx.x * kTileW, y0 = blockIdx.y * kTileH;
+    // workgroups are dispatched to the 8 XCDs round-robin by block id: block b runs on XCD b % 8,
+    // which takes tiles [xcd * q + min(xcd, r), ... + its block count) in row-major order
+    // (a bijection onto the n tiles: XCD x has q + (x < r) blocks)
+    const uint32_t n = gridDim.x, q = n / 8u, rem = n % 8u, xcd = blockIdx.x % 8u;
+    const uint32_t tile = xcd * q + (xcd < rem ? xcd : rem) + blockIdx.x / 8u;
+    const uint32_t tiles_x = (w + kTileW - 1) / kTileW;
+    const int x0 = (int)(tile % tiles_x) * kTileW, y0 = (int)(tile / tiles_x) * kTileH;
     const int w1 = (int)(w / 2), h1 = (int)(h / 2);
     const int qx0 = x0 / 2 - 1, qy0 = y0 / 2 - 1;
     const int tid = threadIdx.x;
@@ -116,7 +129,8 @@ __global__ __launch_bounds__(256) void k_tonemap(const float* __restrict__ hdr, 
 int post_bloom(cupbr_post* p, const float curve[4], uint8_t* rgba, int32_t type, float exposure) {
     if (p->width < 2 || p->height < 2)
         return set_error(CUPBR_ERR_INVALID_ARGUMENT, "bloom needs an image of at least 2x2 pixels");
-    dim3 grid((p->width + kTileW - 1) / kTileW, (p->height + kTileH - 1) / kTileH);
+    const uint32_t tiles = ((p->width + kTileW - 1) / kTileW) * ((p->height + kTileH - 1) / kTileH);
+    dim3 grid(tiles);
     float4 c = make_float4(curve[0], curve[1], curve[2], curve[3]);
     // the registered image is borrowed: finish pending writes to it first, and return only when
     // the outputs are complete (the reference synchronises after every launch, PyramidConstructor.cu:134)

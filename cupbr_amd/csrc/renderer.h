@@ -102,6 +102,7 @@ struct cupbr_renderer {
     uint32_t nrc_steps = 16;              // Adam steps per frame
     uint32_t nrc_batch = 1u << 14;        // rows per step
     float nrc_lr = 1e-3f;
+    float nrc_camera_min_roughness = __builtin_inff();  // camera-hit cache vertices (+inf: none)
     uint32_t nrc_max_rows = 1u << 18;     // training rows kept per frame
     cupbr::DevBuf nrc_queries, nrc_pending, nrc_rows;
     // second stream for cache inference that overlaps the training paths' later bounces

@@ -20,10 +20,6 @@
 
 #include "scene_types.h"
 
-#ifndef CUPBR_SPECULATIVE
-#define CUPBR_SPECULATIVE 0
-#endif
-
 namespace cupbr {
 
 // Triangle::computeRayIntersection, lib/src/Geometry/TriangleDetail.h:21-57.
@@ -348,49 +344,6 @@ __device__ __forceinline__ bool ray_step(const DevScene& S, RayState& r, LaneSta
     r.cur = st.get(--r.sp);
     return true;
 }
-#if CUPBR_SPECULATIVE
-// Speculative form of ray_step: a leaf reached during an inner-node step is parked in `pend` and
-// the lane goes on with the next stack entry, so that it keeps stepping inner nodes until it meets
-// a second leaf. leaf_step: test one primitive of the parked leaf (or of cur when none is parked);
-// otherwise step the inner node cur. cur == kChildEmpty: the stack is exhausted. Returns false
-// when the traversal has ended.
-template <bool kAny, int kLds>
-__device__ __forceinline__ bool ray_step_spec(const DevScene& S, RayState& r, LaneStack<kLds>& st, uint32_t& pend,
-                                              bool leaf_step, bool& hit) {
-    if (leaf_step) {
-        const bool parked = pend != 0u;
-        const uint32_t lw = parked ? pend : r.cur;
-        const uint32_t first = leaf_first(lw);
-        const PrimRec pr = S.prims[first];
-        if (kAny) {
-            float t;
-            if (prim_test(pr, r.o, r.d, t) && t <= r.best.t && object_box_ok(S, pr, r.o, r.d)) {
-                hit = true;
-                return false;
-            }
-        } else {
-            closest_prim(S, pr, first, r.o, r.d, r.best);
-        }
-        const uint32_t next = leaf_count(lw) > 1u ? lw + 15u : 0u;
-        if (parked) {
-            pend = next;
-        } else if (next) {
-            r.cur = next;
-        } else {
-            r.cur = r.sp == 0 ? kChildEmpty : st.get(--r.sp);
-        }
-    } else {
-        const float lim = r.best.t * 1.0001f + 1e-6f;
-        if (!visit_inner<kAny>(S, r.cur, r.o, r.ix, r.iy, r.iz, lim, st, r.sp))
-            r.cur = r.sp == 0 ? kChildEmpty : st.get(--r.sp);
-        if (pend == 0u && r.cur != kChildEmpty && (r.cur & kLeafBit)) {
-            pend = r.cur;
-            r.cur = r.sp == 0 ? kChildEmpty : st.get(--r.sp);
-        }
-    }
-    return !(r.cur == kChildEmpty && pend == 0u);
-}
-#endif
 #undef CUPBR_CSWAP
 
 // a load whose address is the same in every lane, issued as a scalar load (records of the

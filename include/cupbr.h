@@ -67,8 +67,11 @@ enum cupbr_material_type {
 enum cupbr_interface { CUPBR_INTERFACE_NONE = 0, CUPBR_INTERFACE_GLASS = 1 };
 /* LightType, lib/include/DataStructure/Light.h:11-15 */
 enum cupbr_light_type { CUPBR_POINT = 0, CUPBR_AREA = 1 };
-/* ToneMappingType, lib/include/Renderer/ToneMapper.h:13-17 */
-enum cupbr_tone_mapping_type { CUPBR_REINHARD = 0, CUPBR_GAMMA = 1 };
+/* ToneMappingType, lib/include/Renderer/ToneMapper.h:13-17. CUPBR_ACES has no counterpart in the
+ * reference: Narkowicz's fit of the ACES filmic curve, f(x) = x(2.51x + 0.03) / (x(2.43x + 0.59) +
+ * 0.14) of exposure * radiance, clamped to [0, 1], then the 1/2.2 gamma of the REINHARD mode and the
+ * same truncating 8-bit cast. */
+enum cupbr_tone_mapping_type { CUPBR_REINHARD = 0, CUPBR_GAMMA = 1, CUPBR_ACES = 2 };
 /* images a renderer owns (PBRenderer.cu:35-43) */
 enum cupbr_image {
     CUPBR_IMAGE_OUTPUT = 0,     /* getOutputImage(), PBRenderer.cu:232-236 */
@@ -445,6 +448,13 @@ int cupbr_nrc_train_device(cupbr_nrc* cache, const float* rows, uint32_t n_rows,
  * 512) at learning rate lr. Images stay the running mean of frames. */
 int cupbr_renderer_set_radiance_cache(cupbr_renderer* r, cupbr_nrc* cache, uint32_t training_paths,
                                       uint32_t train_steps, uint32_t batch, float learning_rate);
+/* Makes the camera hit a cache vertex too, for rough surfaces: a path also ends there (and a
+ * training path leaves a row there) when the hit material is Lambertian and min_roughness <= 1, or
+ * GGX with roughness >= min_roughness. +INFINITY (the default) keeps the camera hit out of the
+ * cache. The pixel is then the camera hit's emission plus the cache's prediction of the light it
+ * reflects: no next-event estimation there, and the image carries the cache's approximation error
+ * (its rel-MSE against a converged reference includes that bias). */
+int cupbr_renderer_set_radiance_cache_camera_hit(cupbr_renderer* r, float min_roughness);
 /* self-check of the matrix-core fragment layout: d[16][16] = a[16][32] b[32][16] (host, row-major,
  * inputs rounded to BF16) through one instruction */
 int cupbr_nrc_matrix_core_product(const float* a, const float* b, float* d);

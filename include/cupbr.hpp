@@ -41,7 +41,11 @@ enum class RenderingMethod : int32_t {  // PBRenderer.h:14-22
     VOLUME = CUPBR_VOLUME
 };
 
-enum class ToneMappingType : int32_t { REINHARD = CUPBR_REINHARD, GAMMA = CUPBR_GAMMA };  // ToneMapper.h:13-17
+enum class ToneMappingType : int32_t {  // ToneMapper.h:13-17 (+ ACES, an addition)
+    REINHARD = CUPBR_REINHARD,
+    GAMMA = CUPBR_GAMMA,
+    ACES = CUPBR_ACES
+};
 
 struct Vector4float {
     float x, y, z, w;

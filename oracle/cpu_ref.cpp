@@ -1540,6 +1540,10 @@ void orc_tonemap(const float* hdr, uint64_t n, int type, float exposure, uint8_t
                 double base = 1.0 - (double)std::exp(-c[k] * exposure);
                 float ex = k == 2 ? (float)(1.0f / 2.2) : 1.0f / 2.2f;
                 m = std::pow((float)base, ex);
+            } else if (type == 2) {  // ACES filmic fit (Narkowicz 2015), then gamma 1/2.2: an addition
+                float x = c[k] * exposure;
+                float f = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+                m = std::pow(clampf(f, 0.0f, 1.0f), 1.0f / 2.2f);
             } else {
                 float x = c[k];
                 m = x <= 0.0031308f ? 12.92f * x : 1.055f * std::pow(x, 1.0f / 2.4f) - 0.055f;

@@ -113,7 +113,7 @@ def test_post_chain_on_a_1080p_frame(million):
     got = post.get_post_process_buffer()
     want = oracle.bloom(hdr, curve)
     assert np.array_equal(got, want), float(np.abs(got - want).max())
-    for kind in (cb.ToneMappingType.REINHARD, cb.ToneMappingType.GAMMA):
+    for kind in (cb.ToneMappingType.REINHARD, cb.ToneMappingType.GAMMA, cb.ToneMappingType.ACES):
         tm = cb.ToneMapper(kind)
         tm.set_exposure(1.0)
         tm.register_image(post.buffer_device_ptr(), w, h)

@@ -217,6 +217,57 @@ def test_cache_paths_keep_the_estimator():
     assert not np.array_equal(cache2.get_weights(), w0)
 
 
+def test_camera_hit_cache_vertices_keep_the_estimator():
+    """With the camera hit a cache vertex, a zero cache leaves each pixel its camera hit's emission:
+    on a Cornell box whose only light is an emissive ceiling (no light list, so next-event estimation
+    adds nothing) and whose floor is GGX of roughness 0.3, the image equals plain PT with depth 1.
+    With a threshold above 0.3 the floor is no camera-hit vertex: its pixels take what the second
+    vertex emits, every other pixel stays. With every path a training path the image equals plain
+    PT, no path queries the cache, and the camera hits add training rows."""
+    w = h = 64
+    desc, ca = scenes.cornell_box()
+    desc["lights"] = desc["lights"][:0]
+    emissive = np.array(desc["mats"], np.float32)
+    emissive[2:4, 0:3] = 4.0  # the ceiling's two triangles
+    emissive[0:2] = scenes.material(scenes.GGX, albedo_s=(0.8, 0.8, 0.8), roughness=0.3)  # the floor
+    desc["mats"] = emissive
+    scene = cb.Scene.from_desc(desc)
+    cam = cb.Camera.look_at(ca["eye"], ca["target"], ca["up"], ca["vfov"], w, h)
+
+    def cached(min_roughness, **kw):
+        cache = cb.RadianceCache((0, 0, 0), (1, 1, 1))
+        cache.set_weights(np.zeros(cb.NRC_PARAMS, np.float32))
+        r = cb.PBRenderer(cb.RenderingMethod.PATHTRACER)
+        r.set_output_size(w, h)
+        r.register_scene(scene)
+        r.set_max_trace_depth(5)
+        r.set_radiance_cache(cache, **kw)
+        r.set_radiance_cache_camera_hit(min_roughness)
+        r.render(cam, 4)
+        return r
+
+    depth1 = _pt(scene, cam, w, h, 4, depth=1).get_output_image()
+    assert depth1.max() > 0 and np.count_nonzero(depth1) < depth1.size  # the ceiling seen, the rest black
+    a = cached(0.0, training_paths=0, train_steps=0)
+    assert np.array_equal(a.get_output_image(), depth1)
+    assert 0 < a.stats()["cache_queries"] <= w * h * 4  # one per camera ray that hits the box
+    floor = cached(0.5, training_paths=0, train_steps=0).get_output_image()
+    changed = np.any(floor != depth1, axis=2)
+    assert changed.any() and np.all(floor[changed] >= depth1[changed])
+    assert np.count_nonzero(changed) < w * h // 2  # the floor's pixels only
+    # Lambertian camera hits are vertices only for thresholds <= 1: above it the second vertex is the
+    # first cache vertex, as without the option
+    second = cached(float("inf"), training_paths=0, train_steps=0).get_output_image()
+    assert np.array_equal(cached(1.5, training_paths=0, train_steps=0).get_output_image(), second)
+    assert not np.array_equal(second, depth1)
+    full = cached(0.0, training_paths=w * h, train_steps=1, batch=512)
+    st = full.stats()
+    assert np.array_equal(full.get_output_image(), _pt(scene, cam, w, h, 4).get_output_image())
+    assert st["cache_queries"] == 0
+    rows_second = cached(float("inf"), training_paths=w * h, train_steps=1, batch=512).stats()["cache_training_rows"]
+    assert st["cache_training_rows"] > rows_second
+
+
 def rel_mse(x, ref):
     return float(np.mean((x.astype(np.float64) - ref) ** 2 / (ref ** 2 + 1e-2)))
 

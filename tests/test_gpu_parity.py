@@ -435,7 +435,7 @@ def test_bloom_and_tonemap_parity():
         out = post.get_post_process_buffer()
         want = oracle.bloom(hdr, curve)
         assert np.array_equal(out, want), (h, w, float(np.abs(out - want).max()))
-        for kind in (cb.ToneMappingType.REINHARD, cb.ToneMappingType.GAMMA):
+        for kind in (cb.ToneMappingType.REINHARD, cb.ToneMappingType.GAMMA, cb.ToneMappingType.ACES):
             tm = cb.ToneMapper(kind)
             tm.set_exposure(0.8)
             tm.register_image(post.buffer_device_ptr(), w, h)

@@ -508,15 +508,20 @@ def test_bloom_energy_conservation():
 
 
 def test_tonemap_against_fp64_formulas():
-    """Reinhard-named exponential map and the sRGB OETF (ToneMapper.cu:30-36,45-48) in FP64 numpy;
-    uint8 values agree except where the FP64 value sits within 1e-4 of a truncation boundary."""
+    """Reinhard-named exponential map and the sRGB OETF (ToneMapper.cu:30-36,45-48), and the ACES
+    mode (Narkowicz's fit x(2.51x + 0.03) / (x(2.43x + 0.59) + 0.14), then gamma 1/2.2), in FP64
+    numpy; uint8 values agree except where the FP64 value sits within 1e-3 of a truncation
+    boundary."""
     rng = np.random.default_rng(9)
     hdr = (rng.exponential(1.0, (64, 64, 3)) * rng.choice([0.01, 1, 5], (64, 64, 1))).astype(np.float32)
-    for kind in (0, 1):
+    for kind in (0, 1, 2):
         got = oracle.tonemap(hdr, kind, 1.3)
         x = hdr.astype(np.float64)
         if kind == 0:
             m = (1 - np.exp(-x * 1.3)) ** (1 / 2.2)
+        elif kind == 2:
+            y = x * 1.3
+            m = np.clip(y * (2.51 * y + 0.03) / (y * (2.43 * y + 0.59) + 0.14), 0, 1) ** (1 / 2.2)
         else:
             m = np.where(x <= 0.0031308, 12.92 * x, 1.055 * x ** (1 / 2.4) - 0.055)
         v = np.clip(m, 0, 1) * 255
@@ -524,6 +529,15 @@ def test_tonemap_against_fp64_formulas():
         assert np.array_equal(got[..., :3][far], np.floor(v[far]).astype(np.uint8))
         assert (got[..., 3] == 255).all()
 
+
+
+def test_aces_known_values():
+    """The ACES fit at 0 is 0; at x = 1 it is 2.54 / 3.16 (0.80380 -> 0.80380^(1/2.2) x 255 = 230.9,
+    truncated to 230); it reaches 1 at the positive root of (2.51 - 2.43) x^2 + (0.03 - 0.59) x - 0.14,
+    x = 7.2417 (at 7.0: 0.99886 -> 254.87 -> 254), and saturates at 255 beyond."""
+    hdr = np.array([[[0.0, 1.0, 7.0], [7.3, 100.0, 1e6]]], np.float32)
+    got = oracle.tonemap(hdr, 2, 1.0)[..., :3]
+    assert got[0, 0].tolist() == [0, 230, 254] and got[0, 1].tolist() == [255, 255, 255]
 
 def test_gdpt_reconstruction_fixed_points():
     """optimization_kernel (GradientDomain.cu:303-350) keeps an affine ramp with consistent
