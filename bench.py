@@ -576,10 +576,16 @@ def bench_grid(args, log):
     adj_launch_ms = float(np.mean(adj_ms))
     samples = W * H * SPP * args.steps
     value = samples / t_total / 1e6
-    achieved = adj_bytes / (adj_launch_ms * 1e-3) / 1e9
+    alg_gbs = adj_bytes / (adj_launch_ms * 1e-3) / 1e9
+    atomic_rate = cnt["atomics"] / (adj_launch_ms * 1e-3)  # the adjoint's gradient atomics per second
     log(f"grid: {value:.1f} Msamples/s (forward + adjoint), {t_total / args.steps * 1e3:.1f} ms/step, adjoint "
         f"{adj_launch_ms:.2f} ms; per step {cnt['closest_rays'] / 1e6:.1f}M closest + {cnt['shadow_rays'] / 1e6:.1f}M "
         f"shadow rays, {cnt['grid_reads'] / 1e6:.1f}M grid reads, {cnt['atomics'] / 1e6:.1f}M atomics")
+    # the ceiling: FP32 atomics without return to scattered words of 8 per-XCD copies of the grid
+    micro = cb.measure_atomic_peak(256, RES ** 3)
+    frac = atomic_rate / micro["atomics_per_s"]
+    log(f"atomics: adjoint {atomic_rate / 1e9:.2f} G/s; scattered-atomic microkernel {micro['atomics_per_s'] / 1e9:.2f} "
+        f"G/s ({micro['atomics'] / 1e6:.0f}M in {micro['ms']:.2f} ms) -> {frac:.3f} of it")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -620,10 +626,14 @@ def bench_grid(args, log):
         "config": {"workload": f"heterogeneous medium {RES}^3 in a 10-triangle room, {W}x{H}, {SPP} spp, depth "
                                f"{DEPTH}: forward render + path-replay adjoint of an L2 image loss",
                    "width": W, "height": H, "spp": SPP, "grid": RES, "max_depth": DEPTH},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_grid<true> (path-replay adjoint)",
-                     "bytes_per_launch": int(adj_bytes), "avg_launch_ms": round(adj_launch_ms, 4)},
+        "roofline": {"bound": "l2_atomics", "achieved": round(atomic_rate / 1e9, 3),
+                     "peak": round(micro["atomics_per_s"] / 1e9, 3), "unit": "G atomics/s",
+                     "frac": round(frac, 4), "traffic": None,
+                     "peak_source": "scattered FP32 atomic adds without return into 8 per-XCD copies of the "
+                                    f"{RES}^3 grid (cupbr_measure_atomic_peak), measured in this run",
+                     "kernel": "k_grid<true> (path-replay adjoint)", "atomics_per_launch": int(cnt["atomics"]),
+                     "algorithmic_bytes_per_launch": int(adj_bytes), "algorithmic_gbs": round(alg_gbs, 1),
+                     "avg_launch_ms": round(adj_launch_ms, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -719,28 +729,31 @@ def bench_nrc(args, log):
             N.render(cam, S_NRC)
         N.synchronize()
         t_warm = time.perf_counter() - t0
-        times, errs, imgs = [], [], []
+        times, errs = [], []
         for _ in range(args.steps):
-            times.append(frame_ms(N, S_NRC))
-            imgs.append(N.get_output_image().astype(np.float64))
-            errs.append(rel_mse(imgs[-1], ref))
+            times.append(frame_ms(N, S_NRC))  # frames 0 .. S_NRC - 1 each time
+            errs.append(rel_mse(N.get_output_image(), ref))
         st = N.stats()
         t_c, e_c = float(np.median(times)), float(np.mean(errs))
-        # the error of the calls' mean image: their variance divided by the number of calls, plus
-        # the cache's bias (each call has fresh seeds; the cache moves a little between calls)
-        e_mean = rel_mse(np.mean(imgs, axis=0), ref)
+        # the running mean of `steps` successive calls (frames 0 .. steps * S_NRC - 1): its error is
+        # the variance of one call divided by the number of calls plus the cache's bias, which more
+        # samples do not remove
+        N.reset()
+        for _ in range(args.steps):
+            N.render(cam, S_NRC)
+        e_mean = rel_mse(N.get_output_image(), ref)
         hi = next((i for i, t in enumerate(pt_ms) if t >= t_c), len(pt_ms) - 1)
         lo = max(0, hi - 1) if pt_ms[hi] > t_c else hi
         res = {"camera_min_roughness": camera_min_roughness, "cache_call_ms": t_c, "rel_mse_cache": e_c,
-               "rel_mse_mean_of_calls": e_mean,
+               "rel_mse_running_mean_of_calls": e_mean,
                "queries": int(st["cache_queries"]), "training_rows": int(st["cache_training_rows"]),
                "hi": hi, "lo": lo, "ratio": pt_err[hi] / e_c, "ratio_lo": pt_err[lo] / e_c, "cache": cache}
         log(f"cache (camera hit: {'never' if camera_min_roughness == float('inf') else f'roughness >= {camera_min_roughness}'}): "
             f"{warm} training frames in calls of {S_NRC} in {t_warm:.1f}s; {S_NRC} spp in {t_c:.2f} ms, rel-MSE "
             f"{e_c:.4f} ({res['queries']} queries, {res['training_rows']} training rows); plain PT at {pt_spp[hi]} spp "
             f"({pt_ms[hi]:.2f} ms, at least equal time): {pt_err[hi]:.4f}, ratio {res['ratio']:.2f}; at {pt_spp[lo]} "
-            f"spp ({pt_ms[lo]:.2f} ms): {pt_err[lo]:.4f}, ratio {res['ratio_lo']:.2f}; mean of the {args.steps} "
-            f"calls' images: rel-MSE {e_mean:.4f}")
+            f"spp ({pt_ms[lo]:.2f} ms): {pt_err[lo]:.4f}, ratio {res['ratio_lo']:.2f}; running mean of {args.steps} "
+            f"successive calls ({args.steps * S_NRC} spp): rel-MSE {e_mean:.4f}")
         return res
 
     modes = [float(x) for x in args.nrc_camera_roughness.split(",")]
@@ -816,7 +829,7 @@ def bench_nrc(args, log):
                    "modes": [{"camera_min_roughness": None if r_["camera_min_roughness"] == float("inf") else
                               r_["camera_min_roughness"], "ratio": round(r_["ratio"], 3),
                               "rel_mse_cache": round(r_["rel_mse_cache"], 5),
-                              "rel_mse_mean_of_calls": round(r_["rel_mse_mean_of_calls"], 5),
+                              "rel_mse_running_mean_of_calls": round(r_["rel_mse_running_mean_of_calls"], 5),
                               "cache_call_ms": round(r_["cache_call_ms"], 3)} for r_ in runs]},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 5), "traffic": None, "kernel": "k_nrc_infer<false>",

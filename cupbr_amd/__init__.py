@@ -56,6 +56,11 @@ class ToneMappingType(enum.IntEnum):  # ToneMapper.h:13-17
     ACES = 2  # not in the reference: Narkowicz's ACES filmic fit, then gamma 1/2.2
 
 
+class BloomPyramid(enum.IntEnum):  # cupbr_bloom_pyramid; GAUSSIAN5 is not in the reference
+    REFERENCE = 0
+    GAUSSIAN5 = 1
+
+
 class GeometryType(enum.IntEnum):  # Geometry.h:15-23
     SPHERE = 0
     PLANE = 1
@@ -169,6 +174,7 @@ def lib():
             "cupbr_last_error": (C.c_char_p, []),
             "cupbr_device_name": (C.c_int, [C.c_char_p, SZ]),
             "cupbr_measure_valu_peak": (C.c_int, [U32, P, P]),
+            "cupbr_measure_atomic_peak": (C.c_int, [U32, U32, P, P]),
             "cupbr_camera_init": (C.c_int, [P, U32, U32]),
             "cupbr_camera_look_at": (C.c_int, [P, P, P, P, C.c_float, U32, U32]),
             "cupbr_camera_controller_init": (C.c_int, [P, U32, U32]),
@@ -214,6 +220,7 @@ def lib():
             "cupbr_post_destroy": (C.c_int, [P]),
             "cupbr_post_register_image": (C.c_int, [P, P, U32, U32]),
             "cupbr_post_bloom": (C.c_int, [P, P]),
+            "cupbr_post_set_bloom_pyramid": (C.c_int, [P, C.c_int32]),
             "cupbr_post_get_buffer_device": (C.c_int, [P, P]),
             "cupbr_post_get_buffer": (C.c_int, [P, P, SZ]),
             "cupbr_threshold_curve": (C.c_int, [C.c_float, C.c_float, P]),
@@ -281,6 +288,16 @@ def measure_valu_peak(iterations: int = 20000) -> dict:
     _check(lib().cupbr_measure_valu_peak(iterations, C.byref(ms), C.byref(n)))
     return {"ms": ms.value, "wave_instructions": n.value,
             "wave_instructions_per_s": n.value / (ms.value * 1e-3) if ms.value > 0 else float("nan")}
+
+
+def measure_atomic_peak(iterations: int = 256, words_per_copy: int = 64 ** 3) -> dict:
+    """cupbr_measure_atomic_peak: time and count of FP32 atomic adds (no return) to pseudo-random
+    words of 8 per-XCD copies of words_per_copy floats (the grid adjoint's gradient layout)."""
+    ms = C.c_double()
+    n = C.c_uint64()
+    _check(lib().cupbr_measure_atomic_peak(iterations, words_per_copy, C.byref(ms), C.byref(n)))
+    return {"ms": ms.value, "atomics": n.value,
+            "atomics_per_s": n.value / (ms.value * 1e-3) if ms.value > 0 else float("nan")}
 
 
 def threshold_curve(threshold: float = 1.0, knee: float = 1.0) -> np.ndarray:
@@ -904,6 +921,10 @@ class PostProcessor:
     def bloom(self, threshold):
         c = (C.c_float * 4)(*[float(x) for x in threshold])
         _check(lib().cupbr_post_bloom(self._h, c))
+
+    def set_bloom_pyramid(self, pyramid: "BloomPyramid"):
+        """BloomPyramid.REFERENCE (the default) or GAUSSIAN5; see cupbr_post_set_bloom_pyramid."""
+        _check(lib().cupbr_post_set_bloom_pyramid(self._h, int(pyramid)))
 
     def get_post_process_buffer(self) -> np.ndarray:
         w, h = self._size

@@ -669,6 +669,14 @@ int cupbr_post_bloom(cupbr_post* p, const float curve[4]) {
     return post_bloom(p, curve, nullptr, 0, 1.0f);
 }
 
+int cupbr_post_set_bloom_pyramid(cupbr_post* p, int32_t pyramid) {
+    CHECK_ARG(p, "null post processor");
+    CHECK_ARG(pyramid == CUPBR_BLOOM_REFERENCE || pyramid == CUPBR_BLOOM_GAUSSIAN5, "unknown bloom pyramid");
+    CUPBR_HIP_CHECK(hipStreamSynchronize(p->stream));
+    p->pyramid = pyramid;
+    return CUPBR_OK;
+}
+
 int cupbr_post_get_buffer_device(const cupbr_post* p, float** dev) {
     CHECK_ARG(p && dev, "null argument");
     if (!p->registered) return set_error(CUPBR_ERR_NOT_REGISTERED, "[PostProcessor]  No HDR image has been registered.", true);

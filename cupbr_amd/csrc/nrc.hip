@@ -80,9 +80,20 @@ __device__ __forceinline__ void derive(const NrcBox& box, const float* raw, floa
     for (int k = 0; k < 6; ++k) d[8 + k] = raw[10 + k];
 }
 
+// sin(pi y) and cos(pi y) from the hardware's sine and cosine, whose argument is in revolutions
+// (v_sin_f32(r) = sin(2 pi r)): r = y / 2 is exact, and its fraction keeps the argument in [0, 1),
+// well inside the instructions' input range. Their error (~1e-6 absolute) is far below the BF16
+// rounding the feature then takes.
+__device__ __forceinline__ void sincospi_hw(float y, float* s, float* c) {
+    float r = 0.5f * y;
+    r -= floorf(r);
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
+}
+
 // B operand of the first layer for lane group g: slot (kc, j) holds feature perm(kc, 8g + j)
 // Slots j and j + 1 (j even) hold features i and i + 1 with i even: below 52 these are the sine
-// and cosine of the same argument 2^k pi x, computed together by sincospif on the exact 2^k x.
+// and cosine of the same argument 2^k pi x, computed together from the exact 2^k x.
 __device__ __forceinline__ void encode(const NrcBox& box, const float* raw, int g, bf16x8 b[2]) {
     float d[14];
     derive(box, raw, d);
@@ -100,7 +111,7 @@ __device__ __forceinline__ void encode(const NrcBox& box, const float* raw, int 
                     a = (i < 44 ? 3 : 5) + (r >> 2);
                     k = (r & 3) >> 1;
                 }
-                sincospif((float)(1 << k) * d[a], &f0, &f1);
+                sincospi_hw((float)(1 << k) * d[a], &f0, &f1);
             } else {
                 f0 = feature(i, d);
                 f1 = feature(i + 1, d);
@@ -122,6 +133,26 @@ __device__ __forceinline__ void hidden_layer(const bf16x8* w, int lane, const bf
         for (int j = 0; j < 8; ++j) out[kc][j] = to_bf16(fmaxf(acc[2 * kc + (j >> 2)][j & 3], 0.0f));
     if (pre)
         for (int ot = 0; ot < 4; ++ot) pre[ot] = acc[ot];
+}
+
+// hidden_layer for two tiles of 16 queries: each weight fragment read from LDS feeds both
+__device__ __forceinline__ void hidden_layer2(const bf16x8* w, int lane, const bf16x8 in0[2], const bf16x8 in1[2],
+                                              bf16x8 out0[2], bf16x8 out1[2]) {
+    f32x4 acc0[4], acc1[4];
+    for (int ot = 0; ot < 4; ++ot) {
+        acc0[ot] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        acc1[ot] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int kc = 0; kc < 2; ++kc) {
+            const bf16x8 wf = w[(ot * 2 + kc) * 64 + lane];
+            acc0[ot] = mfma(wf, in0[kc], acc0[ot]);
+            acc1[ot] = mfma(wf, in1[kc], acc1[ot]);
+        }
+    }
+    for (int kc = 0; kc < 2; ++kc)
+        for (int j = 0; j < 8; ++j) {
+            out0[kc][j] = to_bf16(fmaxf(acc0[2 * kc + (j >> 2)][j & 3], 0.0f));
+            out1[kc][j] = to_bf16(fmaxf(acc1[2 * kc + (j >> 2)][j & 3], 0.0f));
+        }
 }
 
 // output layer: lane group 0 gets the 3 outputs (v = 0..2) of query lane%16
@@ -151,43 +182,48 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_infer(const bf16x8* __restric
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t tiles = (n - begin + 15) / 16;
-    for (uint32_t t = wave; t < tiles; t += n_waves) {
-        const uint32_t q = begin + t * 16 + (lane & 15);
-        float raw[16];
-        if (q < n) {
-            const float4* r4 = reinterpret_cast<const float4*>(rec + (size_t)q * stride);
-            for (int k = 0; k < 4; ++k) {
-                const float4 v = r4[k];
-                raw[4 * k] = v.x;
-                raw[4 * k + 1] = v.y;
-                raw[4 * k + 2] = v.z;
-                raw[4 * k + 3] = v.w;
-            }
-        } else {
-            for (int k = 0; k < 16; ++k) raw[k] = 0.0f;
-            raw[5] = 1.0f;
-            raw[8] = 1.0f;
-        }
-        bf16x8 a[2], b[2];
-        encode(box, raw, g, a);
-        hidden_layer(w + 0 * (kFwdLayer / 8), lane, a, b);
-        hidden_layer(w + 1 * (kFwdLayer / 8), lane, b, a);
-        hidden_layer(w + 2 * (kFwdLayer / 8), lane, a, b);
-        hidden_layer(w + 3 * (kFwdLayer / 8), lane, b, a);
-        const f32x4 o = output_layer(w + 4 * (kFwdLayer / 8), lane, a);
-        if (g == 0 && q < n) {
-            if (kAccumulate) {
-                const float* r = rec + (size_t)q * stride;
-                const uint32_t path = __float_as_uint(r[19]);
-                float* px = out + 4 * (size_t)path;
-                px[0] += r[16] * fmaxf(o[0], 0.0f);
-                px[1] += r[17] * fmaxf(o[1], 0.0f);
-                px[2] += r[18] * fmaxf(o[2], 0.0f);
+    const uint32_t pairs = (n - begin + 31) / 32;  // two tiles of 16 queries per wave and step
+    for (uint32_t t = wave; t < pairs; t += n_waves) {
+        uint32_t q[2];
+        bf16x8 a[2][2], b[2][2];
+        for (int h = 0; h < 2; ++h) {
+            q[h] = begin + t * 32 + 16 * h + (lane & 15);
+            float raw[16];
+            if (q[h] < n) {
+                const float4* r4 = reinterpret_cast<const float4*>(rec + (size_t)q[h] * stride);
+                for (int k = 0; k < 4; ++k) {
+                    const float4 v = r4[k];
+                    raw[4 * k] = v.x;
+                    raw[4 * k + 1] = v.y;
+                    raw[4 * k + 2] = v.z;
+                    raw[4 * k + 3] = v.w;
+                }
             } else {
-                out[3 * (size_t)q + 0] = o[0];
-                out[3 * (size_t)q + 1] = o[1];
-                out[3 * (size_t)q + 2] = o[2];
+                for (int k = 0; k < 16; ++k) raw[k] = 0.0f;
+                raw[5] = 1.0f;
+                raw[8] = 1.0f;
+            }
+            encode(box, raw, g, a[h]);
+        }
+        hidden_layer2(w + 0 * (kFwdLayer / 8), lane, a[0], a[1], b[0], b[1]);
+        hidden_layer2(w + 1 * (kFwdLayer / 8), lane, b[0], b[1], a[0], a[1]);
+        hidden_layer2(w + 2 * (kFwdLayer / 8), lane, a[0], a[1], b[0], b[1]);
+        hidden_layer2(w + 3 * (kFwdLayer / 8), lane, b[0], b[1], a[0], a[1]);
+        for (int h = 0; h < 2; ++h) {
+            const f32x4 o = output_layer(w + 4 * (kFwdLayer / 8), lane, a[h]);
+            if (g == 0 && q[h] < n) {
+                if (kAccumulate) {
+                    const float* r = rec + (size_t)q[h] * stride;
+                    const uint32_t path = __float_as_uint(r[19]);
+                    float* px = out + 4 * (size_t)path;
+                    px[0] += r[16] * fmaxf(o[0], 0.0f);
+                    px[1] += r[17] * fmaxf(o[1], 0.0f);
+                    px[2] += r[18] * fmaxf(o[2], 0.0f);
+                } else {
+                    out[3 * (size_t)q[h] + 0] = o[0];
+                    out[3 * (size_t)q[h] + 1] = o[1];
+                    out[3 * (size_t)q[h] + 2] = o[2];
+                }
             }
         }
     }
@@ -458,8 +494,8 @@ int nrc_create(const float lo[3], const float hi[3], uint32_t seed, cupbr_nrc** 
 int nrc_infer(cupbr_nrc* c, const float* rec, uint32_t n, const uint32_t* n_dev, uint32_t stride, float* out,
               bool accumulate, hipStream_t stream, const uint32_t* begin_dev) {
     if (n == 0) return CUPBR_OK;
-    const uint32_t tiles = (n + 15) / 16;
-    const int blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>((tiles + 3) / 4, (uint32_t)device_cu_count() * 4));
+    const uint32_t pairs = (n + 31) / 32;  // a wave takes two 16-query tiles at a time
+    const int blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>((pairs + 3) / 4, (uint32_t)device_cu_count() * 4));
     if (accumulate)
         hipLaunchKernelGGL(k_nrc_infer<true>, dim3(blocks), dim3(kNrcBlock), 0, stream, c->packed_fwd.as<bf16x8>(),
                            c->box, rec, n, n_dev, stride, out, begin_dev);

@@ -557,34 +557,34 @@ __global__ __launch_bounds__(kPtBlock) void k_gdpt_raygen(cupbr_camera cam, uint
                                                           float4* __restrict__ q_d, float4* __restrict__ weight,
                                                           float4* __restrict__ rad, uint32_t* __restrict__ slot0,
                                                           Counters* c) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g == 0) {
+    // one thread per path, so that the queue and state writes of a wave are contiguous
+    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid == 0) {
         c->queue_count[0].v = 5u * n_groups;
         for (int k = 0; k < 8; ++k) c->work[k].v = c->swork[k].v = 0;
         c->shadow_count[0].v = 0;
         c->shadow_count[1].v = 0;
     }
-    if (g >= n_groups) return;
+    if (pid >= 5u * n_groups) return;
+    const uint32_t g = pid / 5u, k = pid - 5u * g;
     const uint32_t s = g / n_int, gi = g - s * n_int;
     uint32_t ix, iy;
     tile_pixel(gi, w - 2u, h - 2u, ix, iy);
     const uint32_t px = ix + 1u, py = iy + 1u;
-    // the five camera rays from one chain, base, left, right, up, down; each path continues from
-    // the chain as it stood after its own ray (:180-213)
+    // the five camera rays of a group come from one chain, base, left, right, up, down, two draws
+    // each; each path continues from the chain as it stood after its own ray (:180-213). Path k
+    // skips the 2k draws of the rays before its own.
     uint32_t chain = tea4(py * w + px, frame0 + s);
-#pragma unroll 1
-    for (uint32_t k = 0; k < 5; ++k) {
-        const uint32_t nx = k == 1 ? px - 1u : (k == 2 ? px + 1u : px);
-        const uint32_t ny = k == 3 ? py + 1u : (k == 4 ? py - 1u : py);
-        v3 o, d;
-        launch_ray(nx, ny, w, h, cam, chain, o, d);
-        const uint32_t pid = 5u * g + k;
-        q_o[pid] = make_float4(o.x, o.y, o.z, 0.0f);
-        q_d[pid] = make_float4(d.x, d.y, d.z, __uint_as_float(pid));
-        weight[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(chain));
-        rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    slot0[g] = 5u * g;
+    for (uint32_t j = 0; j < 2u * k; ++j) chain = 1664525u * chain + 1013904223u;  // rnd's state step
+    const uint32_t nx = k == 1 ? px - 1u : (k == 2 ? px + 1u : px);
+    const uint32_t ny = k == 3 ? py + 1u : (k == 4 ? py - 1u : py);
+    v3 o, d;
+    launch_ray(nx, ny, w, h, cam, chain, o, d);
+    q_o[pid] = make_float4(o.x, o.y, o.z, 0.0f);
+    q_d[pid] = make_float4(d.x, d.y, d.z, __uint_as_float(pid));
+    weight[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(chain));
+    rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (k == 0) slot0[g] = pid;
 }
 
 // one bounce of collect_radiance (:121-156) for every queued path
@@ -1203,7 +1203,7 @@ int gdpt_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t fr
     for (uint32_t s0 = 0; s0 < spp; s0 += batch) {
         const uint32_t nb = std::min(batch, spp - s0);
         const uint32_t groups = nb * n_int, npath = 5u * groups;
-        hipLaunchKernelGGL(k_gdpt_raygen, dim3((groups + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
+        hipLaunchKernelGGL(k_gdpt_raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
                            h, frame0 + s0, n_int, groups, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
                            wf.weight.as<float4>(), wf.radiance.as<float4>(), slots[0], c);
         if ((rc = stage_mark(r, 3))) return rc;

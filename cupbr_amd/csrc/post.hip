@@ -126,9 +126,111 @@ __global__ __launch_bounds__(256) void k_tonemap(const float* __restrict__ hdr, 
     rgba[i] = tonemap_px(ld3(hdr + 3 * i), type, exposure);
 }
 
+// ------------------------------------------------------------------ Gaussian pyramid bloom
+// (CUPBR_BLOOM_GAUSSIAN5, an addition: see cupbr_post_set_bloom_pyramid for the definition). Each
+// level is one launch over its pixels; every level fits L2 from the second on, so these are
+// latency- rather than bandwidth-bound at 1080p.
+__global__ __launch_bounds__(256) void k_gauss_threshold(const float* __restrict__ hdr, float* __restrict__ l0, size_t n,
+                                                         float4 curve) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) st3(l0 + 3 * i, threshold_px(ld3(hdr + 3 * i), curve));
+}
+
+// dst(x, y) = blur(src)(2x, 2y): rows of five taps [1 4 6 4 1] summed first, then the five rows,
+// then / 256 (edge-clamped)
+__global__ __launch_bounds__(256) void k_gauss_down(const float* __restrict__ src, uint32_t sw, uint32_t sh,
+                                                    float* __restrict__ dst, uint32_t dw, uint32_t dh) {
+    const uint32_t x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= dw || y >= dh) return;
+    const float wt[5] = {1.0f, 4.0f, 6.0f, 4.0f, 1.0f};
+    v3 acc = splat(0.0f);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const int sy = min(max((int)(2 * y) + j - 2, 0), (int)sh - 1);
+        v3 row = splat(0.0f);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const int sx = min(max((int)(2 * x) + i - 2, 0), (int)sw - 1);
+            row += ld_px(src, sw, sx, sy) * wt[i];
+        }
+        acc += row * wt[j];
+    }
+    st3(dst + 3 * ((size_t)y * dw + x), acc / 256.0f);
+}
+
+// coarse level (cw x ch) sampled at fine pixel (x, y): the texel (x/2, y/2), the mean of two
+// texels along an odd coordinate (edge-clamped)
+__device__ __forceinline__ v3 gauss_up(const float* coarse, uint32_t cw, uint32_t ch, uint32_t x, uint32_t y) {
+    const uint32_t x0 = x >> 1, y0 = y >> 1;
+    const uint32_t x1 = (x & 1u) ? min(x0 + 1u, cw - 1u) : x0, y1 = (y & 1u) ? min(y0 + 1u, ch - 1u) : y0;
+    const v3 a = ld_px(coarse, cw, (int)x0, (int)y0), b = ld_px(coarse, cw, (int)x1, (int)y0);
+    const v3 c = ld_px(coarse, cw, (int)x0, (int)y1), d = ld_px(coarse, cw, (int)x1, (int)y1);
+    return ((a + b) * 0.5f + (c + d) * 0.5f) * 0.5f;
+}
+
+// fine += up(coarse)
+__global__ __launch_bounds__(256) void k_gauss_up_add(const float* __restrict__ coarse, uint32_t cw, uint32_t ch,
+                                                      float* __restrict__ fine, uint32_t fw, uint32_t fh) {
+    const uint32_t x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= fw || y >= fh) return;
+    float* f = fine + 3 * ((size_t)y * fw + x);
+    st3(f, ld3(f) + gauss_up(coarse, cw, ch, x, y));
+}
+
+// out = hdr + (L0 + up(U1)) / 6, with the tone map fused when rgba is given
+__global__ __launch_bounds__(256) void k_gauss_combine(const float* __restrict__ hdr, const float* __restrict__ l0,
+                                                       const float* __restrict__ u1, uint32_t w, uint32_t h,
+                                                       uint32_t w1, uint32_t h1, float* __restrict__ out,
+                                                       uchar4* __restrict__ rgba, int32_t type, float exposure) {
+    const uint32_t x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= w || y >= h) return;
+    const size_t i = (size_t)y * w + x;
+    const v3 u0 = ld3(l0 + 3 * i) + gauss_up(u1, w1, h1, x, y);
+    const v3 o = ld3(hdr + 3 * i) + u0 / 6.0f;
+    st3(out + 3 * i, o);
+    if (rgba) rgba[i] = tonemap_px(o, type, exposure);
+}
+
+static int gauss_bloom(cupbr_post* p, float4 curve, uint8_t* rgba, int32_t type, float exposure) {
+    uint32_t lw[kGaussLevels + 1], lh[kGaussLevels + 1];
+    size_t off[kGaussLevels + 2];
+    lw[0] = p->width;
+    lh[0] = p->height;
+    off[0] = 0;
+    for (int k = 0; k <= kGaussLevels; ++k) {
+        if (k > 0) {
+            lw[k] = (lw[k - 1] + 1) / 2;
+            lh[k] = (lh[k - 1] + 1) / 2;
+        }
+        off[k + 1] = off[k] + (size_t)lw[k] * lh[k] * 3;
+    }
+    if (int rc = p->levels.ensure(off[kGaussLevels + 1] * sizeof(float))) return rc;
+    float* L = p->levels.as<float>();
+    auto grid = [](uint32_t w, uint32_t h) { return dim3((w + 63) / 64, (h + 3) / 4); };
+    const size_t n = (size_t)p->width * p->height;
+    hipLaunchKernelGGL(k_gauss_threshold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, p->hdr, L, n, curve);
+    for (int k = 1; k <= kGaussLevels; ++k)
+        hipLaunchKernelGGL(k_gauss_down, grid(lw[k], lh[k]), dim3(256), 0, p->stream, L + off[k - 1], lw[k - 1],
+                           lh[k - 1], L + off[k], lw[k], lh[k]);
+    for (int k = kGaussLevels - 1; k >= 1; --k)  // U_k = L_k + up(U_{k+1}), in place
+        hipLaunchKernelGGL(k_gauss_up_add, grid(lw[k], lh[k]), dim3(256), 0, p->stream, L + off[k + 1], lw[k + 1],
+                           lh[k + 1], L + off[k], lw[k], lh[k]);
+    hipLaunchKernelGGL(k_gauss_combine, grid(lw[0], lh[0]), dim3(256), 0, p->stream, p->hdr, L, L + off[1], lw[0],
+                       lh[0], lw[1], lh[1], p->output.as<float>(), reinterpret_cast<uchar4*>(rgba), type, exposure);
+    CUPBR_HIP_CHECK(hipGetLastError());
+    return CUPBR_OK;
+}
+
 int post_bloom(cupbr_post* p, const float curve[4], uint8_t* rgba, int32_t type, float exposure) {
     if (p->width < 2 || p->height < 2)
         return set_error(CUPBR_ERR_INVALID_ARGUMENT, "bloom needs an image of at least 2x2 pixels");
+    if (p->pyramid == CUPBR_BLOOM_GAUSSIAN5) {
+        CUPBR_HIP_CHECK(hipDeviceSynchronize());  // the registered image is borrowed, as below
+        if (int rc = gauss_bloom(p, make_float4(curve[0], curve[1], curve[2], curve[3]), rgba, type, exposure))
+            return rc;
+        CUPBR_HIP_CHECK(hipStreamSynchronize(p->stream));
+        return CUPBR_OK;
+    }
     const uint32_t tiles = ((p->width + kTileW - 1) / kTileW) * ((p->height + kTileH - 1) / kTileH);
     dim3 grid(tiles);
     float4 c = make_float4(curve[0], curve[1], curve[2], curve[3]);

@@ -189,6 +189,11 @@ int cupbr_device_name(char* out, size_t n);
  * returns its time and its wave64 FMA instruction count, i.e. the VALU issue ceiling against which
  * the trace kernels' roofline is stated. */
 int cupbr_measure_valu_peak(uint32_t iterations, double* milliseconds, uint64_t* fma_instructions);
+/* Measurement aid (no counterpart in the reference): times FP32 atomic adds without return, one per
+ * lane and iteration, to pseudo-random words of 8 copies of words_per_copy floats (block b into copy
+ * b % 8, the grid adjoint's per-XCD layout); `atomics` = lane atomics of the timed launch. The
+ * grid adjoint's roofline is stated against this rate. */
+int cupbr_measure_atomic_peak(uint32_t iterations, uint32_t words_per_copy, double* milliseconds, uint64_t* atomics);
 
 /* ------------------------------------------------------------------ camera */
 /* Camera(width, height), lib/src/DataStructure/Camera.cpp:6-11: origin, looking +z, 90 deg vertical FOV */
@@ -336,6 +341,16 @@ int cupbr_post_destroy(cupbr_post* p);
 int cupbr_post_register_image(cupbr_post* p, const float* device_rgb, uint32_t width, uint32_t height);
 /* bloom(Vector4float threshold), PostProcessor.cpp:125-142; curve = Interactor.cpp:436-440 */
 int cupbr_post_bloom(cupbr_post* p, const float threshold_curve[4]);
+/* Bloom pyramid of cupbr_post_bloom and cupbr_tonemapper_tonemap_bloom. CUPBR_BLOOM_REFERENCE (the
+ * default) is the reference's (PostProcessor.cpp:125-142, PyramidConstructor.cu:9-169).
+ * CUPBR_BLOOM_GAUSSIAN5 has no counterpart in the reference: L0 = the thresholded image, five
+ * levels L_k(x, y) = B_{k-1}(2x, 2y) with B = the separable binomial blur [1 4 6 4 1] / 16
+ * (edge-clamped) of L_{k-1}, level sizes (n + 1) / 2 per axis; then U_5 = L_5, U_k = L_k +
+ * up(U_{k+1}) where up() samples U_{k+1} at (x / 2, y / 2) bilinearly (the average of the two or
+ * four neighbours at odd coordinates, edge-clamped); output = hdr + U_0 / 6 (a constant image c
+ * gives c + threshold(c)). */
+enum cupbr_bloom_pyramid { CUPBR_BLOOM_REFERENCE = 0, CUPBR_BLOOM_GAUSSIAN5 = 1 };
+int cupbr_post_set_bloom_pyramid(cupbr_post* p, int32_t pyramid);
 int cupbr_post_get_buffer_device(const cupbr_post* p, float** device_rgb);   /* getPostProcessBuffer */
 int cupbr_post_get_buffer(const cupbr_post* p, float* host_rgb, size_t n_floats);
 /* threshold curve from (threshold, knee), Interactor::Impl::compute_threshold */

@@ -185,6 +185,8 @@ public:
         const float curve[4] = {threshold.x, threshold.y, threshold.z, threshold.w};
         return cupbr_post_bloom(p_, curve);
     }
+    // an addition: CUPBR_BLOOM_GAUSSIAN5 instead of the reference's pyramid
+    int setBloomPyramid(int32_t pyramid) { return cupbr_post_set_bloom_pyramid(p_, pyramid); }
     Image getPostProcessBuffer() const {
         Image im;
         cupbr_post_get_buffer_device(p_, &im.data);

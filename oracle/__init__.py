@@ -61,6 +61,7 @@ def lib():
                                C.c_uint32, C.c_int, P, P, P, P]
         L.orc_gdpt_reconstruct_l2.argtypes = [C.c_uint32, C.c_uint32, P, C.c_float, C.c_int, P]
         L.orc_bloom.argtypes = [P, C.c_uint32, C.c_uint32, P, P]
+        L.orc_bloom_gaussian.argtypes = [P, C.c_uint32, C.c_uint32, P, P]
         L.orc_tonemap.argtypes = [P, C.c_uint64, C.c_int, C.c_float, P]
         L.orc_brdf.argtypes = [P, P, P, P, P]
         L.orc_sample_direction.argtypes = [P, C.c_uint32, P, P, C.c_uint32, P]
@@ -239,6 +240,16 @@ def bloom(hdr, threshold_curve):
     thr = _c(threshold_curve, np.float32)
     out = np.zeros_like(hdr)
     lib().orc_bloom(_p(hdr), w, h, _p(thr), _p(out))
+    return out
+
+
+def bloom_gaussian(hdr, threshold_curve):
+    """The Gaussian-pyramid bloom mode (an addition; orc_bloom_gaussian)."""
+    hdr = _c(hdr, np.float32)
+    h, w, _ = hdr.shape
+    thr = _c(threshold_curve, np.float32)
+    out = np.zeros_like(hdr)
+    lib().orc_bloom_gaussian(_p(hdr), w, h, _p(thr), _p(out))
     return out
 
 

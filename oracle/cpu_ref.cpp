@@ -1509,7 +1509,7 @@ void orc_bloom(const float* hdr, uint32_t w, uint32_t h, const float* thr, float
     up[depth - 1] = down[depth - 1];
     const float k3[3] = {1.0f, 2.0f, 1.0f};
     for (int l = depth - 2; l >= 0; --l)  // upscale_and_combine_kernel :67-112 (reads down[l+1])
-        for (uint32_t y = 0; y < lh[l]; ++y)
+        for (uint32_t y = 0; y < lh[l] && lw[l + 1] > 0 && lh[l + 1] > 0; ++y)  // small images: empty levels
             for (uint32_t x = 0; x < lw[l]; ++x) {
                 V3<float> r(0);
                 int dx = (int)x / 2, dy = (int)y / 2;
@@ -1528,6 +1528,59 @@ void orc_bloom(const float* hdr, uint32_t w, uint32_t h, const float* thr, float
         V3<float> o = V3<float>(hdr[3 * i], hdr[3 * i + 1], hdr[3 * i + 2]) + up[0][i];
         out[3 * i] = o.x; out[3 * i + 1] = o.y; out[3 * i + 2] = o.z;
     }
+}
+
+// Gaussian-pyramid bloom: an addition with no counterpart in the reference (the C-ABI's
+// CUPBR_BLOOM_GAUSSIAN5). L0 = threshold(hdr); L_k(x, y) = blur(L_{k-1})(2x, 2y), blur = the
+// separable binomial [1 4 6 4 1] / 16 with clamped edges (each row's five taps summed, then the five
+// rows, then / 256); U_5 = L_5, U_k = L_k + up(U_{k+1}) with up = the coarse texel at (x/2, y/2),
+// averaged with its clamped neighbour along an odd coordinate; out = hdr + U_0 / 6.
+void orc_bloom_gaussian(const float* hdr, uint32_t w, uint32_t h, const float* thr, float* out) {
+    const int levels = 5;
+    std::vector<std::vector<V3<float>>> L(levels + 1);
+    uint32_t lw[levels + 1], lh[levels + 1];
+    lw[0] = w;
+    lh[0] = h;
+    L[0].resize((size_t)w * h);
+    for (size_t i = 0; i < (size_t)w * h; ++i) L[0][i] = threshold_px(V3<float>(hdr[3 * i], hdr[3 * i + 1], hdr[3 * i + 2]), thr);
+    const float wt[5] = {1.0f, 4.0f, 6.0f, 4.0f, 1.0f};
+    for (int k = 1; k <= levels; ++k) {
+        lw[k] = (lw[k - 1] + 1) / 2;
+        lh[k] = (lh[k - 1] + 1) / 2;
+        L[k].resize((size_t)lw[k] * lh[k]);
+        for (uint32_t y = 0; y < lh[k]; ++y)
+            for (uint32_t x = 0; x < lw[k]; ++x) {
+                V3<float> acc(0);
+                for (int j = 0; j < 5; ++j) {
+                    const int sy = std::min(std::max((int)(2 * y) + j - 2, 0), (int)lh[k - 1] - 1);
+                    V3<float> row(0);
+                    for (int i = 0; i < 5; ++i) {
+                        const int sx = std::min(std::max((int)(2 * x) + i - 2, 0), (int)lw[k - 1] - 1);
+                        row += L[k - 1][(size_t)sy * lw[k - 1] + sx] * wt[i];
+                    }
+                    acc += row * wt[j];
+                }
+                L[k][(size_t)y * lw[k] + x] = acc / 256.0f;
+            }
+    }
+    auto up = [&](int k, uint32_t x, uint32_t y) {  // level k sampled at level k-1's pixel (x, y)
+        const uint32_t x0 = x / 2, y0 = y / 2;
+        const uint32_t x1 = (x & 1u) ? std::min(x0 + 1u, lw[k] - 1u) : x0, y1 = (y & 1u) ? std::min(y0 + 1u, lh[k] - 1u) : y0;
+        const auto& C = L[k];
+        const V3<float> a = C[(size_t)y0 * lw[k] + x0], b = C[(size_t)y0 * lw[k] + x1];
+        const V3<float> c = C[(size_t)y1 * lw[k] + x0], d = C[(size_t)y1 * lw[k] + x1];
+        return ((a + b) * 0.5f + (c + d) * 0.5f) * 0.5f;
+    };
+    for (int k = levels - 1; k >= 1; --k)
+        for (uint32_t y = 0; y < lh[k]; ++y)
+            for (uint32_t x = 0; x < lw[k]; ++x) L[k][(size_t)y * lw[k] + x] += up(k + 1, x, y);
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x) {
+            const size_t i = (size_t)y * w + x;
+            const V3<float> u0 = L[0][i] + up(1, x, y);
+            const V3<float> o = V3<float>(hdr[3 * i], hdr[3 * i + 1], hdr[3 * i + 2]) + u0 / 6.0f;
+            out[3 * i] = o.x; out[3 * i + 1] = o.y; out[3 * i + 2] = o.z;
+        }
 }
 
 // reinhard_kernel / gamma_kernel, lib/src/Renderer/ToneMapper.cu:17-69 (truncating uint8 casts)

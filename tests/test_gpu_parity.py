@@ -452,6 +452,35 @@ def test_bloom_and_tonemap_parity():
             assert np.array_equal(post.get_post_process_buffer(), want)
 
 
+def test_gaussian_bloom_parity():
+    """The Gaussian-pyramid bloom mode (an addition) against the oracle, bit for bit, at sizes with
+    odd and even levels; the fused tone map writes the same buffers."""
+    import torch
+    rng = np.random.default_rng(41)
+    for (h, w) in [(2, 2), (37, 53), (64, 64), (270, 480), (1080, 1920)]:
+        hdr = (rng.exponential(0.7, (h, w, 3)) * rng.choice([0.2, 1.0, 6.0], (h, w, 1))).astype(np.float32)
+        t = torch.from_numpy(hdr).cuda()
+        torch.cuda.synchronize()
+        post = cb.PostProcessor()
+        post.set_bloom_pyramid(cb.BloomPyramid.GAUSSIAN5)
+        post.register_image(t.data_ptr(), w, h)
+        curve = cb.threshold_curve(1.0, 0.5)
+        post.bloom(curve)
+        want = oracle.bloom_gaussian(hdr, curve)
+        out = post.get_post_process_buffer()
+        assert np.array_equal(out, want), (h, w, float(np.abs(out - want).max()))
+        tm = cb.ToneMapper(cb.ToneMappingType.REINHARD)
+        tm.register_image(post.buffer_device_ptr(), w, h)
+        tm.tone_map()
+        tm2 = cb.ToneMapper(cb.ToneMappingType.REINHARD)
+        tm2.register_image(t.data_ptr(), w, h)
+        tm2.tone_map_bloom(post, curve)
+        assert np.array_equal(tm2.get_render_buffer(), tm.get_render_buffer())
+        post.set_bloom_pyramid(cb.BloomPyramid.REFERENCE)  # and back to the reference's pyramid
+        post.bloom(curve)
+        assert np.array_equal(post.get_post_process_buffer(), oracle.bloom(hdr, curve))
+
+
 def test_save_to_file_writes_a_bottom_up_bmp(tmp_path):
     import torch
     hdr = np.zeros((4, 6, 3), np.float32)

@@ -507,6 +507,55 @@ def test_bloom_energy_conservation():
     np.testing.assert_allclose(out.sum((0, 1)) - img.sum((0, 1)), 2 * thr.sum((0, 1)), rtol=1e-5)
 
 
+def _thr(img):
+    t = _curve()
+    b = img.max(-1)
+    rq = np.clip(b - t[1], 0, t[2])
+    s = np.maximum(t[3] * rq * rq, b - t[0]) / np.maximum(b, 1e-5)
+    return img * s[..., None]
+
+
+def test_gaussian_bloom_constant_image_closed_form():
+    """Gaussian-pyramid mode (an addition): on a constant image every level is the constant
+    thresholded value t, U_0 = 6 t, so out = c + t."""
+    for val in (0.5, 1.7, 4.0):
+        img = np.tile(np.array([val, val * 0.5, val * 0.25], np.float32), (40, 64, 1))
+        np.testing.assert_allclose(oracle.bloom_gaussian(img, _curve()), img + _thr(img), rtol=1e-6)
+
+
+def test_bloom_smallest_images_closed_form():
+    """Images down to 2 x 2, where the reference pyramid's deeper levels are empty: both modes keep
+    their constant-image closed forms (c (1 + 2 s) and c + threshold(c))."""
+    for (h, w) in [(2, 2), (2, 3), (3, 2), (5, 7)]:
+        img = np.tile(np.array([3.0, 1.5, 0.75], np.float32), (h, w, 1))
+        t = _thr(img)
+        np.testing.assert_allclose(oracle.bloom(img, _curve()), img + 2 * t, rtol=1e-6)
+        np.testing.assert_allclose(oracle.bloom_gaussian(img, _curve()), img + t, rtol=1e-6)
+
+
+def test_gaussian_bloom_energy_conservation():
+    """The binomial [1 4 6 4 1] puts half its weight on even and half on odd taps, so keeping every
+    other blurred texel keeps a quarter of a level's sum; the up-sampling spreads each coarse texel
+    over fine pixels with weights summing to 4. Away from the borders the six levels' sums then add
+    to 6 times the thresholded image's, and sum(out) - sum(hdr) = sum(threshold(hdr))."""
+    img = np.zeros((512, 512, 3), np.float32)
+    img[250:262, 248:259] = [5.0, 3.0, 2.0]
+    out = oracle.bloom_gaussian(img, _curve())
+    np.testing.assert_allclose(out.sum((0, 1), dtype=np.float64) - img.sum((0, 1), dtype=np.float64),
+                               _thr(img).sum((0, 1), dtype=np.float64), rtol=1e-5)
+    assert (out >= img).all()
+
+
+def test_gaussian_bloom_mirror_symmetry():
+    """On a 65 x 129 image every level has odd sizes (33, 17, 9, 5, 3 and 65, 33, 17, 9, 5), so the
+    kept texels sit symmetrically and mirroring the input mirrors the output."""
+    rng = np.random.default_rng(12)
+    img = (rng.exponential(0.8, (129, 65, 3)) * rng.choice([0.3, 1.0, 5.0], (129, 65, 1))).astype(np.float32)
+    out = oracle.bloom_gaussian(img, _curve())
+    np.testing.assert_allclose(oracle.bloom_gaussian(img[:, ::-1].copy(), _curve())[:, ::-1], out, rtol=2e-6)
+    np.testing.assert_allclose(oracle.bloom_gaussian(img[::-1].copy(), _curve())[::-1], out, rtol=2e-6)
+
+
 def test_tonemap_against_fp64_formulas():
     """Reinhard-named exponential map and the sRGB OETF (ToneMapper.cu:30-36,45-48), and the ACES
     mode (Narkowicz's fit x(2.51x + 0.03) / (x(2.43x + 0.59) + 0.14), then gamma 1/2.2), in FP64
