@@ -231,8 +231,13 @@ int scene_finalize(cupbr_scene* s) {
         rank += cnt;
     }
 
+    // leaf words hold the first primitive in 27 bits; inner-node loads use 32-bit byte offsets
+    if (bprims.size() >= (size_t(1) << 27))
+        return set_error(CUPBR_ERR_INVALID_ARGUMENT, "scene too large: at most 2^27 bounded primitives");
     BvhBuildOptions opt;
     Bvh4 bvh = build_bvh4(bprims, opt);
+    if (bvh.nodes.size() >= (size_t(1) << 25))
+        return set_error(CUPBR_ERR_INVALID_ARGUMENT, "scene too large: its BVH needs more than 2^25 nodes");
     std::vector<uint32_t> rank_prim(rank, 0xFFFFFFFFu);
     for (size_t k = 0; k < bvh.order.size(); ++k) rank_prim[src[bvh.order[k]].rank] = (uint32_t)k;
     for (size_t k = 0; k < unb.size(); ++k) rank_prim[float_to_bits(unb[k].n.w)] = kUnbBit | (uint32_t)k;

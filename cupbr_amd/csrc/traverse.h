@@ -20,10 +20,6 @@
 
 #include "scene_types.h"
 
-#ifndef CUPBR_NODE_LANE_OFFSETS
-#define CUPBR_NODE_LANE_OFFSETS 0
-#endif
-
 namespace cupbr {
 
 // Triangle::computeRayIntersection, lib/src/Geometry/TriangleDetail.h:21-57.
@@ -187,23 +183,18 @@ __device__ __forceinline__ bool visit_inner(const DevScene& S, uint32_t& cur, v3
     const float inf = __builtin_inff();
     const bool sx = ix >= 0.0f, sy = iy >= 0.0f, sz = iz >= 0.0f;
     const float tlo = -1e-5f;
-#if CUPBR_NODE_LANE_OFFSETS
     // each lane loads its near and far planes directly: byte offsets lo (0, 16, 32) or hi (48,
     // 64, 80) chosen by the sign of the inverse direction
-    const char* nb = reinterpret_cast<const char*>(S.nodes + cur);
+    const char* base = reinterpret_cast<const char*>(S.nodes);
+    const uint32_t nb = cur * 128u;  // 32-bit byte offsets: scene_finalize keeps the nodes below 2^25
     const uint32_t ox = sx ? 0u : 48u, oy = sy ? 16u : 64u, oz = sz ? 32u : 80u;
-    const float4 nx = *reinterpret_cast<const float4*>(nb + ox), fx = *reinterpret_cast<const float4*>(nb + (48u - ox));
-    const float4 ny = *reinterpret_cast<const float4*>(nb + oy), fy = *reinterpret_cast<const float4*>(nb + (80u - oy));
-    const float4 nz = *reinterpret_cast<const float4*>(nb + oz), fz = *reinterpret_cast<const float4*>(nb + (112u - oz));
-    const uint4 ch = *reinterpret_cast<const uint4*>(nb + 96);
-#else
-    const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
-    const float4 lx = nf[0], ly = nf[1], lz = nf[2], hx = nf[3], hy = nf[4], hz = nf[5];
-    const uint4 ch = *reinterpret_cast<const uint4*>(nf + 6);
-    const float4 nx = sx ? lx : hx, fx = sx ? hx : lx;
-    const float4 ny = sy ? ly : hy, fy = sy ? hy : ly;
-    const float4 nz = sz ? lz : hz, fz = sz ? hz : lz;
-#endif
+    const float4 nx = *reinterpret_cast<const float4*>(base + (nb + ox));
+    const float4 fx = *reinterpret_cast<const float4*>(base + (nb + (48u - ox)));
+    const float4 ny = *reinterpret_cast<const float4*>(base + (nb + oy));
+    const float4 fy = *reinterpret_cast<const float4*>(base + (nb + (80u - oy)));
+    const float4 nz = *reinterpret_cast<const float4*>(base + (nb + oz));
+    const float4 fz = *reinterpret_cast<const float4*>(base + (nb + (112u - oz)));
+    const uint4 ch = *reinterpret_cast<const uint4*>(base + (nb + 96u));
     float key[4];
     uint32_t val[4] = {ch.x, ch.y, ch.z, ch.w};
 #define CUPBR_CHILD(i, c)                                                                          \
