@@ -869,6 +869,8 @@ def integrator_setup(args):
     r.register_scene(scene)
     r.set_method(method)
     r.set_max_trace_depth(depth)
+    if args.workload == "cornell":
+        r.set_rng(cb.RNG_PCG)  # configuration C0 names a seeded PCG generator
 
     def step():
         r.reset()
@@ -950,7 +952,12 @@ def bench_integrator(args, log):
     extra = {}
     if args.workload == "cornell":  # the frame against the FP64 oracle with the same seeds (untimed)
         img = r.get_output_image().astype(np.float64)
-        r64 = oracle.OracleScene(desc, precisions=(64,)).render(cam.as_dict(), W, H, 0, spp, depth=depth, precision=64)
+        oracle.set_rng(True)  # the oracle's PCG mode, as the renderer's
+        try:
+            r64 = oracle.OracleScene(desc, precisions=(64,)).render(cam.as_dict(), W, H, 0, spp, depth=depth,
+                                                                    precision=64)
+        finally:
+            oracle.set_rng(False)
         extra["rel_mse_vs_fp64_oracle"] = float(np.mean((img - r64) ** 2 / (r64 ** 2 + 1e-2)))
         log(f"cornell: rel-MSE against the FP64 oracle (same seeds) {extra['rel_mse_vs_fp64_oracle']:.2e}")
     cpu = None
@@ -961,7 +968,11 @@ def bench_integrator(args, log):
         cs = {"gdpt": 128, "volume": 640, "cornell": 256}[args.workload]  # ~10 s of CPU work on 16 threads
         t0 = time.perf_counter()
         if args.workload == "cornell":
-            ref.render(cam.as_dict(), W, H, 0, cs, depth=depth, precision=32, threads=threads)
+            oracle.set_rng(True)
+            try:
+                ref.render(cam.as_dict(), W, H, 0, cs, depth=depth, precision=32, threads=threads)
+            finally:
+                oracle.set_rng(False)
         elif args.workload == "gdpt":
             bufs = ref.gdpt(cam.as_dict(), W, H, 0, cs, depth=depth, precision=32, threads=threads)
             oracle.gdpt_reconstruct(bufs.astype(np.float32), 50)
@@ -988,7 +999,8 @@ def bench_integrator(args, log):
         "config": {"value_from": "median step time","workload": {"gdpt": f"gradient-domain PT, Cornell box {W}x{H}, {spp} spp, depth {depth}, 5 paths "
                                          f"per sample + 50 reconstruction iterations",
                                 "volume": f"volume renderer (homogeneous media), {W}x{H}, {spp} spp, depth {depth}",
-                                "cornell": f"PT+NEE, Cornell box (34 triangles), {W}x{H}, {spp} spp, depth {depth}"}[
+                                "cornell": f"PT+NEE, Cornell box (34 triangles), {W}x{H}, {spp} spp, depth {depth}, "
+                                           f"PCG random numbers"}[
                                     args.workload],
                    "width": W, "height": H, "spp": spp, "max_depth": depth,
                    "mrays_per_s": round(rays / t_med / 1e6, 1), **extra},

@@ -39,6 +39,7 @@ class CupbrError(RuntimeError):
 
 
 GDPT_TRIMMED_MEAN, GDPT_SCREENED_POISSON = 0, 1
+RNG_TEA_LCG, RNG_PCG = 0, 1  # cupbr_renderer_set_rng; PCG is not in the reference
 
 
 class RenderingMethod(enum.IntEnum):  # PBRenderer.h:14-22
@@ -245,6 +246,7 @@ def lib():
             "cupbr_nrc_matrix_core_product": (C.c_int, [P, P, P]),
             "cupbr_renderer_set_radiance_cache": (C.c_int, [P, P, U32, U32, U32, C.c_float]),
             "cupbr_renderer_set_radiance_cache_camera_hit": (C.c_int, [P, C.c_float]),
+            "cupbr_renderer_set_rng": (C.c_int, [P, C.c_int32]),
             "cupbr_renderer_save_pfm": (C.c_int, [P, I32, C.c_char_p]),
             "cupbr_pick": (C.c_int, [P, P, U32, U32, U32, U32, P, P]),
             "cupbr_scene_update_material": (C.c_int, [P, I32, P]),
@@ -872,6 +874,11 @@ class PBRenderer:
         and GGX surfaces of roughness >= min_roughness; inf = never (the default). See
         cupbr_renderer_set_radiance_cache_camera_hit."""
         _check(lib().cupbr_renderer_set_radiance_cache_camera_hit(self._h, float(min_roughness)))
+
+    def set_rng(self, rng: int = 0):
+        """RNG_TEA_LCG (the reference's, the default) or RNG_PCG for PATHTRACER frames; see
+        cupbr_renderer_set_rng."""
+        _check(lib().cupbr_renderer_set_rng(self._h, int(rng)))
 
     def set_gdpt_reconstruction(self, mode: int = 0, alpha: float = 0.2, iterations: int = 50):
         """GRADIENTDOMAIN reconstruction: GDPT_TRIMMED_MEAN (the reference's) or

@@ -557,6 +557,14 @@ int cupbr_renderer_set_gdpt_reconstruction(cupbr_renderer* r, int32_t mode, floa
     return CUPBR_OK;
 }
 
+int cupbr_renderer_set_rng(cupbr_renderer* r, int32_t rng) {
+    CHECK_ARG(r, "null renderer");
+    CHECK_ARG(rng == CUPBR_RNG_TEA_LCG || rng == CUPBR_RNG_PCG, "unknown random-number generator");
+    CUPBR_HIP_CHECK(hipStreamSynchronize(r->stream));
+    r->rng = rng;
+    return CUPBR_OK;
+}
+
 int cupbr_renderer_set_profiling(cupbr_renderer* r, int32_t enable) {
     CHECK_ARG(r, "null renderer");
     r->profiling = enable != 0;

@@ -101,6 +101,17 @@ __device__ __forceinline__ bool is_training_path(uint32_t pix, uint32_t frame, u
 }
 
 // rad[p].w carries the path's flags: bit 0 = radiance-cache training path
+// a path's first generator state: the reference's tea<4>(pixel, frame) (PathTracer.cu:112), or for
+// the PCG mode the PCG hash of the pixel keyed by the hash of the frame
+template <class Seed> __device__ __forceinline__ Seed path_seed(uint32_t pix, uint32_t frame);
+template <> __device__ __forceinline__ uint32_t path_seed<uint32_t>(uint32_t pix, uint32_t frame) {
+    return tea4(pix, frame);
+}
+template <> __device__ __forceinline__ Pcg32 path_seed<Pcg32>(uint32_t pix, uint32_t frame) {
+    return Pcg32{pcg_hash(pcg_hash(frame) ^ pix)};
+}
+
+template <class Seed>
 __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0,
                                                         uint32_t npath, float4* __restrict__ q_o,
                                                         float4* __restrict__ q_d, float4* __restrict__ weight,
@@ -123,12 +134,12 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
     tile_pixel(p - s * npix, w, h, px, py);  // the queue holds 8x8 pixel tiles
     const uint32_t pix = py * w + px;
     const uint32_t pid = s * npix + pix;  // path id: frame-major, pixels in row order
-    uint32_t seed = tea4(pix, frame0 + s);  // PathTracer.cu:112
+    Seed seed = path_seed<Seed>(pix, frame0 + s);
     v3 o, d;
     launch_ray(px, py, w, h, cam, seed, o, d);
     q_o[p] = make_float4(o.x, o.y, o.z, 0.0f);
     q_d[p] = make_float4(d.x, d.y, d.z, __uint_as_float(pid));
-    weight[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(seed));
+    weight[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(state_bits(seed)));
     const uint32_t flags = is_training_path(pix, frame0 + s, train_thresh) ? 1u : 0u;
     rad[pid] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(flags));
 }
@@ -145,7 +156,7 @@ struct NrcShade {
     float camera_min_roughness;
 };
 
-template <bool kNrc>
+template <bool kNrc, class Seed>
 __device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bounce, uint32_t max_depth, int rr,
                                          const float4* __restrict__ q_o, const float4* __restrict__ q_d,
                                          const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
@@ -170,7 +181,7 @@ __device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bo
             const float4 W = weight[p];
             const float4 R = rad[p];
             v3 w3 = mk(W.x, W.y, W.z);
-            uint32_t seed = __float_as_uint(W.w);
+            Seed seed = seed_from_bits<Seed>(__float_as_uint(W.w));
             v3 L = mk(R.x, R.y, R.z);
             const uint32_t prim = __float_as_uint(hh.y);
             if (prim == kNoHit) {  // :127-135
@@ -249,7 +260,7 @@ __device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bo
                         }
                     }
                 }
-                weight[p] = make_float4(w3.x, w3.y, w3.z, __uint_as_float(seed));
+                weight[p] = make_float4(w3.x, w3.y, w3.z, __uint_as_float(state_bits(seed)));
             }
             rad[p] = make_float4(L.x, L.y, L.z, R.w);
         }
@@ -278,23 +289,27 @@ __device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bo
 // Shade without the radiance cache. amdgpu_waves_per_eu(5, 8): 96 VGPRs instead of 102 and 5
 // waves per SIMD instead of 4, without spills (23.7 -> 23.1 ms of shading per 1080p 64-spp step;
 // 6 and 8 waves spill and are slower). The cache variant keeps its 123 VGPRs (it would spill).
+template <class Seed>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
 void k_pt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int rr, const float4* __restrict__ q_o,
                 const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
                 float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
                 float4* __restrict__ s_c, float4* __restrict__ weight, float4* __restrict__ rad, Counters* c,
                 NrcShade nrc) {
-    pt_shade<false>(S, cur, bounce, max_depth, rr, q_o, q_d, q_hit, nq_o, nq_d, s_o, s_d, s_c, weight, rad, c, nrc);
+    pt_shade<false, Seed>(S, cur, bounce, max_depth, rr, q_o, q_d, q_hit, nq_o, nq_d, s_o, s_d, s_c, weight, rad, c,
+                          nrc);
 }
 
 // Shade with the radiance cache: queries and training rows (NrcShade)
+template <class Seed>
 __global__ __launch_bounds__(kPtBlock)
 void k_pt_shade_nrc(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int rr, const float4* __restrict__ q_o,
                     const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
                     float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
                     float4* __restrict__ s_c, float4* __restrict__ weight, float4* __restrict__ rad, Counters* c,
                     NrcShade nrc) {
-    pt_shade<true>(S, cur, bounce, max_depth, rr, q_o, q_d, q_hit, nq_o, nq_d, s_o, s_d, s_c, weight, rad, c, nrc);
+    pt_shade<true, Seed>(S, cur, bounce, max_depth, rr, q_o, q_d, q_hit, nq_o, nq_d, s_o, s_d, s_c, weight, rad, c,
+                         nrc);
 }
 
 // query count after the second bounce's shade: the queries inferred on the side stream
@@ -988,7 +1003,10 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     // camera rays as wave packets (the visit-counting pass keeps the per-ray traversal, whose
     // counts define the algorithmic bytes of a trace launch)
     const bool packet = !r->counting && r->scene->stack_entries <= kPacketStack && S.n_nodes > 0;
-    auto shade = nrc ? k_pt_shade_nrc : k_pt_shade;
+    const bool pcg = r->rng == CUPBR_RNG_PCG;
+    auto shade = nrc ? (pcg ? k_pt_shade_nrc<Pcg32> : k_pt_shade_nrc<uint32_t>)
+                     : (pcg ? k_pt_shade<Pcg32> : k_pt_shade<uint32_t>);
+    auto raygen = pcg ? k_pt_raygen<Pcg32> : k_pt_raygen<uint32_t>;
     NrcShade ns{nullptr, nullptr, 0, __builtin_inff()};
     uint32_t train_thresh = 0;
     if (nrc) {
@@ -1012,7 +1030,7 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
         const uint32_t nb = std::min(batch, spp - s0);
         bool side_pending = false;  // cache inference of this batch running on the side stream
         const uint32_t npath = nb * npix;
-        hipLaunchKernelGGL(k_pt_raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
+        hipLaunchKernelGGL(raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
                            h, frame0 + s0, npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
                            wf.weight.as<float4>(), wf.radiance.as<float4>(), c, train_thresh);
         if ((rc = stage_mark(r, 3))) return rc;
@@ -1132,7 +1150,7 @@ int volume_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t 
         const uint32_t npath = nb * npix;
         // the path tracer's ray generation: camera rays in 8x8 tiles, seed tea<4>(pixel, frame)
         // (VolumeRenderer.cu:229-234); rad[p].w = 0: outside every object
-        hipLaunchKernelGGL(k_pt_raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
+        hipLaunchKernelGGL(k_pt_raygen<uint32_t>, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
                            h, frame0 + s0, npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
                            wf.weight.as<float4>(), wf.radiance.as<float4>(), c, 0u);
         if ((rc = stage_mark(r, 3))) return rc;

@@ -85,6 +85,7 @@ struct cupbr_renderer {
     uint32_t max_trace_depth = 5;   // PBRenderer.cu:59
     bool russian_roulette = false;  // uninitialised in the reference (PBRenderer.cu:51); off here
     int32_t gdpt_mode = CUPBR_GDPT_TRIMMED_MEAN;  // reconstruction of GRADIENTDOMAIN frames
+    int32_t rng = CUPBR_RNG_TEA_LCG;              // random numbers of PATHTRACER frames
     float gdpt_alpha = 0.2f;                      // screened-Poisson weight of the primal image
     uint32_t gdpt_iterations = 50;                // screened-Poisson Jacobi iterations
     cupbr::DevBuf images[cupbr::kNumImages];

@@ -77,7 +77,8 @@ struct DirPdf {
     v3 d;
     float pdf;
 };
-__device__ __forceinline__ DirPdf sample_glass(const Mat& m, uint32_t& seed, v3 i, v3 n0) {  // :143-169
+template <class Seed>
+__device__ __forceinline__ DirPdf sample_glass(const Mat& m, Seed& seed, v3 i, v3 n0) {  // :143-169
     const float ndotv = dot(i, n0);
     bool outside = ndotv > 0.0f;
     float eta = outside ? 1.0f / m.eta : m.eta;
@@ -93,7 +94,8 @@ __device__ __forceinline__ DirPdf sample_glass(const Mat& m, uint32_t& seed, v3 
     return {dir, 1.0f};
 }
 // Material::sampleDirection, MaterialDetail.h:76-111
-__device__ __forceinline__ DirPdf sample_direction(const Mat& m, uint32_t& seed, v3 i, v3 n) {
+template <class Seed>
+__device__ __forceinline__ DirPdf sample_direction(const Mat& m, Seed& seed, v3 i, v3 n) {
     switch (m.type) {
         case CUPBR_LAMBERT:
         case CUPBR_PHONG: {  // sample_lambert :113-132
@@ -133,7 +135,8 @@ __device__ __forceinline__ float hg_phase(float g, float cos_t) {
     return (1.0f - g2) / area * powf(1.0f + g2 - 2.0f * g * cos_t, -1.5f);
 }
 // sampleHenyeyGreensteinPhaseFunction, :288-313
-__device__ __forceinline__ DirPdf hg_sample(float g_, v3 fwd, uint32_t& seed) {
+template <class Seed>
+__device__ __forceinline__ DirPdf hg_sample(float g_, v3 fwd, Seed& seed) {
     float g = g_ < 0.0f ? fminf(-1e-3f, g_) : fmaxf(1e-3f, g_);
     float u1 = rnd(seed), u2 = rnd(seed);
     float g2 = g * g;
@@ -146,7 +149,8 @@ __device__ __forceinline__ DirPdf hg_sample(float g_, v3 fwd, uint32_t& seed) {
 }
 
 // Light::sample, lib/src/DataStructure/LightDetail.h:8-52
-__device__ __forceinline__ v3 light_sample(const cupbr_light& L, uint32_t& seed, v3 pos, v3& dir, float& dist) {
+template <class Seed>
+__device__ __forceinline__ v3 light_sample(const cupbr_light& L, Seed& seed, v3 pos, v3& dir, float& dist) {
     if (L.type == CUPBR_POINT) {
         v3 lp = ld3(L.position);
         dir = normalize(lp - pos);
@@ -183,7 +187,8 @@ __device__ __forceinline__ v3 env_lookup(const DevScene& S, v3 d) {
 // VolumeRenderer.cu / GradientDomain.cu). Returns false when there is nothing to sample:
 // no lights and no environment, or an environment direction from a failed BSDF sample
 // (a zero vector, whose shadow ray would carry NaN).
-__device__ __forceinline__ bool nee_sample(const DevScene& S, const Mat& m, uint32_t& seed, v3 P, v3 inc, v3 N, v3& ldir,
+template <class Seed>
+__device__ __forceinline__ bool nee_sample(const DevScene& S, const Mat& m, Seed& seed, v3 P, v3 inc, v3 N, v3& ldir,
                                            v3& le, float& dist) {
     const uint32_t pick = (uint32_t)(rnd(seed) * (float)(S.n_lights + (S.use_env ? 1u : 0u)));
     if (pick != S.n_lights && S.n_lights) {
@@ -200,8 +205,9 @@ __device__ __forceinline__ bool nee_sample(const DevScene& S, const Mat& m, uint
 }
 
 // Tracing::launchRay, TracingDetail.h:15-45 (+ Ray's normalisation)
+template <class Seed>
 __device__ __forceinline__ void launch_ray(uint32_t px, uint32_t py, uint32_t w, uint32_t h, const cupbr_camera& cam,
-                                           uint32_t& seed, v3& o, v3& d) {
+                                           Seed& seed, v3& o, v3& d) {
     float jx = rnd(seed), jy = rnd(seed);
     const float rx = 2.0f * (((float)px + jx) / (float)w - 0.5f);
     const float ry = 2.0f * (((float)py + jy) / (float)h - 0.5f);

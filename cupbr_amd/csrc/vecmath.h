@@ -74,6 +74,29 @@ CUPBR_HD float rnd(uint32_t& s) {
     s = 1664525u * s + 1013904223u;
     return (float)(s & 0x00FFFFFFu) / (float)0x01000000;
 }
+// PCG random numbers, the path tracer's alternative to the reference's TEA + LCG (no counterpart in
+// the reference; O'Neill 2014, "PCG: a family of simple fast space-efficient statistically good
+// algorithms for random number generation"): a 32-bit LCG state (multiplier 747796405, increment
+// 2891336453) whose output is the RXS-M-XS permutation of the advanced state; the top 24 bits make
+// the float. The state fits the same 32 bits as the reference's LCG state.
+struct Pcg32 {
+    uint32_t s;
+};
+CUPBR_HD uint32_t pcg_rxs_m_xs(uint32_t state) {
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+CUPBR_HD uint32_t pcg_hash(uint32_t x) { return pcg_rxs_m_xs(x * 747796405u + 2891336453u); }
+CUPBR_HD float rnd(Pcg32& g) {
+    g.s = g.s * 747796405u + 2891336453u;
+    return (float)(pcg_rxs_m_xs(g.s) >> 8) / (float)0x01000000;
+}
+// the 32 bits a path keeps between bounces, and back
+CUPBR_HD uint32_t state_bits(uint32_t s) { return s; }
+CUPBR_HD uint32_t state_bits(Pcg32 g) { return g.s; }
+template <class Seed> CUPBR_HD Seed seed_from_bits(uint32_t s);
+template <> CUPBR_HD uint32_t seed_from_bits<uint32_t>(uint32_t s) { return s; }
+template <> CUPBR_HD Pcg32 seed_from_bits<Pcg32>(uint32_t s) { return Pcg32{s}; }
 CUPBR_HD v3 to_local_frame(v3 N, v3 d) {
     const float sz = (N.z >= 0.0f) ? 1.0f : -1.0f;
     const float a = 1.0f / (sz + N.z);

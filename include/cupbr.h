@@ -334,6 +334,17 @@ int cupbr_renderer_synchronize(cupbr_renderer* r);
 #define CUPBR_GDPT_SCREENED_POISSON 1
 int cupbr_renderer_set_gdpt_reconstruction(cupbr_renderer* r, int32_t mode, float alpha, uint32_t iterations);
 
+/* Random numbers of the PATHTRACER method. CUPBR_RNG_TEA_LCG is the reference's: a path's state is
+ * tea<4>(pixel, frame) and each draw one step of the LCG 1664525 s + 1013904223, its low 24 bits
+ * divided by 2^24 (FunctionsDetail.h:70-98, PathTracer.cu:112). CUPBR_RNG_PCG has no counterpart in
+ * the reference: the state starts at pcg_hash(pcg_hash(frame) ^ pixel) and each draw advances it as
+ * s = 747796405 s + 2891336453 and returns the top 24 bits of the RXS-M-XS permutation of s
+ * (O'Neill 2014), divided by 2^24, where pcg_hash(x) is that permutation of 747796405 x +
+ * 2891336453. The other methods always use the reference's generator. */
+#define CUPBR_RNG_TEA_LCG 0
+#define CUPBR_RNG_PCG 1
+int cupbr_renderer_set_rng(cupbr_renderer* r, int32_t rng);
+
 /* ------------------------------------------------------------------ post processing (PostProcessor.h:12-52) */
 int cupbr_post_create(cupbr_post** out);
 int cupbr_post_destroy(cupbr_post* p);

@@ -48,6 +48,9 @@ def lib():
         L.orc_tea4.restype = C.c_uint32
         L.orc_tea4.argtypes = [C.c_uint32, C.c_uint32]
         L.orc_rnd_seq.argtypes = [C.c_uint32, C.c_uint32, P, P]
+        L.orc_set_rng.argtypes = [C.c_int]
+        L.orc_pcg_hash.restype = C.c_uint32
+        L.orc_pcg_hash.argtypes = [C.c_uint32]
         L.orc_leaf_order.restype = C.c_uint32
         L.orc_leaf_order.argtypes = [P, P, P]
         L.orc_trace.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, P, P, P, P, P, C.c_int]
@@ -87,6 +90,15 @@ def _c(a, dt):
 
 def tea4(a: int, b: int) -> int:
     return int(lib().orc_tea4(a & 0xFFFFFFFF, b & 0xFFFFFFFF))
+
+
+def set_rng(pcg: bool):
+    """PCG generator (the product's CUPBR_RNG_PCG) for rnd() and path-tracer seeding while set."""
+    lib().orc_set_rng(int(bool(pcg)))
+
+
+def pcg_hash(x: int) -> int:
+    return int(lib().orc_pcg_hash(x & 0xFFFFFFFF))
 
 
 def rnd_seq(seed: int, n: int):

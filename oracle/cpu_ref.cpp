@@ -21,6 +21,8 @@
 
 namespace orc {
 
+int g_rng_pcg = 0;
+
 // ------------------------------------------------------------------ scene
 template <typename R>
 struct RefNode {  // BoundingVolumeHierarchy node: box + two children (>=0 node, <0 object ~idx)
@@ -426,7 +428,7 @@ static inline bool nee_sample(const SceneT<R>& S, const Material<R>& mat, uint32
 template <typename R>
 static V3<R> pt_sample(const SceneT<R>& S, const Tracer<R>& T, const Camera<R>& cam, uint32_t tid, uint32_t w, uint32_t h,
                        uint32_t frame, uint32_t maxDepth, bool rr) {
-    uint32_t seed = tea4(tid, frame);
+    uint32_t seed = g_rng_pcg ? pcg_hash(pcg_hash(frame) ^ tid) : tea4(tid, frame);
     V3<R> o, d;
     launch_ray(tid % w, tid / w, w, h, cam, seed, o, d);
     V3<R> radiance(0), weight(1), out_dir;
@@ -483,7 +485,7 @@ template <typename R>
 static V3<R> vol_sample(const SceneT<R>& S, const Tracer<R>& T, const Camera<R>& cam, uint32_t tid, uint32_t w, uint32_t h,
                         uint32_t frame, uint32_t maxDepth, bool rr) {
     const R inf = std::numeric_limits<R>::infinity();
-    uint32_t seed = tea4(tid, frame);
+    uint32_t seed = g_rng_pcg ? pcg_hash(pcg_hash(frame) ^ tid) : tea4(tid, frame);
     V3<R> o, d;
     launch_ray(tid % w, tid / w, w, h, cam, seed, o, d);
     V3<R> radiance(0), weight(1), ray_start, out_dir;
@@ -1313,6 +1315,8 @@ void orc_scene_destroy(void* p) {
 }
 
 uint32_t orc_tea4(uint32_t a, uint32_t b) { return tea4(a, b); }
+void orc_set_rng(int pcg) { g_rng_pcg = pcg ? 1 : 0; }
+uint32_t orc_pcg_hash(uint32_t x) { return pcg_hash(x); }
 
 void orc_rnd_seq(uint32_t seed, uint32_t n, float* out, uint32_t* states) {
     for (uint32_t i = 0; i < n; ++i) {

@@ -87,8 +87,22 @@ inline uint32_t tea4(uint32_t val0, uint32_t val1) {
     }
     return v0;
 }
-// FunctionsDetail.h:89-98
+// The PCG generator of the product's CUPBR_RNG_PCG mode (no counterpart in the reference; O'Neill
+// 2014): the 32-bit LCG s = 747796405 s + 2891336453 with the RXS-M-XS output permutation. While
+// g_rng_pcg is set (orc_set_rng), rnd() draws from it and path-tracer paths start at
+// pcg_hash(pcg_hash(frame) ^ pixel); the tests set it only around path-tracer renders.
+extern int g_rng_pcg;
+inline uint32_t pcg_rxs_m_xs(uint32_t state) {
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+inline uint32_t pcg_hash(uint32_t x) { return pcg_rxs_m_xs(x * 747796405u + 2891336453u); }
+// FunctionsDetail.h:89-98 (or the PCG step)
 inline float rnd(uint32_t& prev) {
+    if (g_rng_pcg) {
+        prev = prev * 747796405u + 2891336453u;
+        return (float)(pcg_rxs_m_xs(prev) >> 8) / (float)0x01000000;
+    }
     prev = 1664525u * prev + 1013904223u;
     return (float)(prev & 0x00FFFFFFu) / (float)0x01000000;
 }

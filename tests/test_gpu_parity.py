@@ -182,6 +182,62 @@ def test_pathtracer_mixed_scene_with_environment():
     assert rel_mse(img, ref) <= 1e-6
 
 
+@pytest.mark.parametrize("make,depth", [(lambda: scenes.cornell_box(), 5), (lambda: scenes.mixed_primitives(env=True), 6)])
+def test_pathtracer_pcg_same_seed_parity(make, depth):
+    """The PCG generator (an addition): the GPU against the oracle in its PCG mode with the same
+    seeds, and an image different from the reference generator's."""
+    desc, ca = make()
+    w = h = 48
+    cam = _camera(ca, w, h)
+    scene = cb.Scene.from_desc(desc)
+    r = cb.PBRenderer(cb.RenderingMethod.PATHTRACER)
+    r.set_output_size(w, h)
+    r.register_scene(scene)
+    r.set_max_trace_depth(depth)
+    r.set_rng(cb.RNG_PCG)
+    r.render(cam, 16)
+    img = r.get_output_image()
+    o = oracle.OracleScene(desc)
+    oracle.set_rng(True)
+    try:
+        ref = o.render(cam.as_dict(), w, h, 0, 16, depth=depth, precision=32)
+    finally:
+        oracle.set_rng(False)
+    e = rel_mse(img, ref)
+    print(f"PT (PCG): rel-MSE vs same-seed FP32 oracle {e:.2e}; exact pixels {np.mean(img == ref):.4f}", flush=True)
+    assert e <= 1e-6
+    assert not np.array_equal(img, _render(scene, cam, w, h, 16, depth=depth).get_output_image())
+
+
+def test_pathtracer_pcg_estimates_the_same_image():
+    """Both generators estimate one image: on the Cornell box with the light lowered to y = 0.9
+    (at 0.999 rare direct-light samples of order 1e5 dominate any two-render comparison), 256 PCG
+    frames differ from 256 reference frames by no more than two independent sets of 256 reference
+    frames (frames 0..255 and 256..511) differ from each other, up to a factor 2."""
+    desc, ca = scenes.cornell_box()
+    desc["lights"][0, 2] = 0.9
+    w = h = 48
+    cam = _camera(ca, w, h)
+    scene = cb.Scene.from_desc(desc)
+
+    def frames(n, pcg):
+        r = cb.PBRenderer(cb.RenderingMethod.PATHTRACER)
+        r.set_output_size(w, h)
+        r.register_scene(scene)
+        r.set_max_trace_depth(5)
+        if pcg:
+            r.set_rng(cb.RNG_PCG)
+        r.render(cam, n)
+        return r.get_output_image().astype(np.float64)
+
+    b1 = frames(256, False)
+    b2 = 2.0 * frames(512, False) - b1  # the mean of frames 256..511
+    pa = frames(256, True)
+    e_pcg, e_ref = rel_mse(pa, b1), rel_mse(b2, b1)
+    print(f"256 frames: PCG vs reference generator rel-MSE {e_pcg:.3e}; two reference halves {e_ref:.3e}", flush=True)
+    assert e_pcg <= 2.0 * e_ref
+
+
 def test_pathtracer_against_fp64_oracle():
     """rel-MSE <= 1e-4 against the FP64 oracle with the same seeds, and a per-pixel 3-sigma test on
     independent seeds: GPU frames 0..4095 against oracle frames 4096..8191.

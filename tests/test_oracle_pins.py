@@ -1018,3 +1018,31 @@ def test_volume_single_scattering_inside_an_object():
 
     want = quad(integrand, 0, 2 * R, limit=200)[0]
     assert abs(mean - want) < 4 * math.sqrt(var), (mean, want, math.sqrt(var))
+
+
+def test_pcg_generator_statistics():
+    """The PCG mode's generator (an addition: LCG 747796405 s + 2891336453 with the RXS-M-XS
+    output, top 24 bits): 2^20 draws from one state are uniform (mean and variance within 4 sigma,
+    chi-square over 256 bins within 4 sigma of its mean), uncorrelated at lag 1, and the state
+    follows the LCG; the hash's output permutation is a bijection (no collision over 2^16 inputs)."""
+    oracle.set_rng(True)
+    try:
+        v, st = oracle.rnd_seq(2024, 1 << 20)
+    finally:
+        oracle.set_rng(False)
+    n = len(v)
+    x = v.astype(np.float64)
+    assert abs(x.mean() - 0.5) < 4 * math.sqrt(1 / 12 / n)
+    assert abs(x.var() - 1 / 12) < 4 * math.sqrt(1 / 180 / n)
+    cnt = np.bincount((x * 256).astype(np.int64), minlength=256)
+    chi2 = float(((cnt - n / 256) ** 2 / (n / 256)).sum())
+    assert abs(chi2 - 255) < 4 * math.sqrt(2 * 255)
+    assert abs(np.corrcoef(x[:-1], x[1:])[0, 1]) < 4 / math.sqrt(n)
+    s = st.astype(np.uint64)
+    assert np.array_equal(s[1:], (s[:-1] * 747796405 + 2891336453) % (1 << 32))
+    h = np.array([oracle.pcg_hash(int(k)) for k in range(0, 1 << 16)], np.uint64)
+    assert len(np.unique(h)) == len(h)
+    # the reference's generator is untouched when the mode is off
+    v0, _ = oracle.rnd_seq(2024, 4)
+    np.testing.assert_array_equal(v0, oracle.rnd_seq(2024, 4)[0])
+    assert not np.array_equal(v0, v[:4])
