@@ -159,7 +159,11 @@ def cpu_baseline_record(value, cpu, sample):
 
 
 # --------------------------------------------------------------------------- counters (rocprofv3 children)
-VALU_COUNTERS = ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVES", "GRBM_GUI_ACTIVE"]
+VALU_COUNTERS = ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVES", "GRBM_GUI_ACTIVE",
+                 "TA_TA_BUSY_sum", "TD_TD_BUSY_sum", "TCP_TOTAL_CACHE_ACCESSES_sum"]
+# rocprofv3 reports GRBM_GUI_ACTIVE summed over the MI355X's 8 XCDs (a dispatch of t seconds reads
+# about 8 x 2.4e9 x t), so a dispatch lasts GRBM_GUI_ACTIVE / 8 cycles of each CU's clock
+N_XCD = 8
 VALU_FMA_ITERS = 20000
 COPY_MIB = 256
 # VALU issue ceiling from the published FP32 vector peak: 157.3 TFLOP/s counts packed FMAs (2 lanes
@@ -207,6 +211,8 @@ def _run_counter_child(args, counters, log, copy=False):
     if not os.path.exists(prof):
         log("counters: rocprofv3 not found")
         return None
+    if args.counter_dir:
+        os.makedirs(args.counter_dir, exist_ok=True)
     out_dir = tempfile.mkdtemp(prefix="cupbr_pmc_", dir=args.counter_dir or None)
     cmd = [prof, "--pmc", *counters, "--output-format", "csv", "-d", out_dir, "-o", "c", "--", sys.executable,
            os.path.join(ROOT, "bench.py"), "--counters-child", "--spp", str(args.spp), "--width", str(args.width),
@@ -238,7 +244,7 @@ def is_render_kernel(k):
     return "cupbr::" in k and "k_valu_peak" not in k
 
 
-def measure_counters(args, log, is_trace=is_trace_kernel):
+def measure_counters(args, log, is_trace=is_trace_kernel, n_cu=256):
     """VALU and HBM counters of the dispatches whose kernel name satisfies is_trace (the trace
     kernels by default) from three rocprofv3 runs of one step each."""
     res = {}
@@ -253,6 +259,15 @@ def measure_counters(args, log, is_trace=is_trace_kernel):
             res["valu_instructions"] = sum(d.get("SQ_INSTS_VALU", 0.0) for d in tr)
             act = sum(d.get("SQ_ACTIVE_INST_VALU", 0.0) for d in tr)
             res["lane_utilisation"] = sum(d.get("SQ_THREAD_CYCLES_VALU", 0.0) for d in tr) / max(64 * act, 1.0)
+            # vector-memory address (TA) and data (TD) units, one of each per CU: busy cycles over the
+            # dispatches' cycles
+            cyc = sum(d.get("GRBM_GUI_ACTIVE", 0.0) for d in tr) / N_XCD * n_cu
+            if cyc > 0 and any("TA_TA_BUSY_sum" in d for d in tr):
+                res["ta_busy"] = sum(d.get("TA_TA_BUSY_sum", 0.0) for d in tr) / cyc
+                res["td_busy"] = sum(d.get("TD_TD_BUSY_sum", 0.0) for d in tr) / cyc
+                # vector L1 (TCP) accesses per CU per cycle; the L1 serves one access (one cache
+                # line of one load instruction) per clock
+                res["l1_access_rate"] = sum(d.get("TCP_TOTAL_CACHE_ACCESSES_sum", 0.0) for d in tr) / cyc
             res["per_dispatch"] = [{"kernel": d["kernel"].split("(")[0].replace("void ", ""),
                                     "valu_instructions": d.get("SQ_INSTS_VALU", 0.0),
                                     "lane_utilisation": d.get("SQ_THREAD_CYCLES_VALU", 0.0) /
@@ -438,7 +453,7 @@ def bench_pt(args, log):
             "algorithmic_bytes_per_launch": int(alg_bytes_per_launch), "algorithmic_gbs": round(alg_gbs, 1),
             "bytes_per_ext_ray": round(bytes_per_ext_ray, 1), "bytes_per_shadow_ray": round(bytes_per_sh_ray, 1)}
     if not args.no_counters:
-        c = measure_counters(args, log)
+        c = measure_counters(args, log, n_cu=torch.cuda.get_device_properties(0).multi_processor_count)
         if c.get("valu_instructions"):
             t_tr = trace_ms_per_step * 1e-3  # the child's dispatches are one step's trace launches
             issue = c["valu_instructions"] / t_tr  # wave instructions per second
@@ -458,6 +473,12 @@ def bench_pt(args, log):
             log(f"trace VALU: {c['valu_instructions'] / 1e9:.1f} G wave instructions per step, lane utilisation "
                 f"{lane:.3f}, issue {issue * 64 / 1e12:.2f} T lane-slots/s = {issue * 64 / 1e12 / VALU_PEAK_TOPS:.3f} "
                 f"of peak; useful {achieved:.2f} TOP/s = {achieved / VALU_PEAK_TOPS:.3f} of peak")
+        if c.get("ta_busy") is not None:
+            roof.update({"vmem_address_unit_busy": round(c["ta_busy"], 4), "vmem_data_unit_busy": round(c["td_busy"], 4),
+                         "l1_accesses_per_cu_cycle": round(c["l1_access_rate"], 4)})
+            log(f"trace vector memory: address unit (TA) busy {c['ta_busy']:.3f}, data unit (TD) busy "
+                f"{c['td_busy']:.3f}, L1 accesses {c['l1_access_rate']:.3f} per CU per cycle, of the trace "
+                f"dispatches' cycles under the profiler")
         if c.get("fetch_bytes_per_launch") is not None and c.get("write_bytes_per_launch") is not None:
             traffic = c["fetch_bytes_per_launch"] + c["write_bytes_per_launch"]
             hbm_gbs = traffic / (trace_launch_ms * 1e-3) / 1e9
@@ -888,6 +909,7 @@ def bench_integrator(args, log):
     algorithmic BVH bytes of all traversals (counted in an instrumented pass) are reported beside
     it: these scenes' BVHs stay in L1/L2."""
     import numpy as np
+    import torch
 
     import oracle
 
@@ -928,7 +950,7 @@ def bench_integrator(args, log):
             "device_ms_per_step": round(dev_step_ms, 3), "algorithmic_bvh_bytes_per_step": int(bytes_step),
             "algorithmic_bvh_gbs": round(alg_gbs, 1)}
     if not args.no_counters:
-        c = measure_counters(args, log, is_render_kernel)
+        c = measure_counters(args, log, is_render_kernel, torch.cuda.get_device_properties(0).multi_processor_count)
         t_dev = dev_step_ms * 1e-3  # the child's dispatches are one render call
         if c.get("valu_instructions"):
             issue = c["valu_instructions"] / t_dev
@@ -942,6 +964,11 @@ def bench_integrator(args, log):
                 f"{c['trace_dispatches']} dispatches, lane utilisation {lane:.3f}, issue "
                 f"{issue * 64 / 1e12 / VALU_PEAK_TOPS:.3f} of peak; useful {achieved:.2f} TOP/s = "
                 f"{achieved / VALU_PEAK_TOPS:.3f} of peak")
+        if c.get("ta_busy") is not None:
+            roof.update({"vmem_address_unit_busy": round(c["ta_busy"], 4), "vmem_data_unit_busy": round(c["td_busy"], 4),
+                         "l1_accesses_per_cu_cycle": round(c["l1_access_rate"], 4)})
+            log(f"{args.workload} vector memory: address unit (TA) busy {c['ta_busy']:.3f}, data unit (TD) busy "
+                f"{c['td_busy']:.3f}, L1 accesses {c['l1_access_rate']:.3f} per CU per cycle")
         if c.get("fetch_bytes_total") is not None and c.get("write_bytes_total") is not None:
             traffic = c["fetch_bytes_total"] + c["write_bytes_total"]
             hbm_gbs = traffic / t_dev / 1e9

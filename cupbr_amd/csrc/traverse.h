@@ -140,17 +140,17 @@ struct LaneStack {
         if (i < kLds) lds[i * lds_stride] = v;
         else spill[(size_t)(i - kLds) * spill_stride] = v;
     }
+    // The LDS read is unconditional (its address clamped into the LDS part) and the spill read a
+    // relaxed atomic load, so that the compiler cannot merge the two into one generic (flat) load
+    // of a selected pointer: a pop stays a ds_read.
     __device__ __forceinline__ uint32_t get(int i) const {
-        if (i < kLds) return lds[i * lds_stride];
-        return spill[(size_t)(i - kLds) * spill_stride];
+        uint32_t v = lds[min(i, kLds - 1) * lds_stride];
+        if (i >= kLds)
+            v = __hip_atomic_load(spill + (size_t)(i - kLds) * spill_stride, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
+        return v;
     }
 };
-
-#define CUPBR_CSWAP(a, b)                                             \
-    if (key[b] < key[a]) {                                             \
-        float tk = key[a]; key[a] = key[b]; key[b] = tk;               \
-        uint32_t tv = val[a]; val[a] = val[b]; val[b] = tv;            \
-    }
 
 // visit counters of one lane (only updated by the counting instantiation)
 struct TravCount {
@@ -176,13 +176,18 @@ __device__ __forceinline__ bool visit_leaf(const DevScene& S, uint32_t word, v3 
 
 // One inner node `cur`: the children whose boxes overlap [tlo, lim] (conservatively) are pushed
 // farthest first, and cur becomes the nearest (closest hit) or a hit one (any hit); false when
-// no child overlaps.
+// no child overlaps. An empty child slot has the box lo = +inf, hi = -inf, which no ray overlaps.
+// Children are ordered by integer keys, the entry distance's float bits read as a signed integer
+// (non-negative distances in their order, the slightly negative ones of boxes around the origin
+// ahead of them), INT_MAX for a culled child; each compare-exchange of the network is a min, a max
+// and two selects of the child words, with no branch. The order only decides which overlapping
+// child is visited first; the closest hit is the same for any order.
 template <bool kAny, int kLds>
 __device__ __forceinline__ bool visit_inner(const DevScene& S, uint32_t& cur, v3 o, float ix, float iy, float iz,
                                             float lim, LaneStack<kLds>& st, int& sp) {
-    const float inf = __builtin_inff();
     const bool sx = ix >= 0.0f, sy = iy >= 0.0f, sz = iz >= 0.0f;
     const float tlo = -1e-5f;
+    constexpr int kCulled = 0x7FFFFFFF;
     // each lane loads its near and far planes directly: byte offsets lo (0, 16, 32) or hi (48,
     // 64, 80) chosen by the sign of the inverse direction
     const char* base = reinterpret_cast<const char*>(S.nodes);
@@ -195,35 +200,48 @@ __device__ __forceinline__ bool visit_inner(const DevScene& S, uint32_t& cur, v3
     const float4 nz = *reinterpret_cast<const float4*>(base + (nb + oz));
     const float4 fz = *reinterpret_cast<const float4*>(base + (nb + (112u - oz)));
     const uint4 ch = *reinterpret_cast<const uint4*>(base + (nb + 96u));
-    float key[4];
+    int key[4];
     uint32_t val[4] = {ch.x, ch.y, ch.z, ch.w};
 #define CUPBR_CHILD(i, c)                                                                          \
     {                                                                                              \
         float tn = fmaxf(fmaxf((nx.c - o.x) * ix, (ny.c - o.y) * iy), fmaxf((nz.c - o.z) * iz, tlo)); \
         float tf = fminf(fminf((fx.c - o.x) * ix, (fy.c - o.y) * iy), fminf((fz.c - o.z) * iz, lim)); \
-        bool ok = val[i] != kChildEmpty && tn <= tf * 1.000002f;                                  \
-        key[i] = ok ? tn : inf;                                                                    \
+        key[i] = tn <= tf * 1.000002f ? __float_as_int(tn) : kCulled;                              \
     }
     CUPBR_CHILD(0, x)
     CUPBR_CHILD(1, y)
     CUPBR_CHILD(2, z)
     CUPBR_CHILD(3, w)
 #undef CUPBR_CHILD
-    if (!kAny) {  // nearest child first; an any-hit query only needs the ones that were hit
-        CUPBR_CSWAP(0, 1)
-        CUPBR_CSWAP(2, 3)
-        CUPBR_CSWAP(0, 2)
-        CUPBR_CSWAP(1, 3)
-        CUPBR_CSWAP(1, 2)
-    } else {
-        CUPBR_CSWAP(0, 1)  // a hit child, if any, into slot 0
-        CUPBR_CSWAP(0, 2)
-        CUPBR_CSWAP(0, 3)
+    // The child words are only needed once a child overlaps; without this use the compiler moves
+    // their load behind that test, and the step then waits for a second memory round trip.
+    asm volatile("" ::"v"(val[0]), "v"(val[1]), "v"(val[2]), "v"(val[3]));
+#define CUPBR_KSWAP(a, b)                                   \
+    {                                                       \
+        const bool sw = key[b] < key[a];                    \
+        const int klo = min(key[a], key[b]);                \
+        key[b] = max(key[a], key[b]);                       \
+        key[a] = klo;                                       \
+        const uint32_t va = val[a];                         \
+        val[a] = sw ? val[b] : va;                          \
+        val[b] = sw ? va : val[b];                          \
     }
-    if (!(key[0] < inf)) return false;
-    if (key[3] < inf) st.put(sp++, val[3]);
-    if (key[2] < inf) st.put(sp++, val[2]);
-    if (key[1] < inf) st.put(sp++, val[1]);
+    if (!kAny) {  // nearest child first; an any-hit query only needs the ones that were hit
+        CUPBR_KSWAP(0, 1)
+        CUPBR_KSWAP(2, 3)
+        CUPBR_KSWAP(0, 2)
+        CUPBR_KSWAP(1, 3)
+        CUPBR_KSWAP(1, 2)
+    } else {
+        CUPBR_KSWAP(0, 1)  // the nearest hit child, if any, into slot 0
+        CUPBR_KSWAP(0, 2)
+        CUPBR_KSWAP(0, 3)
+    }
+#undef CUPBR_KSWAP
+    if (key[0] == kCulled) return false;
+    if (key[3] != kCulled) st.put(sp++, val[3]);
+    if (key[2] != kCulled) st.put(sp++, val[2]);
+    if (key[1] != kCulled) st.put(sp++, val[1]);
     cur = val[0];
     return true;
 }
@@ -350,7 +368,6 @@ __device__ __forceinline__ bool ray_step(const DevScene& S, RayState& r, LaneSta
     r.cur = st.get(--r.sp);
     return true;
 }
-#undef CUPBR_CSWAP
 
 // a load whose address is the same in every lane, issued as a scalar load (records of the
 // BVH and primitive arrays, which no kernel writes while it traverses them)
