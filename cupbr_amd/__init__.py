@@ -190,6 +190,7 @@ def lib():
             "cupbr_scene_get_lights": (C.c_int, [P, P, U32]),
             "cupbr_scene_get_volume": (C.c_int, [P, P]),
             "cupbr_scene_get_bvh_stats": (C.c_int, [P, P]),
+            "cupbr_scene_get_bvh_nodes": (C.c_int, [P, P, P, C.c_uint32]),
             "cupbr_scene_get_environment": (C.c_int, [P, P, SZ, P, P]),
             "cupbr_scene_get_reference_order": (C.c_int, [P, P, U32]),
             "cupbr_trace_closest": (C.c_int, [P, P, SZ, P, P, P]),
@@ -260,6 +261,8 @@ def lib():
             "cupbr_grid_adjoint": (C.c_int, [P, P, P, P, P, P, U32, U32, U32, U32, U32, P, P, P]),
         }
         for name, (res, args) in sig.items():
+            if os.environ.get("CUPBR_LIBRARY") and not hasattr(L, name):
+                continue  # another build, from before this entry point: calling it fails by name
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
@@ -477,6 +480,15 @@ class Scene:
         s = _BvhStats()
         _check(lib().cupbr_scene_get_bvh_stats(self.handle, C.byref(s)))
         return {k: getattr(s, k) for k, _ in _BvhStats._fields_}
+
+    def bvh_nodes(self):
+        """(fp32, half): the BVH4 nodes in both device forms as raw 128-byte records (uint8 arrays of
+        shape (nodes, 128)); the layouts are described at cupbr_scene_get_bvh_nodes."""
+        n = self.bvh_stats()["nodes"]
+        f = np.zeros((max(n, 1), 128), np.uint8)
+        h = np.zeros((max(n, 1), 128), np.uint8)
+        _check(lib().cupbr_scene_get_bvh_nodes(self.handle, _ptr(f), _ptr(h), max(n, 1)))
+        return f[:n], h[:n]
 
     def reference_order(self) -> np.ndarray:
         n = self.counts()["objects"]

@@ -4,6 +4,7 @@
 #include <array>
 #include <atomic>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <thread>
 
@@ -285,6 +286,89 @@ struct Collapser {
 };
 
 }  // namespace
+
+namespace {
+
+uint16_t f16_bits(_Float16 h) {
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+_Float16 f16_of(uint16_t b) {
+    _Float16 h;
+    std::memcpy(&h, &b, 2);
+    return h;
+}
+// the binary16 value next to b towards +inf (up) or -inf
+uint16_t f16_step(uint16_t b, bool up) {
+    const bool neg = b & 0x8000u;
+    if ((b & 0x7FFFu) == 0) return up ? 0x0001u : 0x8001u;  // +-0: smallest subnormal
+    return (neg == up) ? (uint16_t)(b - 1u) : (uint16_t)(b + 1u);
+}
+// binary16 rounding of x towards +inf (up) or -inf
+uint16_t f16_directed(double x, bool up) {
+    uint16_t b = f16_bits((_Float16)x);  // nearest (overflow to +-inf)
+    const double d = (double)f16_of(b);
+    if (up ? d < x : d > x) b = f16_step(b, up);
+    return b;
+}
+
+}  // namespace
+
+std::vector<NodeH> half_nodes(const std::vector<Node4>& nodes) {
+    constexpr uint16_t kPosInf = 0x7C00u, kNegInf = 0xFC00u;
+    std::vector<NodeH> out(nodes.size());
+    for (size_t n = 0; n < nodes.size(); ++n) {
+        const Node4& a = nodes[n];
+        NodeH h{};
+        const float* lo[3] = {a.lo_x, a.lo_y, a.lo_z};
+        const float* hi[3] = {a.hi_x, a.hi_y, a.hi_z};
+        uint16_t(*dst[3])[8] = {h.x, h.y, h.z};
+        float c[3];
+        uint32_t exps = 0;
+        for (int ax = 0; ax < 3; ++ax) {
+            double cmin = INFINITY, hmax = -INFINITY;
+            for (int i = 0; i < 4; ++i) {
+                if (std::isfinite(lo[ax][i])) cmin = std::min(cmin, (double)lo[ax][i]);
+                if (std::isfinite(hi[ax][i])) hmax = std::max(hmax, (double)hi[ax][i]);
+            }
+            if (!std::isfinite(cmin)) cmin = 0.0;
+            const double e = std::isfinite(hmax) ? std::max(0.0, hmax - cmin) : 0.0;
+            // the reference point: a float near the middle, so that offsets reach only about e / 2
+            // either way (binary16 keeps 11 bits of each)
+            c[ax] = (float)(cmin + 0.5 * e);
+            const double cref = (double)c[ax];
+            // |offsets| up to about e / 2^(k+1) <= 2^15, well inside binary16 (k capped at 0: a
+            // wider node rounds its outer offsets to the largest finite value or to infinity,
+            // which only loosens it)
+            int k = e > 0.0 ? (int)std::ceil(std::log2(e)) - 16 : 0;
+            k = std::min(0, std::max(-100, k));
+            const double scale = std::ldexp(1.0, -k), margin = 16.0 * std::ldexp(e, -24);
+            exps |= (uint32_t)(uint8_t)(int8_t)k << (8 * ax);
+            for (int i = 0; i < 4; ++i) {
+                const float l = lo[ax][i], u = hi[ax][i];
+                uint16_t ql, qu;
+                if (std::isnan(l)) ql = kNegInf;
+                else if (std::isinf(l)) ql = l > 0 ? kPosInf : kNegInf;
+                else ql = f16_directed(((double)l - cref - margin) * scale, false);
+                if (std::isnan(u)) qu = kPosInf;
+                else if (std::isinf(u)) qu = u > 0 ? kPosInf : kNegInf;
+                else qu = f16_directed(((double)u - cref + margin) * scale, true);
+                dst[ax][0][i] = ql;
+                dst[ax][0][4 + i] = qu;
+                dst[ax][1][i] = qu;
+                dst[ax][1][4 + i] = ql;
+            }
+        }
+        h.cx = c[0];
+        h.cy = c[1];
+        h.cz = c[2];
+        h.exps = exps;
+        for (int i = 0; i < 4; ++i) h.child[i] = a.child[i];
+        out[n] = h;
+    }
+    return out;
+}
 
 Bvh4 build_bvh4(const std::vector<BuildPrim>& prims, const BvhBuildOptions& opt_in) {
     Bvh4 out;

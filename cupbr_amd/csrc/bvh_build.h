@@ -50,4 +50,11 @@ struct BvhBuildOptions {
 
 Bvh4 build_bvh4(const std::vector<BuildPrim>& prims, const BvhBuildOptions& opt);
 
+// The NodeH form of every node. A decoded plane c + h 2^k lies at least 16 x 2^-24 x E outside the
+// FP32 plane it stands for (E: the node's extent on that axis), which covers the rounding of the
+// traversal's decode t = fma(h, ix 2^k, (c - o) ix): its error is at most 2^-23 (E |ix|) plus a
+// few ulp of t, and the slab test's relative slack takes the latter. NaN bounds become the
+// infinities that never cull; infinite ones stay infinite.
+std::vector<NodeH> half_nodes(const std::vector<Node4>& nodes);
+
 }  // namespace cupbr

@@ -264,6 +264,16 @@ int cupbr_scene_get_bvh_stats(const cupbr_scene* s, cupbr_bvh_stats* out) {
     return CUPBR_OK;
 }
 
+int cupbr_scene_get_bvh_nodes(const cupbr_scene* s, void* fp32_nodes, void* half_nodes, uint32_t n) {
+    CHECK_ARG(s, "null argument");
+    CHECK_ARG(n >= s->h_nodes.size(), "buffer too small");
+    if (fp32_nodes && !s->h_nodes.empty())
+        std::memcpy(fp32_nodes, s->h_nodes.data(), s->h_nodes.size() * sizeof(cupbr::Node4));
+    if (half_nodes && !s->h_hnodes.empty())
+        std::memcpy(half_nodes, s->h_hnodes.data(), s->h_hnodes.size() * sizeof(cupbr::NodeH));
+    return CUPBR_OK;
+}
+
 int cupbr_scene_get_reference_order(const cupbr_scene* s, int32_t* order, uint32_t n) {
     CHECK_ARG(s && (order || s->ref_order.empty()), "null argument");
     CHECK_ARG(n >= s->ref_order.size(), "buffer too small");

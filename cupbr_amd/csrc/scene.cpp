@@ -116,7 +116,8 @@ int scene_upload(const cupbr_scene* cs) {
     if (hipGetDevice(&s->device) != hipSuccess) return set_error(CUPBR_ERR_HIP, "no HIP device available");
     DevScene& d = s->dev;
     int rc;
-    if ((rc = upload(s, s->h_nodes, &d.nodes)) || (rc = upload(s, s->h_prims, &d.prims)) ||
+    if ((rc = upload(s, s->h_nodes, &d.nodes)) || (rc = upload(s, s->h_hnodes, &d.hnodes)) ||
+        (rc = upload(s, s->h_prims, &d.prims)) ||
         (rc = upload(s, s->h_normals, &d.normals)) || (rc = upload(s, s->h_unb, &d.unb)) ||
         (rc = upload(s, s->h_info, &d.objs)) || (rc = upload(s, s->h_mats, &d.mats)) ||
         (rc = upload(s, s->lights, &d.lights)) || (rc = upload(s, s->env, &d.env)) ||
@@ -274,6 +275,7 @@ int scene_finalize(cupbr_scene* s) {
     for (uint32_t i = 0; i < n_obj; ++i) mats[i] = s->objects[i].material;
     scene_release_device(s);
     s->h_nodes = std::move(bvh.nodes);
+    s->h_hnodes = half_nodes(s->h_nodes);
     s->h_prims = std::move(prims);
     s->h_normals = std::move(normals);
     s->h_unb = std::move(unb);

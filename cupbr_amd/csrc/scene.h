@@ -26,6 +26,7 @@ struct cupbr_scene {
     bool uploaded = false;
     // host copies of the device arrays (built by scene_finalize, uploaded by scene_upload)
     std::vector<cupbr::Node4> h_nodes;
+    std::vector<cupbr::NodeH> h_hnodes;
     std::vector<cupbr::PrimRec> h_prims;
     std::vector<float4> h_normals;
     std::vector<cupbr::UnbRec> h_unb;

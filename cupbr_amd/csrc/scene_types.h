@@ -40,6 +40,23 @@ struct alignas(128) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
 
+// The same node for per-ray traversal: each child plane as a binary16 offset from a reference
+// point c of the node (the middle of its children's finite extent, as a float), in units of 2^k
+// per axis, rounded outward with a margin; empty slots and unbounded sides are binary16
+// infinities. Per axis two
+// 16-byte blocks, one per sign of the ray's inverse direction, each holding that sign's near
+// planes of the 4 children then their far planes, so a lane loads its axis in one 16-byte load:
+// five loads per node (corner, x, y, z, child words) instead of seven.
+struct alignas(128) NodeH {
+    float cx, cy, cz;     // reference point c
+    uint32_t exps;        // k_x, k_y, k_z as signed bytes 0..2 (k <= 0)
+    uint16_t x[2][8];     // [0]: lo0..lo3 hi0..hi3 (inverse direction >= 0), [1]: hi0..hi3 lo0..lo3
+    uint16_t y[2][8];
+    uint16_t z[2][8];
+    uint32_t child[4];
+};
+static_assert(sizeof(NodeH) == 128, "NodeH must be one 128-byte line");
+
 enum PrimKind : uint32_t { kPrimTriangle = 0, kPrimSphere = 1 };
 constexpr uint32_t kPrimNeedsBoxCheck = 4u;  // object box is finite (sphere, mesh): test it lazily
 // normals[prim].w (bits): object id in bits 0..30, bit 31 set for a sphere (whose normal is
@@ -74,7 +91,8 @@ struct ObjInfo {
 
 // Everything a kernel needs, passed by value.
 struct DevScene {
-    const Node4* nodes;
+    const Node4* nodes;    // wave-uniform (packet) traversal
+    const NodeH* hnodes;   // per-lane traversal
     const PrimRec* prims;
     const float4* normals;  // per prim in leaf order: triangle normal; w = object id | kNormalSphereBit
     const UnbRec* unb;

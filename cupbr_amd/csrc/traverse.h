@@ -18,6 +18,8 @@
 // distinct banks), continued in a global spill area for deep trees.
 #pragma once
 
+#include <hip/hip_fp16.h>
+
 #include "scene_types.h"
 
 namespace cupbr {
@@ -176,7 +178,12 @@ __device__ __forceinline__ bool visit_leaf(const DevScene& S, uint32_t word, v3 
 
 // One inner node `cur`: the children whose boxes overlap [tlo, lim] (conservatively) are pushed
 // farthest first, and cur becomes the nearest (closest hit) or a hit one (any hit); false when
-// no child overlaps. An empty child slot has the box lo = +inf, hi = -inf, which no ray overlaps.
+// no child overlaps. The node is read in its NodeH form: a lane loads its reference point, the 16-byte block
+// of each axis for the sign of its inverse direction (near planes of the 4 children, then far
+// planes, as binary16 offsets) and the child words, and takes each plane's distance as one
+// mixed-precision FMA, t = h (i 2^k) + (c - o) i. The decoded planes lie outside the FP32 ones by
+// a margin that covers the decode's rounding (half_nodes), so the test stays conservative. An
+// empty child slot has the box lo = +inf, hi = -inf, which no ray overlaps.
 // Children are ordered by integer keys, the entry distance's float bits read as a signed integer
 // (non-negative distances in their order, the slightly negative ones of boxes around the origin
 // ahead of them), INT_MAX for a culled child; each compare-exchange of the network is a min, a max
@@ -185,33 +192,38 @@ __device__ __forceinline__ bool visit_leaf(const DevScene& S, uint32_t word, v3 
 template <bool kAny, int kLds>
 __device__ __forceinline__ bool visit_inner(const DevScene& S, uint32_t& cur, v3 o, float ix, float iy, float iz,
                                             float lim, LaneStack<kLds>& st, int& sp) {
-    const bool sx = ix >= 0.0f, sy = iy >= 0.0f, sz = iz >= 0.0f;
     const float tlo = -1e-5f;
     constexpr int kCulled = 0x7FFFFFFF;
-    // each lane loads its near and far planes directly: byte offsets lo (0, 16, 32) or hi (48,
-    // 64, 80) chosen by the sign of the inverse direction
-    const char* base = reinterpret_cast<const char*>(S.nodes);
+    const char* base = reinterpret_cast<const char*>(S.hnodes);
     const uint32_t nb = cur * 128u;  // 32-bit byte offsets: scene_finalize keeps the nodes below 2^25
-    const uint32_t ox = sx ? 0u : 48u, oy = sy ? 16u : 64u, oz = sz ? 32u : 80u;
-    const float4 nx = *reinterpret_cast<const float4*>(base + (nb + ox));
-    const float4 fx = *reinterpret_cast<const float4*>(base + (nb + (48u - ox)));
-    const float4 ny = *reinterpret_cast<const float4*>(base + (nb + oy));
-    const float4 fy = *reinterpret_cast<const float4*>(base + (nb + (80u - oy)));
-    const float4 nz = *reinterpret_cast<const float4*>(base + (nb + oz));
-    const float4 fz = *reinterpret_cast<const float4*>(base + (nb + (112u - oz)));
-    const uint4 ch = *reinterpret_cast<const uint4*>(base + (nb + 96u));
+    const float4 co = *reinterpret_cast<const float4*>(base + nb);
+    const uint4 px = *reinterpret_cast<const uint4*>(base + (nb + (ix >= 0.0f ? 16u : 32u)));
+    const uint4 py = *reinterpret_cast<const uint4*>(base + (nb + (iy >= 0.0f ? 48u : 64u)));
+    const uint4 pz = *reinterpret_cast<const uint4*>(base + (nb + (iz >= 0.0f ? 80u : 96u)));
+    const uint4 ch = *reinterpret_cast<const uint4*>(base + (nb + 112u));
     int key[4];
     uint32_t val[4] = {ch.x, ch.y, ch.z, ch.w};
-#define CUPBR_CHILD(i, c)                                                                          \
+    // per axis: A = i 2^k (exact: k <= 0, and |i| >= 1 for a unit direction), B = (c - o) i
+    const uint32_t ex = __float_as_uint(co.w);
+    const float ax = __builtin_ldexpf(ix, (int)(int8_t)(ex & 0xFFu));
+    const float ay = __builtin_ldexpf(iy, (int)(int8_t)((ex >> 8) & 0xFFu));
+    const float az = __builtin_ldexpf(iz, (int)(int8_t)((ex >> 16) & 0xFFu));
+    const float bx = (co.x - o.x) * ix, by = (co.y - o.y) * iy, bz = (co.z - o.z) * iz;
+    auto lo16 = [](uint32_t w) { return __low2float(*reinterpret_cast<const __half2*>(&w)); };
+    auto hi16 = [](uint32_t w) { return __high2float(*reinterpret_cast<const __half2*>(&w)); };
+    // child i's near plane: half i of words 0..1 of its axis block; far plane: half i of words 2..3
+#define CUPBR_CHILD(i, half, wn, wf)                                                               \
     {                                                                                              \
-        float tn = fmaxf(fmaxf((nx.c - o.x) * ix, (ny.c - o.y) * iy), fmaxf((nz.c - o.z) * iz, tlo)); \
-        float tf = fminf(fminf((fx.c - o.x) * ix, (fy.c - o.y) * iy), fminf((fz.c - o.z) * iz, lim)); \
+        const float tn = fmaxf(fmaxf(fmaf(half(px.wn), ax, bx), fmaf(half(py.wn), ay, by)),        \
+                               fmaxf(fmaf(half(pz.wn), az, bz), tlo));                              \
+        const float tf = fminf(fminf(fmaf(half(px.wf), ax, bx), fmaf(half(py.wf), ay, by)),        \
+                               fminf(fmaf(half(pz.wf), az, bz), lim));                              \
         key[i] = tn <= tf * 1.000002f ? __float_as_int(tn) : kCulled;                              \
     }
-    CUPBR_CHILD(0, x)
-    CUPBR_CHILD(1, y)
-    CUPBR_CHILD(2, z)
-    CUPBR_CHILD(3, w)
+    CUPBR_CHILD(0, lo16, x, z)
+    CUPBR_CHILD(1, hi16, x, z)
+    CUPBR_CHILD(2, lo16, y, w)
+    CUPBR_CHILD(3, hi16, y, w)
 #undef CUPBR_CHILD
     // The child words are only needed once a child overlaps; without this use the compiler moves
     // their load behind that test, and the step then waits for a second memory round trip.

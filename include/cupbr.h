@@ -257,6 +257,14 @@ int cupbr_scene_get_volume(const cupbr_scene* scene, cupbr_volume* out);
 int cupbr_scene_get_environment(const cupbr_scene* scene, float* out, size_t n_floats, uint32_t* width,
                                 uint32_t* height);
 int cupbr_scene_get_bvh_stats(const cupbr_scene* scene, cupbr_bvh_stats* out);
+/* No counterpart in the reference (inspection of this build's BVH, for tests and tools): copies
+ * the BVH4 nodes in both device forms, 128 bytes per node: the FP32 node used by wave-uniform
+ * traversal (child boxes as lo x, y, z then hi x, y, z, each a float[4] over the children, then
+ * the 4 child words) and the form used by per-ray traversal (reference point c as 3 floats, a word of
+ * three signed-byte exponents k, per axis two blocks of 8 binary16 plane offsets, lo0..3 hi0..3
+ * then hi0..3 lo0..3, a plane being c + offset * 2^k, then the 4 child words). Either pointer may
+ * be NULL; n must be at least cupbr_bvh_stats.nodes. */
+int cupbr_scene_get_bvh_nodes(const cupbr_scene* scene, void* fp32_nodes, void* half_nodes, uint32_t n);
 /* reference BVH leaf order of the objects (BoundingVolumeHierarchy.cpp:46-136); n >= n_objects */
 int cupbr_scene_get_reference_order(const cupbr_scene* scene, int32_t* order, uint32_t n);
 
