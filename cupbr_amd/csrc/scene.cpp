@@ -116,7 +116,7 @@ int scene_upload(const cupbr_scene* cs) {
     if (hipGetDevice(&s->device) != hipSuccess) return set_error(CUPBR_ERR_HIP, "no HIP device available");
     DevScene& d = s->dev;
     int rc;
-    if ((rc = upload(s, s->h_nodes, &d.nodes)) || (rc = upload(s, s->h_hnodes, &d.hnodes)) ||
+    if ((rc = upload(s, s->h_hnodes, &d.hnodes)) ||
         (rc = upload(s, s->h_prims, &d.prims)) ||
         (rc = upload(s, s->h_normals, &d.normals)) || (rc = upload(s, s->h_unb, &d.unb)) ||
         (rc = upload(s, s->h_info, &d.objs)) || (rc = upload(s, s->h_mats, &d.mats)) ||

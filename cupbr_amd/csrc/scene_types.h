@@ -1,8 +1,9 @@
 // Device-resident scene layout (HBM), shared by every kernel.
 //
-//   Node4    128-byte BVH4 node: per-axis child boxes as float4 SoA (96 B) +
-//            4 child words (16 B) + 16 B spare, 128-byte aligned so one
-//            node is one HBM/L2 line.
+//   Node4    128-byte BVH4 node as built on the host: per-axis child boxes as
+//            float4 SoA (96 B) + 4 child words (16 B) + 16 B spare.
+//   NodeH    the same node as uploaded and traversed: binary16 plane offsets
+//            (below), 128-byte aligned so one node is one HBM/L2 line.
 //   PrimRec  48-byte primitive record in BVH leaf order:
 //              triangle  v1.xyz | e1 = v2-v1 | e2 = v3-v1  (precomputed edges give the
 //                        same FP32 values the reference computes per test,
@@ -40,7 +41,7 @@ struct alignas(128) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
 
-// The same node for per-ray traversal: each child plane as a binary16 offset from a reference
+// The same node as the device reads it: each child plane as a binary16 offset from a reference
 // point c of the node (the middle of its children's finite extent, as a float), in units of 2^k
 // per axis, rounded outward with a margin; empty slots and unbounded sides are binary16
 // infinities. Per axis two
@@ -91,8 +92,7 @@ struct ObjInfo {
 
 // Everything a kernel needs, passed by value.
 struct DevScene {
-    const Node4* nodes;    // wave-uniform (packet) traversal
-    const NodeH* hnodes;   // per-lane traversal
+    const NodeH* hnodes;   // the BVH4 (the FP32 Node4 form stays on the host)
     const PrimRec* prims;
     const float4* normals;  // per prim in leaf order: triangle normal; w = object id | kNormalSphereBit
     const UnbRec* unb;

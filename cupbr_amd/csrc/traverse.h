@@ -432,10 +432,11 @@ __device__ __forceinline__ void traverse_packet_impl(const DevScene& S, v3 o, v3
     const bool sx = kUniformSigns ? ux : ix >= 0.0f, sy = kUniformSigns ? uy : iy >= 0.0f,
                sz = kUniformSigns ? uz : iz >= 0.0f;
     const float tlo = -1e-5f;
-    // float4 offsets of the near and far planes in a node (lo x, y, z at 0, 1, 2; hi at 3, 4, 5)
-    const uint32_t onx = __builtin_amdgcn_readfirstlane(sx ? 0 : 3), ofx = 3 - onx;
-    const uint32_t ony = __builtin_amdgcn_readfirstlane(sy ? 1 : 4), ofy = 5 - ony;
-    const uint32_t onz = __builtin_amdgcn_readfirstlane(sz ? 2 : 5), ofz = 7 - onz;
+    // byte offsets of the NodeH plane blocks for the wave's signs (near planes of the 4 children,
+    // then far planes); the mixed-sign instantiation loads both blocks of an axis and selects
+    const uint32_t obx = __builtin_amdgcn_readfirstlane(sx ? 16 : 32);
+    const uint32_t oby = __builtin_amdgcn_readfirstlane(sy ? 48 : 64);
+    const uint32_t obz = __builtin_amdgcn_readfirstlane(sz ? 80 : 96);
     const uint32_t lane = __lane_id();
     uint32_t stk0 = 0, stk1 = 0;  // the wave's stack
     uint32_t sp = 0, cur = 0;     // wave-uniform
@@ -444,24 +445,41 @@ __device__ __forceinline__ void traverse_packet_impl(const DevScene& S, v3 o, v3
             const uint32_t first = leaf_first(cur), n_leaf = leaf_count(cur);
             for (uint32_t k = 0; k < n_leaf; ++k) closest_prim(S, uniform_prim(S.prims + first + k), first + k, o, d, best);
         } else {
-            const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
-            const float4 cw = uniform_load4(nf + 6);
+            // the node in its NodeH form, as in visit_inner: plane t = h (i 2^k) + (c - o) i
+            const char* hb = reinterpret_cast<const char*>(S.hnodes) + (size_t)cur * 128u;
+            const float4 co = uniform_load4(hb);
+            const float4 cw = uniform_load4(hb + 112);
             const uint32_t child[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
                                        __float_as_uint(cw.w)};
-            float4 NX, NY, NZ, FX, FY, FZ;  // per child: near and far plane of each axis
-            if (kUniformSigns) {  // the wave loads its near and far planes directly
-                NX = uniform_load4(nf + onx), NY = uniform_load4(nf + ony), NZ = uniform_load4(nf + onz);
-                FX = uniform_load4(nf + ofx), FY = uniform_load4(nf + ofy), FZ = uniform_load4(nf + ofz);
+            float4 BX, BY, BZ;  // near planes of the 4 children in words 0..1, far planes in 2..3
+            // (below, nx .. fz are the distances to those planes)
+            if (kUniformSigns) {
+                BX = uniform_load4(hb + obx), BY = uniform_load4(hb + oby), BZ = uniform_load4(hb + obz);
             } else {
-                const float4 lx = uniform_load4(nf), ly = uniform_load4(nf + 1), lz = uniform_load4(nf + 2);
-                const float4 hx = uniform_load4(nf + 3), hy = uniform_load4(nf + 4), hz = uniform_load4(nf + 5);
-                NX = sx ? lx : hx, FX = sx ? hx : lx;
-                NY = sy ? ly : hy, FY = sy ? hy : ly;
-                NZ = sz ? lz : hz, FZ = sz ? hz : lz;
+                const float4 x0 = uniform_load4(hb + 16), x1 = uniform_load4(hb + 32);
+                const float4 y0 = uniform_load4(hb + 48), y1 = uniform_load4(hb + 64);
+                const float4 z0 = uniform_load4(hb + 80), z1 = uniform_load4(hb + 96);
+                BX = sx ? x0 : x1, BY = sy ? y0 : y1, BZ = sz ? z0 : z1;
             }
-            const float nx[4] = {NX.x, NX.y, NX.z, NX.w}, ny[4] = {NY.x, NY.y, NY.z, NY.w};
-            const float nz[4] = {NZ.x, NZ.y, NZ.z, NZ.w}, fx[4] = {FX.x, FX.y, FX.z, FX.w};
-            const float fy[4] = {FY.x, FY.y, FY.z, FY.w}, fz[4] = {FZ.x, FZ.y, FZ.z, FZ.w};
+            const uint32_t ex = __float_as_uint(co.w);
+            const float ax = __builtin_ldexpf(ix, (int)(int8_t)(ex & 0xFFu));
+            const float ay = __builtin_ldexpf(iy, (int)(int8_t)((ex >> 8) & 0xFFu));
+            const float az = __builtin_ldexpf(iz, (int)(int8_t)((ex >> 16) & 0xFFu));
+            const float bx = (co.x - o.x) * ix, by = (co.y - o.y) * iy, bz = (co.z - o.z) * iz;
+            auto lo16 = [](float w) { const uint32_t u = __float_as_uint(w); return __low2float(*reinterpret_cast<const __half2*>(&u)); };
+            auto hi16 = [](float w) { const uint32_t u = __float_as_uint(w); return __high2float(*reinterpret_cast<const __half2*>(&u)); };
+            const float nx[4] = {fmaf(lo16(BX.x), ax, bx), fmaf(hi16(BX.x), ax, bx), fmaf(lo16(BX.y), ax, bx),
+                                 fmaf(hi16(BX.y), ax, bx)};
+            const float fx[4] = {fmaf(lo16(BX.z), ax, bx), fmaf(hi16(BX.z), ax, bx), fmaf(lo16(BX.w), ax, bx),
+                                 fmaf(hi16(BX.w), ax, bx)};
+            const float ny[4] = {fmaf(lo16(BY.x), ay, by), fmaf(hi16(BY.x), ay, by), fmaf(lo16(BY.y), ay, by),
+                                 fmaf(hi16(BY.y), ay, by)};
+            const float fy[4] = {fmaf(lo16(BY.z), ay, by), fmaf(hi16(BY.z), ay, by), fmaf(lo16(BY.w), ay, by),
+                                 fmaf(hi16(BY.w), ay, by)};
+            const float nz[4] = {fmaf(lo16(BZ.x), az, bz), fmaf(hi16(BZ.x), az, bz), fmaf(lo16(BZ.y), az, bz),
+                                 fmaf(hi16(BZ.y), az, bz)};
+            const float fz[4] = {fmaf(lo16(BZ.z), az, bz), fmaf(hi16(BZ.z), az, bz), fmaf(lo16(BZ.w), az, bz),
+                                 fmaf(hi16(BZ.w), az, bz)};
             const float lim = best.t * 1.0001f + 1e-6f;  // < tlo for an invalid lane: nothing overlaps
             // ordering key of a child: the entry distance of the first lane that overlaps it, as
             // float bits (negative distances as 0, so the keys order as unsigned integers on the
@@ -469,8 +487,8 @@ __device__ __forceinline__ void traverse_packet_impl(const DevScene& S, v3 o, v3
             uint32_t key[4], val[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const float tn = fmaxf(fmaxf((nx[c] - o.x) * ix, (ny[c] - o.y) * iy), fmaxf((nz[c] - o.z) * iz, tlo));
-                const float tf = fminf(fminf((fx[c] - o.x) * ix, (fy[c] - o.y) * iy), fminf((fz[c] - o.z) * iz, lim));
+                const float tn = fmaxf(fmaxf(nx[c], ny[c]), fmaxf(nz[c], tlo));
+                const float tf = fminf(fminf(fx[c], fy[c]), fminf(fz[c], lim));
                 const unsigned long long m = child[c] != kChildEmpty ? __ballot(tn <= tf * 1.000002f) : 0ull;
                 uint32_t k = 0xFFFFFFFFu;
                 if (m) {
