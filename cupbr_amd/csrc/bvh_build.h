@@ -35,11 +35,12 @@ struct Bvh4 {
 #ifndef CUPBR_BVH_MAX_LEAF
 #define CUPBR_BVH_MAX_LEAF 8
 #endif
-// SAH cost of a BVH4 node visit (its 4 box tests) relative to one primitive test. 1.5 measured
-// best for PT+NEE at 1080p on 1M triangles (Msamples/s: 0.7 -> 627, 1.0 -> 633, 1.5 -> 637,
-// 2.0 -> 631, 3.0 -> 610; the earlier greedy collapse of a binary tree with SAH leaves: 605).
+// SAH cost of a BVH4 node visit (its 4 box tests) relative to one primitive test. 2.0 measured
+// best for PT+NEE at 1080p on 1M triangles with the binary16 node step (Msamples/s: 1.0 -> 1023,
+// 1.25 -> 1015, 1.5 -> 1016, 1.75 -> 1021, 2.0 -> 1026-1028, 2.5 -> 1015, 3.0 -> 999); with the
+// earlier FP32 node step 1.5 was best (0.7 -> 627, 1.0 -> 633, 1.5 -> 637, 2.0 -> 631, 3.0 -> 610).
 #ifndef CUPBR_BVH_TRAVERSAL_COST
-#define CUPBR_BVH_TRAVERSAL_COST 1.5f
+#define CUPBR_BVH_TRAVERSAL_COST 2.0f
 #endif
 struct BvhBuildOptions {
     uint32_t max_leaf = CUPBR_BVH_MAX_LEAF;  // <= kMaxLeafPrims
