@@ -77,6 +77,9 @@ SCENES = {
     "soup": (lambda: scenes.random_triangle_soup(3000, seed=1), -12, 12),
     "meshes": (lambda: scenes.random_mesh_scene(40, 300, seed=2), -12, 12),
     "degenerate": (_degenerate_scene, -2.0, 2.0),
+    # nodes far wider than 2^16: the traversal's binary16 plane offsets overflow at the top of the
+    # tree (clamped outward), and small ones far from the origin
+    "large_coordinates": (lambda: scenes.random_triangle_soup(3000, seed=5, extent=5e4, size=3e3), -6e4, 6e4),
 }
 
 
