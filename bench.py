@@ -6,8 +6,8 @@ of the 1,021,120-triangle scene (cupbr_amd.scenes.million_triangles), followed
 by bloom (threshold 1, knee 1) and the Reinhard tone map, all on the GPU with
 the scene resident in HBM. Prints one JSON line:
 
-  value        Msamples/s = 1920*1080*64*steps / timed seconds (samples = camera paths)
-  value        Msamples/s = 1920*1080*64 / median step time (ms_per_step is the mean)
+  value        Msamples/s = 1920*1080*64 / median step time (samples = camera paths;
+               ms_per_step is the mean)
   roofline     the trace kernels (closest hit for extension rays + any hit for shadow rays,
                one launch per bounce), the dominant stage, against the VALU issue ceiling
                they saturate: achieved = VALU instructions of a step's trace launches x 64
@@ -18,7 +18,10 @@ the scene resident in HBM. Prints one JSON line:
                traffic = HBM bytes per trace launch (FETCH_SIZE + WRITE_SIZE, two more child
                runs, scaled by what the counters read for a copy of known size); the
                algorithmic bytes (nodes x 128 B + triangles x 48 B + queue records, mostly
-               served by L2 and the MALL) are reported beside them
+               served by L2 and the MALL) are reported beside them, and so are the vector
+               memory units' busy fractions (TA_TA_BUSY, TD_TD_BUSY), the L1 access rate
+               (TCP_TOTAL_CACHE_ACCESSES) and the L1 and L2 hit rates (TCP_TCC_READ_REQ,
+               TCC_HIT, TCC_MISS; a fourth child run)
   cpu_baseline the CPU oracle (the reference estimator restated in C++, SAH BVH, FP32)
                timed on this host on a bounded sample of the same workload, on this
                process's CPU share (affinity mask, bounded by a cgroup quota), with the CPU
@@ -62,8 +65,8 @@ def main():
                          "gdpt: gradient-domain frame (5 paths per sample + 50 reconstruction iterations) of the "
                          "Cornell box at 512x512, 128 spp; volume: the homogeneous-media renderer at 512x512, 64 spp")
     ap.add_argument("--no-counters", action="store_true",
-                    help="skip the three rocprofv3 child runs that measure VALU and HBM counters (pt: of the "
-                         "trace kernels; cornell, gdpt, volume: of all kernels of a render call)")
+                    help="skip the four rocprofv3 child runs that measure VALU, cache and HBM counters (pt: of "
+                         "the trace kernels; cornell, gdpt, volume: of all kernels of a render call)")
     ap.add_argument("--counter-dir", default=None, help="directory for the rocprofv3 child runs' output")
     ap.add_argument("--counters-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -161,6 +164,9 @@ def cpu_baseline_record(value, cpu, sample):
 # --------------------------------------------------------------------------- counters (rocprofv3 children)
 VALU_COUNTERS = ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVES", "GRBM_GUI_ACTIVE",
                  "TA_TA_BUSY_sum", "TD_TD_BUSY_sum", "TCP_TOTAL_CACHE_ACCESSES_sum"]
+# cache hit rates: L2 (TCC) hits and misses, and the L1's (TCP) requests to L2 against its accesses
+CACHE_COUNTERS = ["TCC_HIT_sum", "TCC_MISS_sum", "TCP_TCC_READ_REQ_sum", "TCP_TOTAL_CACHE_ACCESSES_sum",
+                  "GRBM_GUI_ACTIVE"]
 # rocprofv3 reports GRBM_GUI_ACTIVE summed over the MI355X's 8 XCDs (a dispatch of t seconds reads
 # about 8 x 2.4e9 x t), so a dispatch lasts GRBM_GUI_ACTIVE / 8 cycles of each CU's clock
 N_XCD = 8
@@ -245,12 +251,21 @@ def is_render_kernel(k):
 
 
 def measure_counters(args, log, is_trace=is_trace_kernel, n_cu=256):
-    """VALU and HBM counters of the dispatches whose kernel name satisfies is_trace (the trace
-    kernels by default) from three rocprofv3 runs of one step each."""
+    """VALU, cache and HBM counters of the dispatches whose kernel name satisfies is_trace (the
+    trace kernels by default) from four rocprofv3 runs of one step each."""
     res = {}
     valu = _run_counter_child(args, VALU_COUNTERS, log)
+    cache = _run_counter_child(args, CACHE_COUNTERS, log)
     fetch = _run_counter_child(args, ["FETCH_SIZE"], log, copy=True)
     write = _run_counter_child(args, ["WRITE_SIZE"], log, copy=True)
+    if cache:
+        tc = [d for d in cache if is_trace(d["kernel"])]
+        hit, miss = sum(d.get("TCC_HIT_sum", 0.0) for d in tc), sum(d.get("TCC_MISS_sum", 0.0) for d in tc)
+        acc = sum(d.get("TCP_TOTAL_CACHE_ACCESSES_sum", 0.0) for d in tc)
+        if hit + miss > 0:
+            res["l2_hit_rate"] = hit / (hit + miss)
+        if acc > 0:
+            res["l1_hit_rate"] = 1.0 - sum(d.get("TCP_TCC_READ_REQ_sum", 0.0) for d in tc) / acc
     if valu:
         tr = [d for d in valu if is_trace(d["kernel"])]
         mk = [d for d in valu if "k_valu_peak" in d["kernel"]]
@@ -479,6 +494,10 @@ def bench_pt(args, log):
             log(f"trace vector memory: address unit (TA) busy {c['ta_busy']:.3f}, data unit (TD) busy "
                 f"{c['td_busy']:.3f}, L1 accesses {c['l1_access_rate']:.3f} per CU per cycle, of the trace "
                 f"dispatches' cycles under the profiler")
+        if c.get("l2_hit_rate") is not None:
+            roof.update({"l2_hit_rate": round(c["l2_hit_rate"], 4), "l1_hit_rate": round(c.get("l1_hit_rate", 0.0), 4)})
+            log(f"trace caches: L2 hit rate {c['l2_hit_rate']:.3f} (TCC_HIT / (TCC_HIT + TCC_MISS)), L1 hit rate "
+                f"{c.get('l1_hit_rate', 0.0):.3f} (1 - TCP_TCC_READ_REQ / TCP_TOTAL_CACHE_ACCESSES)")
         if c.get("fetch_bytes_per_launch") is not None and c.get("write_bytes_per_launch") is not None:
             traffic = c["fetch_bytes_per_launch"] + c["write_bytes_per_launch"]
             hbm_gbs = traffic / (trace_launch_ms * 1e-3) / 1e9
@@ -969,6 +988,9 @@ def bench_integrator(args, log):
                          "l1_accesses_per_cu_cycle": round(c["l1_access_rate"], 4)})
             log(f"{args.workload} vector memory: address unit (TA) busy {c['ta_busy']:.3f}, data unit (TD) busy "
                 f"{c['td_busy']:.3f}, L1 accesses {c['l1_access_rate']:.3f} per CU per cycle")
+        if c.get("l2_hit_rate") is not None:
+            roof.update({"l2_hit_rate": round(c["l2_hit_rate"], 4), "l1_hit_rate": round(c.get("l1_hit_rate", 0.0), 4)})
+            log(f"{args.workload} caches: L2 hit rate {c['l2_hit_rate']:.3f}, L1 hit rate {c.get('l1_hit_rate', 0.0):.3f}")
         if c.get("fetch_bytes_total") is not None and c.get("write_bytes_total") is not None:
             traffic = c["fetch_bytes_total"] + c["write_bytes_total"]
             hbm_gbs = traffic / t_dev / 1e9
