@@ -52,6 +52,9 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-nrc", action="store_true",
+                    help="pt workload: skip the neural-radiance-cache half of the metric (the equal-time rel-MSE "
+                         "ratio of --workload nrc, reported in the same JSON line under neural_cache)")
     ap.add_argument("--nrc-spp", type=int, default=4, help="samples per render call of the radiance-cache workload")
     ap.add_argument("--nrc-camera-roughness", default="0.1,inf",
                     help="comma list of camera-hit roughness thresholds of the radiance cache (inf: the camera hit "
@@ -102,7 +105,8 @@ def main():
     if args.workload == "grid":
         return bench_grid(args, log)
     if args.workload == "nrc":
-        return bench_nrc(args, log)
+        print(json.dumps(bench_nrc(args, log)), flush=True)
+        return None
     if args.workload in ("gdpt", "volume", "cornell"):
         return bench_integrator(args, log)
     return bench_pt(args, log)
@@ -555,6 +559,20 @@ def bench_pt(args, log):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if not args.no_nrc:
+        # the metric's second half, on the same scene in the same run (after the Msamples/s line's own
+        # buffers are released): the neural radiance cache's equal-time rel-MSE against plain PT
+        del scene
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        n = bench_nrc(args, log)
+        cfg = n["config"]
+        out["neural_cache"] = {
+            "metric": n["metric"], "value": n["value"], "unit": n["unit"], "higher_is_better": True,
+            "workload": cfg["workload"], "cache_vertices": cfg["cache_vertices"],
+            "rel_mse_cache": cfg["rel_mse_cache"], "cache_call_ms": cfg["cache_call_ms"],
+            "pt_equal_time": cfg["pt_equal_time"], "second_vertex_cache": cfg["second_vertex_cache"],
+            "roofline": n["roofline"]}
     print(json.dumps(out), flush=True)
 
 
@@ -876,7 +894,7 @@ def bench_nrc(args, log):
                      "avg_launch_ms": round(inf_ms, 4), "queries_per_launch": int(x.shape[0])},
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def integrator_setup(args):
