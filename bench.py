@@ -3,7 +3,7 @@
 
 One step = one 64-spp frame (64 reference frames of the running mean, depth 8)
 of the 1,021,120-triangle scene (cupbr_amd.scenes.million_triangles), followed
-by bloom (threshold 1, knee 1) and the Reinhard tone map, all on the GPU with
+by bloom (threshold 1, knee 1; the 5-level Gaussian pyramid) and the Reinhard tone map, all on the GPU with
 the scene resident in HBM. Prints one JSON line:
 
   value        Msamples/s = 1920*1080*64 / median step time (samples = camera paths;
@@ -351,6 +351,7 @@ def counters_child(args):
     r.set_max_trace_depth(args.depth)
     post = cb.PostProcessor()
     post.register_renderer(r)
+    post.set_bloom_pyramid(cb.BloomPyramid.GAUSSIAN5)
     tm = cb.ToneMapper(cb.ToneMappingType.REINHARD)
     tm.register_renderer(r)
     r.render(cam, args.spp)
@@ -385,6 +386,7 @@ def bench_pt(args, log):
     r.set_russian_roulette(False)
     post = cb.PostProcessor()
     post.register_renderer(r)
+    post.set_bloom_pyramid(cb.BloomPyramid.GAUSSIAN5)  # the 5-level Gaussian pyramid the workload names
     tm = cb.ToneMapper(cb.ToneMappingType.REINHARD)
     tm.register_renderer(r)
     curve = cb.threshold_curve(1.0, 1.0)
@@ -547,8 +549,8 @@ def bench_pt(args, log):
         "data": "synthetic (seeded procedural scene, cupbr_amd.scenes.million_triangles)",
         "config": {
             "workload": f"PT+NEE {W}x{H}, {SPP} spp, depth {args.depth}, {len(desc['tris'])} triangles "
-                        f"({len(desc['types'])} mesh objects), 4 area lights + 512x256 env map, then bloom + "
-                        f"Reinhard tone map",
+                        f"({len(desc['types'])} mesh objects), 4 area lights + 512x256 env map, then bloom "
+                        f"(5-level Gaussian pyramid) + Reinhard tone map",
             "width": W, "height": H, "spp": SPP, "max_depth": args.depth, "triangles": int(len(desc["tris"])),
             "russian_roulette": False, "rng": "reference TEA<4> seed + LCG",
             "value_from": "median step time", "median_step_ms": round(t_med * 1e3, 3),
