@@ -111,9 +111,10 @@ template <> __device__ __forceinline__ Pcg32 path_seed<Pcg32>(uint32_t pix, uint
     return Pcg32{pcg_hash(pcg_hash(frame) ^ pix)};
 }
 
+// Camera rays of a batch in SampleOrder: the path id of a path is its queue slot here.
 template <class Seed>
 __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0,
-                                                        uint32_t npath, float4* __restrict__ q_o,
+                                                        uint32_t npath, SampleOrder order, float4* __restrict__ q_o,
                                                         float4* __restrict__ q_d, float4* __restrict__ weight,
                                                         float4* __restrict__ rad, Counters* c, uint32_t train_thresh) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -127,13 +128,12 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
         c->iqueue_count[0].v = 0;
         c->iqueue_count[1].v = 0;
     }
-    if (p >= npath) return;  // p: queue slot
-    const uint32_t npix = w * h;
-    const uint32_t s = p / npix;
-    uint32_t px, py;
-    tile_pixel(p - s * npix, w, h, px, py);  // the queue holds 8x8 pixel tiles
+    if (p >= npath) return;  // p: queue slot and path id
+    uint32_t s, j, px, py;
+    order.sample(p, s, j);
+    tile_pixel(j, w, h, px, py);
     const uint32_t pix = py * w + px;
-    const uint32_t pid = s * npix + pix;  // path id: frame-major, pixels in row order
+    const uint32_t pid = p;
     Seed seed = path_seed<Seed>(pix, frame0 + s);
     v3 o, d;
     launch_ray(px, py, w, h, cam, seed, o, d);
@@ -507,7 +507,8 @@ void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o, const float
 }
 
 // Bounce 0 (PathTracer.cu:112-126): the camera rays, traced as packets. Queue slot chunks of 64
-// are 8x8 pixel tiles whose rays leave the camera position in nearby directions, so each wave
+// (SampleOrder: the samples of one pixel, or of a few neighbouring pixels, or an 8x8 pixel tile)
+// hold rays that leave the camera position in nearby directions, so each wave
 // runs traverse_packet over one chunk. Blocks are dispatched to the 8 XCDs round-robin; block b
 // takes chunk range (b % 8) * (gridDim.x / 8) + b / 8, so each XCD traces a contiguous eighth of
 // the queue (neighbouring tiles share its L2). gridDim.x is a multiple of 8; n = queue 0's length.
@@ -535,22 +536,39 @@ void k_pt_trace_camera(DevScene S, uint32_t n, const float4* __restrict__ q_o, c
     if (valid) q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
 }
 
-// running mean of PathTracer.cu:162-168, applied frame by frame in order
+// The running mean of PathTracer.cu:162-168 per pixel, frame by frame in order. A pixel's samples are runs of consecutive
+// slots (SampleOrder: up to f frames); a thread loads up to 8 of them at once, so that the
+// 128-byte line they share is fetched once.
 __global__ __launch_bounds__(kPtBlock) void k_accumulate(float* __restrict__ img, const float4* __restrict__ rad,
-                                                         uint32_t npix, uint32_t frame0, uint32_t nspp) {
+                                                         uint32_t w, uint32_t h, SampleOrder order, uint32_t frame0,
+                                                         uint32_t nspp) {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= npix) return;
+    if (pix >= w * h) return;
+    const uint32_t py = pix / w, j = tile_slot(pix - py * w, py, w, h);
     v3 acc = ld3(img + 3 * (size_t)pix);
-    for (uint32_t s = 0; s < nspp; ++s) {
-        const float4 R = rad[(size_t)s * npix + pix];
-        const v3 L = mk(R.x, R.y, R.z);
-        const uint32_t f = frame0 + s;
-        if (f > 0) {
-            const float a = 1.0f / ((float)f + 1.0f);
-            acc = (1.0f - a) * acc + a * L;
-        } else {
-            acc = L;
+    for (uint32_t s = 0; s < nspp;) {
+        const uint32_t base = order.slot(s, j);
+        const uint32_t run = min(order.f - s % order.f, nspp - s);  // slots base .. base + run - 1
+        for (uint32_t k = 0; k < run; k += 8) {
+            const uint32_t m = min(8u, run - k);
+            float4 R[8];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q)
+                if (q < m) R[q] = rad[base + k + q];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+                if (q >= m) break;
+                const v3 L = mk(R[q].x, R[q].y, R[q].z);
+                const uint32_t f = frame0 + s + k + q;
+                if (f > 0) {
+                    const float a = 1.0f / ((float)f + 1.0f);
+                    acc = (1.0f - a) * acc + a * L;
+                } else {
+                    acc = L;
+                }
+            }
         }
+        s += run;
     }
     st3(img + 3 * (size_t)pix, acc);
 }
@@ -967,6 +985,14 @@ static int ensure_wavefront(cupbr_renderer* r, uint32_t paths, uint32_t grid_thr
     return CUPBR_OK;
 }
 
+// SampleOrder of a batch of nb frames of npix pixels: as many frames per wave as divide nb (a power
+// of two up to 64) while 64 / frames still divides the pixel count
+SampleOrder sample_order(uint32_t npix, uint32_t nb) {
+    uint32_t f = 64;
+    while (f > 1u && (nb % f != 0u || npix % (64u / f) != 0u)) f >>= 1;
+    return SampleOrder{npix, nb, f};
+}
+
 int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame0, uint32_t spp) {
     const uint32_t w = r->width, h = r->height, npix = w * h;
     if (npix == 0 || spp == 0) return CUPBR_OK;
@@ -1030,8 +1056,9 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
         const uint32_t nb = std::min(batch, spp - s0);
         bool side_pending = false;  // cache inference of this batch running on the side stream
         const uint32_t npath = nb * npix;
+        const SampleOrder order = sample_order(npix, nb);
         hipLaunchKernelGGL(raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
-                           h, frame0 + s0, npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
+                           h, frame0 + s0, npath, order, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
                            wf.weight.as<float4>(), wf.radiance.as<float4>(), c, train_thresh);
         if ((rc = stage_mark(r, 3))) return rc;
         // per bounce: trace (this bounce's extension rays + the previous bounce's shadow rays), then
@@ -1089,7 +1116,7 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
             if ((rc = stage_mark(r, 3))) return rc;
         }
         hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, img,
-                           wf.radiance.as<float4>(), npix, frame0 + s0, nb);
+                           wf.radiance.as<float4>(), w, h, order, frame0 + s0, nb);
         if ((rc = stage_mark(r, 3))) return rc;
         if (nrc) {
             uint32_t counts[2] = {0, 0};
@@ -1148,10 +1175,11 @@ int volume_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t 
     for (uint32_t s0 = 0; s0 < spp; s0 += batch) {
         const uint32_t nb = std::min(batch, spp - s0);
         const uint32_t npath = nb * npix;
-        // the path tracer's ray generation: camera rays in 8x8 tiles, seed tea<4>(pixel, frame)
+        // the path tracer's ray generation (camera rays in SampleOrder), seed tea<4>(pixel, frame)
         // (VolumeRenderer.cu:229-234); rad[p].w = 0: outside every object
+        const SampleOrder order = sample_order(npix, nb);
         hipLaunchKernelGGL(k_pt_raygen<uint32_t>, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
-                           h, frame0 + s0, npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
+                           h, frame0 + s0, npath, order, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
                            wf.weight.as<float4>(), wf.radiance.as<float4>(), c, 0u);
         if ((rc = stage_mark(r, 3))) return rc;
         for (uint32_t b = 0; b <= bounces; ++b) {
@@ -1178,7 +1206,7 @@ int volume_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t 
             if ((rc = stage_mark(r, 1))) return rc;
         }
         hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, img,
-                           wf.radiance.as<float4>(), npix, frame0 + s0, nb);
+                           wf.radiance.as<float4>(), w, h, order, frame0 + s0, nb);
         if ((rc = stage_mark(r, 3))) return rc;
     }
     CUPBR_HIP_CHECK(hipGetLastError());

@@ -263,6 +263,39 @@ __device__ __forceinline__ void tile_pixel(uint32_t j, uint32_t w, uint32_t h, u
     }
 }
 
+// Inverse of tile_pixel: the work item of pixel (px, py).
+__device__ __forceinline__ uint32_t tile_slot(uint32_t px, uint32_t py, uint32_t w, uint32_t h) {
+    const uint32_t w8 = w & ~7u, h8 = h & ~7u, n_full = w8 * h8, wr = w - w8;
+    if (px < w8 && py < h8) return (((py >> 3) * (w8 >> 3) + (px >> 3)) << 6) + ((py & 7u) << 3) + (px & 7u);
+    if (py < h8) return n_full + py * wr + (px - w8);
+    return n_full + wr * h8 + (py - h8) * w + px;
+}
+
+// Queue slot of sample s (frame of a batch of nb frames) of the pixel at work item j (tile_pixel
+// order), and its inverse. With f frames per wave (f a power of two dividing nb, 64 / f dividing
+// w * h) a 64-wide wave holds f consecutive frames of 64 / f consecutive work items, and the waves
+// run over the frame groups of a pixel group before the next pixel group: at 64 frames per batch
+// a wave traces the 64 samples of one pixel. With f = 1 the queue is frame-major.
+struct SampleOrder {
+    uint32_t npix, nb, f;
+    __device__ __forceinline__ uint32_t slot(uint32_t s, uint32_t j) const {
+        if (f == 1u) return s * npix + j;
+        const uint32_t pp = 64u / f, g = nb / f;
+        return (((j / pp) * g + s / f) << 6) + (j % pp) * f + s % f;
+    }
+    __device__ __forceinline__ void sample(uint32_t p, uint32_t& s, uint32_t& j) const {
+        if (f == 1u) {
+            s = p / npix;
+            j = p - s * npix;
+            return;
+        }
+        const uint32_t pp = 64u / f, g = nb / f, wv = p >> 6, k = p & 63u;
+        const uint32_t pg = wv / g;
+        s = (wv - pg * g) * f + k % f;
+        j = pg * pp + k / f;
+    }
+};
+
 // coefficients of the current medium of the volume renderer: the scene's (obj < 0) or an
 // object's interior (Volume.h:15-21; VolumeRenderer.cu:88-98)
 struct VolCoef {
