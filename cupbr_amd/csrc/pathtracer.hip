@@ -111,6 +111,45 @@ template <> __device__ __forceinline__ Pcg32 path_seed<Pcg32>(uint32_t pix, uint
     return Pcg32{pcg_hash(pcg_hash(frame) ^ pix)};
 }
 
+// The camera ray of queue slot p of a batch in SampleOrder (PathTracer.cu:112-126): the pixel it
+// samples, frame frame0 + its sample index, and the generator state after launchRay's two draws.
+template <class Seed>
+__device__ __forceinline__ void camera_path(const cupbr_camera& cam, uint32_t w, uint32_t h, uint32_t frame0,
+                                            const SampleOrder& order, uint32_t p, Seed& seed, uint32_t& pix,
+                                            uint32_t& frame, v3& o, v3& d) {
+    uint32_t s, j, px, py;
+    order.sample(p, s, j);
+    tile_pixel(j, w, h, px, py);
+    pix = py * w + px;
+    frame = frame0 + s;
+    seed = path_seed<Seed>(pix, frame);
+    launch_ray(px, py, w, h, cam, seed, o, d);
+}
+
+// the per-batch counter reset that precedes bounce 0
+__device__ __forceinline__ void reset_batch_counters(Counters* c, uint32_t npath) {
+    c->queue_count[0].v = npath;
+    for (int g = 0; g < 8; ++g) c->work[g].v = c->swork[g].v = 0;
+    c->shadow_count[0].v = 0;
+    c->shadow_count[1].v = 0;
+    c->query_count.v = 0;
+    c->row_count.v = 0;
+    c->iqueue_count[0].v = 0;
+    c->iqueue_count[1].v = 0;
+}
+
+// A camera path's state as bounce 0 leaves it: direction and path id (queue 0), throughput 1 and
+// generator state, radiance 0 and flags (bit 0: radiance-cache training path).
+template <class Seed>
+__device__ __forceinline__ void store_camera_path(uint32_t p, v3 d, Seed seed, uint32_t pix, uint32_t frame,
+                                                  uint32_t train_thresh, float4* __restrict__ q_d,
+                                                  float4* __restrict__ weight, float4* __restrict__ rad) {
+    q_d[p] = make_float4(d.x, d.y, d.z, __uint_as_float(p));
+    weight[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(state_bits(seed)));
+    const uint32_t flags = is_training_path(pix, frame, train_thresh) ? 1u : 0u;
+    rad[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(flags));
+}
+
 // Camera rays of a batch in SampleOrder: the path id of a path is its queue slot here.
 template <class Seed>
 __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0,
@@ -118,30 +157,14 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_raygen(cupbr_camera cam, uint32
                                                         float4* __restrict__ q_d, float4* __restrict__ weight,
                                                         float4* __restrict__ rad, Counters* c, uint32_t train_thresh) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) {
-        c->queue_count[0].v = npath;
-        for (int g = 0; g < 8; ++g) c->work[g].v = c->swork[g].v = 0;
-        c->shadow_count[0].v = 0;
-        c->shadow_count[1].v = 0;
-        c->query_count.v = 0;
-        c->row_count.v = 0;
-        c->iqueue_count[0].v = 0;
-        c->iqueue_count[1].v = 0;
-    }
+    if (p == 0) reset_batch_counters(c, npath);
     if (p >= npath) return;  // p: queue slot and path id
-    uint32_t s, j, px, py;
-    order.sample(p, s, j);
-    tile_pixel(j, w, h, px, py);
-    const uint32_t pix = py * w + px;
-    const uint32_t pid = p;
-    Seed seed = path_seed<Seed>(pix, frame0 + s);
+    Seed seed;
+    uint32_t pix, frame;
     v3 o, d;
-    launch_ray(px, py, w, h, cam, seed, o, d);
+    camera_path(cam, w, h, frame0, order, p, seed, pix, frame, o, d);
     q_o[p] = make_float4(o.x, o.y, o.z, 0.0f);
-    q_d[p] = make_float4(d.x, d.y, d.z, __uint_as_float(pid));
-    weight[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(state_bits(seed)));
-    const uint32_t flags = is_training_path(pix, frame0 + s, train_thresh) ? 1u : 0u;
-    rad[pid] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(flags));
+    store_camera_path(p, d, seed, pix, frame, train_thresh, q_d, weight, rad);
 }
 
 // radiance-cache records written by shade: a query (inputs 16, throughput 3, path) ends a rendering
@@ -162,7 +185,8 @@ __device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bo
                                          const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
                                          float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
                                          float4* __restrict__ s_c, float4* __restrict__ weight,
-                                         float4* __restrict__ rad, Counters* c, const NrcShade& nrc) {
+                                         float4* __restrict__ rad, Counters* c, const NrcShade& nrc,
+                                         float4 origin0) {
     __shared__ uint32_t s_append[kPtWaves + 1];
     const uint32_t n = c->queue_count[cur].v;
     if (blockIdx.x == 0 && threadIdx.x < 8)  // the next trace starts at 0
@@ -174,7 +198,7 @@ __device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bo
         float4 no = {}, nd = {}, so = {}, sd = {}, sc = {};
         float4 rec[6] = {};  // cache query or pending row
         if (i < n) {
-            const float4 o4 = q_o[i], d4 = q_d[i];
+            const float4 o4 = q_o ? q_o[i] : origin0, d4 = q_d[i];
             const float2 hh = q_hit[i];
             const uint32_t p = __float_as_uint(d4.w);
             const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
@@ -295,9 +319,9 @@ void k_pt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int rr
                 const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
                 float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
                 float4* __restrict__ s_c, float4* __restrict__ weight, float4* __restrict__ rad, Counters* c,
-                NrcShade nrc) {
+                NrcShade nrc, float4 origin0) {
     pt_shade<false, Seed>(S, cur, bounce, max_depth, rr, q_o, q_d, q_hit, nq_o, nq_d, s_o, s_d, s_c, weight, rad, c,
-                          nrc);
+                          nrc, origin0);
 }
 
 // Shade with the radiance cache: queries and training rows (NrcShade)
@@ -307,9 +331,9 @@ void k_pt_shade_nrc(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, in
                     const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
                     float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
                     float4* __restrict__ s_c, float4* __restrict__ weight, float4* __restrict__ rad, Counters* c,
-                    NrcShade nrc) {
+                    NrcShade nrc, float4 origin0) {
     pt_shade<true, Seed>(S, cur, bounce, max_depth, rr, q_o, q_d, q_hit, nq_o, nq_d, s_o, s_d, s_c, weight, rad, c,
-                         nrc);
+                         nrc, origin0);
 }
 
 // query count after the second bounce's shade: the queries inferred on the side stream
@@ -533,6 +557,37 @@ void k_pt_trace_camera(DevScene S, uint32_t n, const float4* __restrict__ q_o, c
     }
     TraceResult r;
     traverse_packet(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), valid, r);
+    if (valid) q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
+}
+
+// Bounce 0 of a path-tracer batch with its ray generation: as k_pt_raygen followed by
+// k_pt_trace_camera, except that queue 0 holds no origins (every camera ray leaves the camera
+// position, which the first shade takes from its argument). A wave makes its 64 camera rays in
+// registers, writes their path state and traces them as one packet.
+template <class Seed>
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_pt_camera(DevScene S, uint32_t n, cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0, SampleOrder order,
+                 uint32_t train_thresh, float4* __restrict__ q_d, float4* __restrict__ weight,
+                 float4* __restrict__ rad, float2* __restrict__ q_hit, Counters* c) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        reset_batch_counters(c, n);
+        c->queue_count[1].v = 0;
+        c->extension_rays += n;
+    }
+    const uint32_t blk = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const uint32_t wave_first = blk * kPtBlock + (threadIdx.x & ~63u);
+    if (wave_first >= n) return;  // whole waves only: traverse_packet needs all 64 lanes
+    const uint32_t i = wave_first + __lane_id();
+    const bool valid = i < n;
+    v3 o = splat(0.0f), d = mk(0.0f, 0.0f, 1.0f);
+    if (valid) {
+        Seed seed;
+        uint32_t pix, frame;
+        camera_path(cam, w, h, frame0, order, i, seed, pix, frame, o, d);
+        store_camera_path(i, d, seed, pix, frame, train_thresh, q_d, weight, rad);
+    }
+    TraceResult r;
+    traverse_packet(S, o, d, valid, r);
     if (valid) q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
 }
 
@@ -1033,6 +1088,8 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
     auto shade = nrc ? (pcg ? k_pt_shade_nrc<Pcg32> : k_pt_shade_nrc<uint32_t>)
                      : (pcg ? k_pt_shade<Pcg32> : k_pt_shade<uint32_t>);
     auto raygen = pcg ? k_pt_raygen<Pcg32> : k_pt_raygen<uint32_t>;
+    auto camera = pcg ? k_pt_camera<Pcg32> : k_pt_camera<uint32_t>;
+    const float4 cam_pos = make_float4(cam.position[0], cam.position[1], cam.position[2], 0.0f);
     NrcShade ns{nullptr, nullptr, 0, __builtin_inff()};
     uint32_t train_thresh = 0;
     if (nrc) {
@@ -1057,18 +1114,22 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
         bool side_pending = false;  // cache inference of this batch running on the side stream
         const uint32_t npath = nb * npix;
         const SampleOrder order = sample_order(npix, nb);
-        hipLaunchKernelGGL(raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
-                           h, frame0 + s0, npath, order, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
-                           wf.weight.as<float4>(), wf.radiance.as<float4>(), c, train_thresh);
-        if ((rc = stage_mark(r, 3))) return rc;
+        // with camera packets, bounce 0 makes its own camera rays (k_pt_camera)
+        if (!packet) {
+            hipLaunchKernelGGL(raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
+                               h, frame0 + s0, npath, order, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
+                               wf.weight.as<float4>(), wf.radiance.as<float4>(), c, train_thresh);
+            if ((rc = stage_mark(r, 3))) return rc;
+        }
         // per bounce: trace (this bounce's extension rays + the previous bounce's shadow rays), then
         // shade (appends the next extension rays and this bounce's shadow rays, by bounce parity)
         for (uint32_t b = 0; b <= bounces; ++b) {
             const int cur = (int)(b & 1);
             if (b == 0 && packet) {
                 const uint32_t nblk = (npath + kPtBlock - 1) / kPtBlock;
-                hipLaunchKernelGGL(k_pt_trace_camera, dim3((nblk + 7u) / 8u * 8u), dim3(kPtBlock), 0, r->stream, S,
-                                   npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(), wf.q_hit.as<float2>(), c);
+                hipLaunchKernelGGL(camera, dim3((nblk + 7u) / 8u * 8u), dim3(kPtBlock), 0, r->stream, S, npath, cam, w,
+                                   h, frame0 + s0, order, train_thresh, wf.q_d[0].as<float4>(),
+                                   wf.weight.as<float4>(), wf.radiance.as<float4>(), wf.q_hit.as<float2>(), c);
             } else {
                 hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
                                    wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
@@ -1078,10 +1139,11 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
             if ((rc = stage_mark(r, 0))) return rc;
             if (b == bounces) break;  // the last trace only had the last bounce's shadow rays
             hipLaunchKernelGGL(shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
-                               (int)r->russian_roulette, wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),
-                               wf.q_hit.as<float2>(), wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(),
-                               wf.s_o[cur].as<float4>(), wf.s_d[cur].as<float4>(), wf.s_c[cur].as<float4>(),
-                               wf.weight.as<float4>(), wf.radiance.as<float4>(), c, ns);
+                               (int)r->russian_roulette, b == 0 && packet ? nullptr : wf.q_o[cur].as<float4>(),
+                               wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.q_o[cur ^ 1].as<float4>(),
+                               wf.q_d[cur ^ 1].as<float4>(), wf.s_o[cur].as<float4>(), wf.s_d[cur].as<float4>(),
+                               wf.s_c[cur].as<float4>(), wf.weight.as<float4>(), wf.radiance.as<float4>(), c, ns,
+                               cam_pos);
             if ((rc = stage_mark(r, 1))) return rc;
             if (nrc && b == 1 && bounces > 2) {
                 // The paths that queried the cache so far have no shadow ray left in flight (the
