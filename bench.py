@@ -247,7 +247,7 @@ def _run_counter_child(args, counters, log, copy=False):
 
 
 def is_trace_kernel(k):
-    return "k_pt_trace_camera" in k or "k_pt_trace<false>" in k
+    return "k_pt_camera<" in k or "k_pt_trace<false>" in k
 
 
 def is_render_kernel(k):
@@ -466,7 +466,7 @@ def bench_pt(args, log):
 
     roof = {"bound": "valu", "achieved": None, "peak": round(VALU_PEAK_TOPS, 3), "unit": "TOP/s", "frac": None,
             "traffic": None,
-            "kernel": "k_pt_trace_camera + k_pt_trace<false> (all trace launches of a step: bounce 0 camera rays as "
+            "kernel": "k_pt_camera + k_pt_trace<false> (all trace launches of a step: bounce 0 camera rays made and traced as "
                       "wave packets, then per bounce extension closest hit + previous bounce's shadow any hit, BVH4)",
             "op": "one VALU instruction on one active lane",
             "avg_launch_ms": round(trace_launch_ms, 4), "trace_ms_per_step": round(trace_ms_per_step, 3),

@@ -186,7 +186,7 @@ __device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bo
                                          float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
                                          float4* __restrict__ s_c, float4* __restrict__ weight,
                                          float4* __restrict__ rad, Counters* c, const NrcShade& nrc,
-                                         float4 origin0) {
+                                         uint32_t omask) {
     __shared__ uint32_t s_append[kPtWaves + 1];
     const uint32_t n = c->queue_count[cur].v;
     if (blockIdx.x == 0 && threadIdx.x < 8)  // the next trace starts at 0
@@ -198,7 +198,7 @@ __device__ __forceinline__ void pt_shade(const DevScene& S, int cur, uint32_t bo
         float4 no = {}, nd = {}, so = {}, sd = {}, sc = {};
         float4 rec[6] = {};  // cache query or pending row
         if (i < n) {
-            const float4 o4 = q_o ? q_o[i] : origin0, d4 = q_d[i];
+            const float4 o4 = q_o[i & omask], d4 = q_d[i];
             const float2 hh = q_hit[i];
             const uint32_t p = __float_as_uint(d4.w);
             const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
@@ -319,9 +319,9 @@ void k_pt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int rr
                 const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
                 float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
                 float4* __restrict__ s_c, float4* __restrict__ weight, float4* __restrict__ rad, Counters* c,
-                NrcShade nrc, float4 origin0) {
+                NrcShade nrc, uint32_t omask) {
     pt_shade<false, Seed>(S, cur, bounce, max_depth, rr, q_o, q_d, q_hit, nq_o, nq_d, s_o, s_d, s_c, weight, rad, c,
-                          nrc, origin0);
+                          nrc, omask);
 }
 
 // Shade with the radiance cache: queries and training rows (NrcShade)
@@ -331,9 +331,9 @@ void k_pt_shade_nrc(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, in
                     const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
                     float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
                     float4* __restrict__ s_c, float4* __restrict__ weight, float4* __restrict__ rad, Counters* c,
-                    NrcShade nrc, float4 origin0) {
+                    NrcShade nrc, uint32_t omask) {
     pt_shade<true, Seed>(S, cur, bounce, max_depth, rr, q_o, q_d, q_hit, nq_o, nq_d, s_o, s_d, s_c, weight, rad, c,
-                         nrc, origin0);
+                         nrc, omask);
 }
 
 // query count after the second bounce's shade: the queries inferred on the side stream
@@ -530,70 +530,54 @@ void k_pt_trace(DevScene S, int cur, const float4* __restrict__ q_o, const float
     }
 }
 
-// Bounce 0 (PathTracer.cu:112-126): the camera rays, traced as packets. Queue slot chunks of 64
-// (SampleOrder: the samples of one pixel, or of a few neighbouring pixels, or an 8x8 pixel tile)
-// hold rays that leave the camera position in nearby directions, so each wave
-// runs traverse_packet over one chunk. Blocks are dispatched to the 8 XCDs round-robin; block b
-// takes chunk range (b % 8) * (gridDim.x / 8) + b / 8, so each XCD traces a contiguous eighth of
-// the queue (neighbouring tiles share its L2). gridDim.x is a multiple of 8; n = queue 0's length.
-__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
-void k_pt_trace_camera(DevScene S, uint32_t n, const float4* __restrict__ q_o, const float4* __restrict__ q_d,
-                       float2* __restrict__ q_hit, Counters* c) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {  // as k_pt_trace for cur = 0 (no shadow rays yet)
-        c->queue_count[1].v = 0;
-        c->iqueue_count[1].v = 0;
-        c->shadow_count[0].v = 0;
-        c->extension_rays += n;
-    }
-    const uint32_t blk = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-    const uint32_t wave_first = blk * kPtBlock + (threadIdx.x & ~63u);
-    if (wave_first >= n) return;  // whole waves only: traverse_packet needs all 64 lanes
-    const uint32_t i = wave_first + __lane_id();
-    const bool valid = i < n;
-    float4 o4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d4 = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
-    if (valid) {
-        o4 = q_o[i];
-        d4 = q_d[i];
-    }
-    TraceResult r;
-    traverse_packet(S, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), valid, r);
-    if (valid) q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
-}
-
-// Bounce 0 of a path-tracer batch with its ray generation: as k_pt_raygen followed by
-// k_pt_trace_camera, except that queue 0 holds no origins (every camera ray leaves the camera
-// position, which the first shade takes from its argument). A wave makes its 64 camera rays in
-// registers, writes their path state and traces them as one packet.
-template <class Seed>
-__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
-void k_pt_camera(DevScene S, uint32_t n, cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0, SampleOrder order,
-                 uint32_t train_thresh, float4* __restrict__ q_d, float4* __restrict__ weight,
-                 float4* __restrict__ rad, float2* __restrict__ q_hit, Counters* c) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        reset_batch_counters(c, n);
-        c->queue_count[1].v = 0;
-        c->extension_rays += n;
-    }
+// Bounce 0 (PathTracer.cu:112-126): the camera rays, made where they are traced and traced as
+// packets. Queue slot chunks of 64 (SampleOrder: the samples of one pixel, or of a few neighbouring
+// pixels, or an 8x8 pixel tile; GDPT: the five paths of about 13 neighbouring pixels) hold rays that
+// leave the camera position in nearby directions, so each wave runs traverse_packet over one chunk.
+// Blocks are dispatched to the 8 XCDs round-robin; block b takes chunk range
+// (b % 8) * (gridDim.x / 8) + b / 8, so each XCD traces a contiguous eighth of the queue
+// (neighbouring tiles share its L2). gridDim.x is a multiple of 8; n = queue 0's length.
+// make(i, o, d) makes slot i's ray and writes the path's state. Queue 0 holds one origin, in slot 0:
+// every camera ray leaves the camera position, and the first shade reads it for all (omask = 0).
+template <class Make>
+__device__ __forceinline__ void camera_packets(const DevScene& S, uint32_t n, float4* __restrict__ q_o,
+                                               float2* __restrict__ q_hit, Make make) {
     const uint32_t blk = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     const uint32_t wave_first = blk * kPtBlock + (threadIdx.x & ~63u);
     if (wave_first >= n) return;  // whole waves only: traverse_packet needs all 64 lanes
     const uint32_t i = wave_first + __lane_id();
     const bool valid = i < n;
     v3 o = splat(0.0f), d = mk(0.0f, 0.0f, 1.0f);
-    if (valid) {
-        Seed seed;
-        uint32_t pix, frame;
-        camera_path(cam, w, h, frame0, order, i, seed, pix, frame, o, d);
-        store_camera_path(i, d, seed, pix, frame, train_thresh, q_d, weight, rad);
-    }
+    if (valid) make(i, o, d);
+    if (i == 0u) q_o[0] = make_float4(o.x, o.y, o.z, 0.0f);
     TraceResult r;
     traverse_packet(S, o, d, valid, r);
     if (valid) q_hit[i] = make_float2(r.t, __uint_as_float(r.prim));
 }
 
-// The running mean of PathTracer.cu:162-168 per pixel, frame by frame in order. A pixel's samples are runs of consecutive
-// slots (SampleOrder: up to f frames); a thread loads up to 8 of them at once, so that the
-// 128-byte line they share is fetched once.
+// bounce 0 of a path-tracer or volume-renderer batch: the rays of k_pt_raygen, traced at once;
+// it also resets the batch's counters
+template <class Seed>
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_pt_camera(DevScene S, uint32_t n, cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0, SampleOrder order,
+                 uint32_t train_thresh, float4* __restrict__ q_o, float4* __restrict__ q_d,
+                 float4* __restrict__ weight, float4* __restrict__ rad, float2* __restrict__ q_hit, Counters* c) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        reset_batch_counters(c, n);
+        c->queue_count[1].v = 0;
+        c->extension_rays += n;
+    }
+    camera_packets(S, n, q_o, q_hit, [&](uint32_t i, v3& o, v3& d) {
+        Seed seed;
+        uint32_t pix, frame;
+        camera_path(cam, w, h, frame0, order, i, seed, pix, frame, o, d);
+        store_camera_path(i, d, seed, pix, frame, train_thresh, q_d, weight, rad);
+    });
+}
+
+// The running mean of PathTracer.cu:162-168 per pixel, frame by frame in order. A pixel's samples
+// are runs of consecutive slots (SampleOrder: up to f frames); a thread loads up to 8 of them at
+// once, so that the 128-byte line they share is fetched once.
 __global__ __launch_bounds__(kPtBlock) void k_accumulate(float* __restrict__ img, const float4* __restrict__ rad,
                                                          uint32_t w, uint32_t h, SampleOrder order, uint32_t frame0,
                                                          uint32_t nspp) {
@@ -640,20 +624,10 @@ __global__ __launch_bounds__(kPtBlock) void k_accumulate(float* __restrict__ img
 // writes. Queue slots of a frame's groups follow the 8x8 tiles of the interior pixels.
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
-__global__ __launch_bounds__(kPtBlock) void k_gdpt_raygen(cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0,
-                                                          uint32_t n_int, uint32_t n_groups, float4* __restrict__ q_o,
-                                                          float4* __restrict__ q_d, float4* __restrict__ weight,
-                                                          float4* __restrict__ rad, uint32_t* __restrict__ slot0,
-                                                          Counters* c) {
-    // one thread per path, so that the queue and state writes of a wave are contiguous
-    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pid == 0) {
-        c->queue_count[0].v = 5u * n_groups;
-        for (int k = 0; k < 8; ++k) c->work[k].v = c->swork[k].v = 0;
-        c->shadow_count[0].v = 0;
-        c->shadow_count[1].v = 0;
-    }
-    if (pid >= 5u * n_groups) return;
+// The camera ray of GDPT path pid = 5 g + k (g = frame s of interior pixel gi; k: base, left,
+// right, up, down), and its generator state after the ray's draws (GradientDomain.cu:180-213).
+__device__ __forceinline__ void gdpt_camera_path(const cupbr_camera& cam, uint32_t w, uint32_t h, uint32_t frame0,
+                                                 uint32_t n_int, uint32_t pid, uint32_t& chain, v3& o, v3& d) {
     const uint32_t g = pid / 5u, k = pid - 5u * g;
     const uint32_t s = g / n_int, gi = g - s * n_int;
     uint32_t ix, iy;
@@ -662,17 +636,60 @@ __global__ __launch_bounds__(kPtBlock) void k_gdpt_raygen(cupbr_camera cam, uint
     // the five camera rays of a group come from one chain, base, left, right, up, down, two draws
     // each; each path continues from the chain as it stood after its own ray (:180-213). Path k
     // skips the 2k draws of the rays before its own.
-    uint32_t chain = tea4(py * w + px, frame0 + s);
+    chain = tea4(py * w + px, frame0 + s);
     for (uint32_t j = 0; j < 2u * k; ++j) chain = 1664525u * chain + 1013904223u;  // rnd's state step
     const uint32_t nx = k == 1 ? px - 1u : (k == 2 ? px + 1u : px);
     const uint32_t ny = k == 3 ? py + 1u : (k == 4 ? py - 1u : py);
-    v3 o, d;
     launch_ray(nx, ny, w, h, cam, chain, o, d);
+}
+
+// the GDPT batch's counter reset that precedes bounce 0
+__device__ __forceinline__ void reset_gdpt_counters(Counters* c, uint32_t npath) {
+    c->queue_count[0].v = npath;
+    for (int k = 0; k < 8; ++k) c->work[k].v = c->swork[k].v = 0;
+    c->shadow_count[0].v = 0;
+    c->shadow_count[1].v = 0;
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_gdpt_raygen(cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0,
+                                                          uint32_t n_int, uint32_t n_groups, float4* __restrict__ q_o,
+                                                          float4* __restrict__ q_d, float4* __restrict__ weight,
+                                                          float4* __restrict__ rad, uint32_t* __restrict__ slot0,
+                                                          Counters* c) {
+    // one thread per path, so that the queue and state writes of a wave are contiguous
+    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid == 0) reset_gdpt_counters(c, 5u * n_groups);
+    if (pid >= 5u * n_groups) return;
+    uint32_t chain;
+    v3 o, d;
+    gdpt_camera_path(cam, w, h, frame0, n_int, pid, chain, o, d);
     q_o[pid] = make_float4(o.x, o.y, o.z, 0.0f);
     q_d[pid] = make_float4(d.x, d.y, d.z, __uint_as_float(pid));
     weight[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(chain));
     rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (k == 0) slot0[g] = pid;
+    if (pid % 5u == 0u) slot0[pid / 5u] = pid;
+}
+
+// bounce 0 of a GDPT batch: the rays of k_gdpt_raygen, traced at once (camera_packets); it also
+// resets the batch's counters. n = 5 x the batch's groups.
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_gdpt_camera(DevScene S, uint32_t n, cupbr_camera cam, uint32_t w, uint32_t h, uint32_t frame0, uint32_t n_int,
+                   float4* __restrict__ q_o, float4* __restrict__ q_d, float4* __restrict__ weight,
+                   float4* __restrict__ rad, uint32_t* __restrict__ slot0, float2* __restrict__ q_hit, Counters* c) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        reset_gdpt_counters(c, n);
+        c->queue_count[1].v = 0;
+        c->iqueue_count[1].v = 0;
+        c->extension_rays += n;
+    }
+    camera_packets(S, n, q_o, q_hit, [&](uint32_t i, v3& o, v3& d) {
+        uint32_t chain;
+        gdpt_camera_path(cam, w, h, frame0, n_int, i, chain, o, d);
+        q_d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));
+        weight[i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(chain));
+        rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i % 5u == 0u) slot0[i / 5u] = i;
+    });
 }
 
 // one bounce of collect_radiance (:121-156) for every queued path
@@ -681,7 +698,7 @@ void k_gdpt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, cons
                   const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
                   float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
                   float4* __restrict__ s_c, float4* __restrict__ weight, float4* __restrict__ rad, Counters* c,
-                  const uint32_t* __restrict__ slot_now, uint32_t* __restrict__ slot_next) {
+                  const uint32_t* __restrict__ slot_now, uint32_t* __restrict__ slot_next, uint32_t omask) {
     __shared__ uint32_t s_append[kPtWaves + 1];
     const uint32_t n = c->queue_count[cur].v;
     if (blockIdx.x == 0 && threadIdx.x < 8) c->work[threadIdx.x].v = c->swork[threadIdx.x].v = 0;
@@ -691,7 +708,7 @@ void k_gdpt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, cons
         bool want_next = false, want_shadow = false;
         float4 no = {}, nd = {}, so = {}, sd = {}, sc = {};
         if (i < n) {
-            const float4 o4 = q_o[i], d4 = q_d[i];
+            const float4 o4 = q_o[i & omask], d4 = q_d[i];
             const float2 hh = q_hit[i];
             const uint32_t p = __float_as_uint(d4.w);
             const uint32_t g = p / 5u, k = p - 5u * g;
@@ -825,7 +842,7 @@ void k_vol_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int r
                  const float4* __restrict__ iq_d, float4* __restrict__ nq_o, float4* __restrict__ nq_d,
                  float4* __restrict__ niq_o, float4* __restrict__ niq_d, float4* __restrict__ s_o,
                  float4* __restrict__ s_d, float4* __restrict__ s_c, float4* __restrict__ weight,
-                 float4* __restrict__ rad, Counters* c) {
+                 float4* __restrict__ rad, Counters* c, uint32_t omask) {
     __shared__ uint32_t s_append[kPtWaves + 1];
     const uint32_t n_out = c->queue_count[cur].v, n_in = c->iqueue_count[cur].v, n = n_out + n_in;
     if (blockIdx.x == 0 && threadIdx.x < 8) c->work[threadIdx.x].v = c->swork[threadIdx.x].v = 0;
@@ -837,7 +854,7 @@ void k_vol_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, int r
         float4 no = {}, nd = {}, so = {}, sd = {}, sc = {};
         if (i < n) {
             const bool from_inside = i >= n_out;
-            const float4 o4 = from_inside ? iq_o[i - n_out] : q_o[i];
+            const float4 o4 = from_inside ? iq_o[i - n_out] : q_o[i & omask];
             const float4 d4 = from_inside ? iq_d[i - n_out] : q_d[i];
             const uint32_t p = __float_as_uint(d4.w);
             const float4 W = weight[p];
@@ -1089,7 +1106,6 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
                      : (pcg ? k_pt_shade<Pcg32> : k_pt_shade<uint32_t>);
     auto raygen = pcg ? k_pt_raygen<Pcg32> : k_pt_raygen<uint32_t>;
     auto camera = pcg ? k_pt_camera<Pcg32> : k_pt_camera<uint32_t>;
-    const float4 cam_pos = make_float4(cam.position[0], cam.position[1], cam.position[2], 0.0f);
     NrcShade ns{nullptr, nullptr, 0, __builtin_inff()};
     uint32_t train_thresh = 0;
     if (nrc) {
@@ -1128,7 +1144,7 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
             if (b == 0 && packet) {
                 const uint32_t nblk = (npath + kPtBlock - 1) / kPtBlock;
                 hipLaunchKernelGGL(camera, dim3((nblk + 7u) / 8u * 8u), dim3(kPtBlock), 0, r->stream, S, npath, cam, w,
-                                   h, frame0 + s0, order, train_thresh, wf.q_d[0].as<float4>(),
+                                   h, frame0 + s0, order, train_thresh, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
                                    wf.weight.as<float4>(), wf.radiance.as<float4>(), wf.q_hit.as<float2>(), c);
             } else {
                 hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
@@ -1139,11 +1155,11 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
             if ((rc = stage_mark(r, 0))) return rc;
             if (b == bounces) break;  // the last trace only had the last bounce's shadow rays
             hipLaunchKernelGGL(shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
-                               (int)r->russian_roulette, b == 0 && packet ? nullptr : wf.q_o[cur].as<float4>(),
+                               (int)r->russian_roulette, wf.q_o[cur].as<float4>(),
                                wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.q_o[cur ^ 1].as<float4>(),
                                wf.q_d[cur ^ 1].as<float4>(), wf.s_o[cur].as<float4>(), wf.s_d[cur].as<float4>(),
                                wf.s_c[cur].as<float4>(), wf.weight.as<float4>(), wf.radiance.as<float4>(), c, ns,
-                               cam_pos);
+                               b == 0 && packet ? 0u : ~0u);
             if ((rc = stage_mark(r, 1))) return rc;
             if (nrc && b == 1 && bounces > 2) {
                 // The paths that queried the cache so far have no shadow ray left in flight (the
@@ -1239,17 +1255,22 @@ int volume_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t 
         const uint32_t npath = nb * npix;
         // the path tracer's ray generation (camera rays in SampleOrder), seed tea<4>(pixel, frame)
         // (VolumeRenderer.cu:229-234); rad[p].w = 0: outside every object
+        // (VolumeRenderer.cu:229-234), made inside the camera trace with camera packets
         const SampleOrder order = sample_order(npix, nb);
-        hipLaunchKernelGGL(k_pt_raygen<uint32_t>, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
-                           h, frame0 + s0, npath, order, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
-                           wf.weight.as<float4>(), wf.radiance.as<float4>(), c, 0u);
-        if ((rc = stage_mark(r, 3))) return rc;
+        if (!packet) {
+            hipLaunchKernelGGL(k_pt_raygen<uint32_t>, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0,
+                               r->stream, cam, w, h, frame0 + s0, npath, order, wf.q_o[0].as<float4>(),
+                               wf.q_d[0].as<float4>(), wf.weight.as<float4>(), wf.radiance.as<float4>(), c, 0u);
+            if ((rc = stage_mark(r, 3))) return rc;
+        }
         for (uint32_t b = 0; b <= bounces; ++b) {
             const int cur = (int)(b & 1);
             if (b == 0 && packet) {
                 const uint32_t nblk = (npath + kPtBlock - 1) / kPtBlock;
-                hipLaunchKernelGGL(k_pt_trace_camera, dim3((nblk + 7u) / 8u * 8u), dim3(kPtBlock), 0, r->stream, S,
-                                   npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(), wf.q_hit.as<float2>(), c);
+                hipLaunchKernelGGL(k_pt_camera<uint32_t>, dim3((nblk + 7u) / 8u * 8u), dim3(kPtBlock), 0, r->stream, S,
+                                   npath, cam, w, h, frame0 + s0, order, 0u, wf.q_o[0].as<float4>(),
+                                   wf.q_d[0].as<float4>(), wf.weight.as<float4>(), wf.radiance.as<float4>(),
+                                   wf.q_hit.as<float2>(), c);
             } else {
                 hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
                                    wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
@@ -1259,12 +1280,13 @@ int volume_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t 
             if ((rc = stage_mark(r, 0))) return rc;
             if (b == bounces) break;
             hipLaunchKernelGGL(k_vol_shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
-                               (int)r->russian_roulette, wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),
+                               (int)r->russian_roulette, wf.q_o[cur].as<float4>(),
+                               wf.q_d[cur].as<float4>(),
                                wf.q_hit.as<float2>(), wf.iq_o[cur].as<float4>(), wf.iq_d[cur].as<float4>(),
                                wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(),
                                wf.iq_o[cur ^ 1].as<float4>(), wf.iq_d[cur ^ 1].as<float4>(), wf.s_o[cur].as<float4>(),
                                wf.s_d[cur].as<float4>(), wf.s_c[cur].as<float4>(), wf.weight.as<float4>(),
-                               wf.radiance.as<float4>(), c);
+                               wf.radiance.as<float4>(), c, b == 0 && packet ? 0u : ~0u);
             if ((rc = stage_mark(r, 1))) return rc;
         }
         hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, img,
@@ -1311,16 +1333,20 @@ int gdpt_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t fr
     for (uint32_t s0 = 0; s0 < spp; s0 += batch) {
         const uint32_t nb = std::min(batch, spp - s0);
         const uint32_t groups = nb * n_int, npath = 5u * groups;
-        hipLaunchKernelGGL(k_gdpt_raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream, cam, w,
-                           h, frame0 + s0, n_int, groups, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
-                           wf.weight.as<float4>(), wf.radiance.as<float4>(), slots[0], c);
-        if ((rc = stage_mark(r, 3))) return rc;
+        if (!packet) {  // with camera packets the camera trace makes the rays (k_gdpt_camera)
+            hipLaunchKernelGGL(k_gdpt_raygen, dim3((npath + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream,
+                               cam, w, h, frame0 + s0, n_int, groups, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
+                               wf.weight.as<float4>(), wf.radiance.as<float4>(), slots[0], c);
+            if ((rc = stage_mark(r, 3))) return rc;
+        }
         for (uint32_t b = 0; b <= bounces; ++b) {
             const int cur = (int)(b & 1);
             if (b == 0 && packet) {
                 const uint32_t nblk = (npath + kPtBlock - 1) / kPtBlock;
-                hipLaunchKernelGGL(k_pt_trace_camera, dim3((nblk + 7u) / 8u * 8u), dim3(kPtBlock), 0, r->stream, S,
-                                   npath, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(), wf.q_hit.as<float2>(), c);
+                hipLaunchKernelGGL(k_gdpt_camera, dim3((nblk + 7u) / 8u * 8u), dim3(kPtBlock), 0, r->stream, S, npath,
+                                   cam, w, h, frame0 + s0, n_int, wf.q_o[0].as<float4>(), wf.q_d[0].as<float4>(),
+                                   wf.weight.as<float4>(),
+                                   wf.radiance.as<float4>(), slots[0], wf.q_hit.as<float2>(), c);
             } else {
                 hipLaunchKernelGGL(trace, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, wf.q_o[cur].as<float4>(),
                                    wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(), wf.s_o[cur ^ 1].as<float4>(),
@@ -1330,10 +1356,11 @@ int gdpt_paths_wavefront(cupbr_renderer* r, const cupbr_camera& cam, uint32_t fr
             if ((rc = stage_mark(r, 0))) return rc;
             if (b == bounces) break;
             hipLaunchKernelGGL(k_gdpt_shade, dim3(blocks), dim3(kPtBlock), 0, r->stream, S, cur, b, r->max_trace_depth,
-                               wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(), wf.q_hit.as<float2>(),
-                               wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(), wf.s_o[cur].as<float4>(),
-                               wf.s_d[cur].as<float4>(), wf.s_c[cur].as<float4>(), wf.weight.as<float4>(),
-                               wf.radiance.as<float4>(), c, slots[b & 1], slots[(b + 1) & 1]);
+                               wf.q_o[cur].as<float4>(), wf.q_d[cur].as<float4>(),
+                               wf.q_hit.as<float2>(), wf.q_o[cur ^ 1].as<float4>(), wf.q_d[cur ^ 1].as<float4>(),
+                               wf.s_o[cur].as<float4>(), wf.s_d[cur].as<float4>(), wf.s_c[cur].as<float4>(),
+                               wf.weight.as<float4>(), wf.radiance.as<float4>(), c, slots[b & 1],
+                               slots[(b + 1) & 1], b == 0 && packet ? 0u : ~0u);
             if ((rc = stage_mark(r, 1))) return rc;
         }
         hipLaunchKernelGGL(k_gdpt_accumulate, dim3((n_int + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, r->stream,
