@@ -321,10 +321,10 @@ __global__ __launch_bounds__(kNrcBlock) void k_nrc_forward_backward(const bf16x8
 
 // weight gradients of one layer over one slice of the batch: dW[o][i] = sum_q G[o][q] X[i][q],
 // written (not accumulated) to partial[slice][param offset]
-template <int kOutTiles>  // 4: hidden layers W1..W4 (blockIdx.y = layer); 1: output layer W5
-__global__ __launch_bounds__(64) void k_nrc_weight_grad(const __bf16* __restrict__ X, const __bf16* __restrict__ G,
-                                                        uint32_t B, uint32_t slice, float* __restrict__ partial) {
-    const int layer = kOutTiles == 1 ? 4 : (int)blockIdx.y;
+template <int kOutTiles>  // 4: hidden layers W1..W4; 1: output layer W5
+__device__ __forceinline__ void weight_grad_layer(int layer, const __bf16* __restrict__ X,
+                                                  const __bf16* __restrict__ G, uint32_t B, uint32_t slice,
+                                                  float* __restrict__ partial) {
     const uint32_t s = blockIdx.x;          // slice index
     const int lane = threadIdx.x, g = lane >> 4;
     constexpr int n_ot = kOutTiles;
@@ -358,10 +358,21 @@ __global__ __launch_bounds__(64) void k_nrc_weight_grad(const __bf16* __restrict
             }
 }
 
+// all five layers in one launch: blockIdx.x = slice, blockIdx.y = layer (4: the output layer)
+__global__ __launch_bounds__(64) void k_nrc_weight_grad(const __bf16* __restrict__ X, const __bf16* __restrict__ G,
+                                                        uint32_t B, uint32_t slice, float* __restrict__ partial) {
+    if (blockIdx.y < 4)
+        weight_grad_layer<4>((int)blockIdx.y, X, G, B, slice, partial);
+    else
+        weight_grad_layer<1>(4, X, G, B, slice, partial);
+}
+
+__device__ __forceinline__ void pack_weight(uint32_t p, float w, __bf16* __restrict__ pf, __bf16* __restrict__ pb);
+
 // sum of the slice partials, Adam, and the packed BF16 copies for the next forward/backward
 __global__ void k_nrc_adam(const float* __restrict__ partial, uint32_t n_slices, float inv_b, float* __restrict__ w,
                            float* __restrict__ m, float* __restrict__ v, float lr, float b1, float b2, float eps,
-                           float bc1, float bc2) {
+                           float bc1, float bc2, __bf16* __restrict__ pf, __bf16* __restrict__ pb) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kNrcParams) return;
     float gsum = 0.0f;
@@ -371,17 +382,18 @@ __global__ void k_nrc_adam(const float* __restrict__ partial, uint32_t n_slices,
     const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
     m[p] = mm;
     v[p] = vv;
-    w[p] = w[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    const float wn = w[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    w[p] = wn;
+    pack_weight(p, wn, pf, pb);
 }
 
-// fragment positions of weight W_L[o][i] (see the file comment)
-__global__ void k_nrc_pack(const float* __restrict__ w, __bf16* __restrict__ pf, __bf16* __restrict__ pb) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kNrcParams) return;
+// the BF16 copies of parameter p = weight W_L[o][i] at its fragment positions (see the file
+// comment) in the forward and backward layouts
+__device__ __forceinline__ void pack_weight(uint32_t p, float w, __bf16* __restrict__ pf, __bf16* __restrict__ pb) {
     const int L = p < 4 * 4096 ? (int)(p / 4096) : 4;
     const int local = L < 4 ? (int)(p % 4096) : (int)(p - 4 * 4096);
     const int o = local / 64, i = local % 64;
-    const __bf16 val = (__bf16)w[p];
+    const __bf16 val = (__bf16)w;
     // forward: A[o][k] with k -> input i
     {
         const int t = i / 16, g = (i % 16) / 4, vv = i % 4, kc = t / 2, j = (t % 2) * 4 + vv;
@@ -400,6 +412,11 @@ __global__ void k_nrc_pack(const float* __restrict__ w, __bf16* __restrict__ pf,
             pb[3 * 4096 + (it * 64 + lane) * 8 + j] = val;
         }
     }
+}
+
+__global__ void k_nrc_pack(const float* __restrict__ w, __bf16* __restrict__ pf, __bf16* __restrict__ pb) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < kNrcParams) pack_weight(p, w[p], pf, pb);
 }
 
 // D = A * B through one matrix-core instruction with the fragment layout above (A 16x32, B 32x16,
@@ -539,19 +556,15 @@ int nrc_train(cupbr_nrc* c, const float* rows, uint32_t n_rows, uint32_t batch, 
                            c->packed_bwd.as<bf16x8>(), c->box, rows, n_rows, batch, seed, X, G,
                            loss_dev ? loss_dev + s : nullptr);
         CUPBR_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_nrc_weight_grad<4>, dim3(n_slices, 4), dim3(64), 0, stream, X, G, batch, slice,
-                           c->partial.as<float>());
-        CUPBR_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_nrc_weight_grad<1>, dim3(n_slices, 1), dim3(64), 0, stream, X, G, batch, slice,
+        hipLaunchKernelGGL(k_nrc_weight_grad, dim3(n_slices, 5), dim3(64), 0, stream, X, G, batch, slice,
                            c->partial.as<float>());
         CUPBR_HIP_CHECK(hipGetLastError());
         const float b1 = c->beta1, b2 = c->beta2;
         const float bc1 = 1.0f - std::pow(b1, (float)c->step), bc2 = 1.0f - std::pow(b2, (float)c->step);
         hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcParams + 255) / 256), dim3(256), 0, stream, c->partial.as<float>(),
                            n_slices, 1.0f / (float)batch, c->w.as<float>(), c->m.as<float>(), c->v.as<float>(), lr, b1,
-                           b2, c->eps, bc1, bc2);
+                           b2, c->eps, bc1, bc2, c->packed_fwd.as<__bf16>(), c->packed_bwd.as<__bf16>());
         CUPBR_HIP_CHECK(hipGetLastError());
-        if ((rc = nrc_pack(c, stream))) return rc;
     }
     return CUPBR_OK;
 }

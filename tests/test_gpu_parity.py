@@ -402,6 +402,39 @@ def test_gradient_domain_same_seed_parity():
     assert rel_mse(r.get_output_image(), recon) <= 1e-6
 
 
+@pytest.mark.parametrize("method", ["volume", "gdpt"])
+@pytest.mark.parametrize("w,h", [(67, 45), (64, 48)])
+def test_camera_kernels_equal_separate_ray_generation(method, w, h):
+    """With camera packets the volume renderer and the gradient domain make their camera rays in
+    the camera trace kernel, write the paths' state there and store one origin (queue slot 0) for
+    the first shade; the visit-counting mode writes every ray to the queue in a separate
+    ray-generation pass and traces it one per lane. Each ray comes from the same code either way
+    and the closest hit is a minimum over candidates, so every buffer must agree bit for bit (odd
+    sizes leave a partial last chunk; at 64x48 a wave holds 4 frames of 16 pixels)."""
+    if method == "volume":
+        desc, cam, m, depth = _volume_scene(), cb.Camera(w, h), cb.RenderingMethod.VOLUME, 6
+        names = [cb.Image.OUTPUT]
+    else:
+        desc, ca = scenes.cornell_box()
+        cam, m, depth = _camera(ca, w, h), cb.RenderingMethod.GRADIENTDOMAIN, 5
+        names = [cb.Image.BASE, cb.Image.GRADIENT_X_FORWARD, cb.Image.GRADIENT_X_BACKWARD,
+                 cb.Image.GRADIENT_Y_FORWARD, cb.Image.GRADIENT_Y_BACKWARD, cb.Image.OUTPUT]
+    scene = cb.Scene.from_desc(desc)
+    out = []
+    for counting in (False, True):
+        r = cb.PBRenderer(m)
+        r.set_output_size(w, h)
+        r.register_scene(scene)
+        r.set_method(m)
+        r.set_max_trace_depth(depth)
+        r.set_traversal_counting(counting)
+        r.render(cam, 4)
+        out.append([r.get_image(im) for im in names])
+    for im, a, b in zip(names, out[0], out[1]):
+        assert np.isfinite(a).all() and float(np.abs(a).mean()) > 0, im
+        assert np.array_equal(a, b), (im, float(np.abs(a - b).max()))
+
+
 @pytest.mark.parametrize("alpha,iters", [(0.2, 50), (0.5, 7)])
 def test_gradient_domain_screened_poisson_reconstruction(alpha, iters):
     """The screened-Poisson (L2) mode on the GPU's own five buffers equals the oracle's Jacobi
