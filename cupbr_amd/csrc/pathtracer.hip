@@ -692,8 +692,10 @@ void k_gdpt_camera(DevScene S, uint32_t n, cupbr_camera cam, uint32_t w, uint32_
     });
 }
 
-// one bounce of collect_radiance (:121-156) for every queued path
-__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+// one bounce of collect_radiance (:121-156) for every queued path. amdgpu_waves_per_eu(4, 8): 102
+// VGPRs and 4 waves per SIMD (Cornell box 512², 128 spp: 504 Msamples/s; 496 at 5 waves with 96
+// VGPRs, 478 at 6 waves with 80 VGPRs and a 24-byte spill).
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_gdpt_shade(DevScene S, int cur, uint32_t bounce, uint32_t max_depth, const float4* __restrict__ q_o,
                   const float4* __restrict__ q_d, const float2* __restrict__ q_hit, float4* __restrict__ nq_o,
                   float4* __restrict__ nq_d, float4* __restrict__ s_o, float4* __restrict__ s_d,
