@@ -1199,10 +1199,13 @@ int render_pathtracer(cupbr_renderer* r, const cupbr_camera& cam, uint32_t frame
                            wf.radiance.as<float4>(), w, h, order, frame0 + s0, nb);
         if ((rc = stage_mark(r, 3))) return rc;
         if (nrc) {
-            uint32_t counts[2] = {0, 0};
-            CUPBR_HIP_CHECK(hipMemcpyAsync(&counts[0], &c->query_count.v, 4, hipMemcpyDeviceToHost, r->stream));
-            CUPBR_HIP_CHECK(hipMemcpyAsync(&counts[1], &c->row_count.v, 4, hipMemcpyDeviceToHost, r->stream));
+            // the query and row counts in one copy (three consecutive counter lines)
+            static_assert(offsetof(Counters, row_count) == offsetof(Counters, query_count) + 2 * sizeof(CounterLine),
+                          "query_count, query_early and row_count are consecutive");
+            CounterLine lines[3];
+            CUPBR_HIP_CHECK(hipMemcpyAsync(lines, &c->query_count, sizeof(lines), hipMemcpyDeviceToHost, r->stream));
             CUPBR_HIP_CHECK(hipStreamSynchronize(r->stream));
+            const uint32_t counts[2] = {lines[0].v, lines[2].v};
             const uint32_t n_rows = std::min(counts[1], r->nrc_max_rows);
             r->nrc_queries_total += counts[0];
             r->nrc_rows_total += n_rows;
