@@ -25,10 +25,13 @@
 #include "traverse.h"
 
 // Lanes of the refill loop that must be at a leaf before a wave runs a leaf step (inner-node steps
-// otherwise): 16 of 64 measured best (1080p, 1M triangles: 8 / 16 / 24 / 32 -> 901 / 932 / 920 /
-// 843 Msamples/s; leaf and inner steps taken together every iteration: 889)
+// otherwise), at most: a wave runs its leaf step once half of its rays wait at a leaf, capped at
+// this many lanes. 32 (always half) measured best with the half rule (1080p, 1M triangles: cap 12
+// / 16 / 20 / 24 / 28 / 32 -> 1143 / 1155-1159 / 1163 / 1164-1168 / 1168 / 1168 Msamples/s). With
+// a fixed count instead (an earlier build): 8 / 16 / 24 / 32 -> 901 / 932 / 920 / 843; leaf and
+// inner steps taken together every iteration: 889.
 #ifndef CUPBR_LEAF_BATCH
-#define CUPBR_LEAF_BATCH 16
+#define CUPBR_LEAF_BATCH 32
 #endif
 // idle lanes of a wave that trigger a refill from the ray queue
 #ifndef CUPBR_REFILL_IDLE
@@ -419,14 +422,14 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
             if (!pool) break;
             continue;
         }
-        // Wave-uniform phases: a primitive step for the lanes at a leaf once at least
-        // CUPBR_LEAF_BATCH of them are there, or half of the wave's rays when it holds fewer than
-        // 2 CUPBR_LEAF_BATCH (the last rays of a launch), or all of them (no lane has an inner node
+        // Wave-uniform phases: a primitive step for the lanes at a leaf once half of the wave's rays
+        // are there (at most CUPBR_LEAF_BATCH lanes), or all of them (no lane has an inner node
         // left); an inner-node step for the others otherwise. A lane whose ray reached a leaf waits
         // during inner phases, so that the primitive test runs for many lanes at once instead of a
-        // few in every iteration (before: leaf code in 97% of iterations for 9 of 64 lanes). The
-        // half-of-the-rays rule keeps a nearly empty wave from holding its leaf lanes back until
-        // every other ray reaches a leaf (1080p, 1M triangles: 1147 -> 1157 Msamples/s).
+        // few in every iteration (before: leaf code in 97% of iterations for 9 of 64 lanes). Counted
+        // against the wave's own rays, the rule also keeps a nearly empty wave (the last rays of a
+        // launch) from holding its leaf lanes back until every other ray reaches a leaf (1080p, 1M
+        // triangles: a fixed 16 lanes 1147, half capped at 16 1155-1159, half 1168 Msamples/s).
         const bool at_leaf = active && (rs.cur & kLeafBit);
         const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
         const uint32_t n_act = (uint32_t)__popcll(am);
